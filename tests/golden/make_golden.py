@@ -1,0 +1,154 @@
+"""Writes the golden vectors in tests/golden/*.npz from the FALL3D-7.0
+reference routines themselves (oracle/_ref/libf3dref.so, built from
+/root/reference/Sources by `make -C oracle ref`).
+
+Run in the container that holds the reference tree:
+    python tests/golden/make_golden.py
+Each fixture holds the inputs and the reference outputs; tests/ replays the
+inputs through the oracle (and the GPU library) and demands equality.
+
+Every scenario runs in a fresh subprocess because the reference keeps its
+state in module variables, including the SAVEd vsettl drag coefficient
+(Sources/setvset.f90:89) that a second run in one process would inherit.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+from fall3d_amd import scenario as S  # noqa: E402
+from oracle.oracle import RefRun, REF_PATH, F, P  # noqa: E402
+
+# name -> (scenario kwargs, nsteps, zlayer(1) override)
+RUNS = {
+    "run_9x7x6x3_const_rams": (dict(seed=11), 20, None),
+    "run_9x7x6x3_cmaq_surface": (dict(seed=12, modkv=S.KV_SURFACE, modkh=S.KH_CMAQ), 20, 50.0),
+    "run_9x7x6x3_similarity": (dict(seed=13, modkv=S.KV_SIMILARITY), 3, 50.0),
+    "run_9x7x6x3_arastoopour": (dict(seed=14, modv=S.VS_ARASTOOPOUR), 5, None),
+    "run_9x7x6x3_wilson": (dict(seed=15, modv=S.VS_WILSON), 5, None),
+    "run_9x7x6x3_dellino": (dict(seed=16, modv=S.VS_DELLINO), 5, None),
+    "run_12x10x8x4_wind": (dict(nx=12, ny=10, nz=8, nc=4, seed=17, mean_u=6.0, mean_v=-5.0), 20, None),
+}
+
+
+def make_run(name):
+    kw, nsteps, z1 = RUNS[name]
+    sc = S.config_small(**kw)
+    if z1 is not None:
+        sc.zlayer[0] = z1
+        sc.rl[:] = 0.001
+    r = RefRun(sc)
+    r.meteo_update()
+    rec = dict(nx=sc.nx, ny=sc.ny, nz=sc.nz, nc=sc.nc, dx=sc.dx, dy=sc.dy, zlayer=sc.zlayer,
+               diam=sc.diam, rhop=sc.rhop, sphe=sc.sphe, vx_in=sc.vx, vy=sc.vy, vz=sc.vz,
+               tempe=sc.tempe, vptemp=sc.vptemp, rho_in=sc.rho, ustar=sc.ustar, zi=sc.zi,
+               rl=sc.rl, topg=sc.topg, vdep=sc.vdep, isrc=sc.isrc, jsrc=sc.jsrc, ksrc=sc.ksrc,
+               tmrat=sc.tmrat, flowrate=sc.flowrate, modkv=sc.modkv, modkh=sc.modkh,
+               modv=sc.modv, rkv0=sc.rkv0, rkh0=sc.rkh0, csh=sc.csh,
+               psi=r.psi, dt=r.dt, dtc=r.dtc, vx=r.vx, rho=r.rho, rkhor=r.rkhor, rkh1=r.rkh1,
+               rkver=r.rkver, vset=r.vset, nsteps=nsteps)
+    r.step()
+    c1, cdry1 = r.state()
+    rec["c_step1"], rec["cdry_step1"] = c1, cdry1
+    for _ in range(nsteps - 1):
+        r.step()
+    c, cdry = r.state()
+    rec["c_final"], rec["cdry_final"] = c, cdry
+    b = r.budget()
+    for k, v in b.items():
+        rec["budget_" + k] = v
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **rec)
+
+
+def make_routines():
+    """Per-routine vectors: each reference external called once on seeded
+    inputs with mixed-sign velocities (the serial decomposition)."""
+    L = RefRun.L()
+    rng = np.random.default_rng(2024)
+    nx, ny, nz = 9, 7, 6
+    L.ref_alloc(nx, ny, nz, 1, 1)  # sets up Parallel/Domain state for cmass2d
+    ci = ctypes.c_int
+    dd = ctypes.c_double
+
+    def fi(v):
+        return ctypes.byref(ci(v))
+
+    def fd(v):
+        return ctypes.byref(dd(v))
+
+    c0 = F(rng.uniform(0.0, 1.0, (nx + 2, ny + 2, nz + 2)) ** 3)
+    vx = F(rng.normal(0.0, 10.0, (nx, ny, nz)))
+    vy = F(rng.normal(0.0, 10.0, (nx, ny, nz)))
+    vz = F(rng.normal(0.0, 0.5, (nx, ny, nz)))
+    vset = F(-rng.uniform(0.0, 2.0, (nx, ny, nz)))
+    vdep = F(rng.uniform(0.0, 0.01, (nx, ny)))
+    rkver = F(rng.uniform(10.0, 200.0, (nx, ny, nz)))
+    rkh = F(rng.uniform(500.0, 5000.0, (nx, ny, nz)))
+    rho = F(rng.uniform(0.3, 1.2, (nx, ny, nz)))
+    zl = np.cumsum(rng.uniform(300.0, 900.0, nz)) - 300.0
+    dz = np.empty(nz + 1)
+    dz[1:nz] = np.diff(zl)
+    dz[0], dz[nz] = dz[1], dz[nz - 1]
+    dz = F(dz)
+    dx, dy, dt = 4000.0, 3000.0, 37.5
+    work = np.zeros(2 * max(nx + 1, ny + 1, nz + 1))
+    rec = dict(nx=nx, ny=ny, nz=nz, c0=c0, vx=vx, vy=vy, vz=vz, vset=vset, vdep=vdep,
+               rkver=rkver, rkh=rkh, rho=rho, dz=dz, dx=dx, dy=dy, dt=dt)
+    args = (fi(nx), fi(ny), fi(nz), fi(nz), fi(1))
+
+    c = c0.copy(order="F")
+    L.setbcc_(P(c), *args, P(vx), P(vy), P(vz), P(vset), P(vdep), P(rkver), P(dz))
+    rec["setbcc"] = c.copy(order="F")
+    cb = rec["setbcc"]  # sweeps start from a ghost-consistent field
+    for nm, call in [
+        ("advctzc", lambda c: L.advctzc_(P(c), *args, P(vz), P(vset), P(dz), fd(dt), fi(1))),
+        ("advctx", lambda c: L.advctx_(P(c), *args, P(vx), fd(dx), fd(dt), P(work), fi(1))),
+        ("advcty", lambda c: L.advcty_(P(c), *args, P(vy), fd(dy), fd(dt), P(work), fi(1))),
+        ("diffz", lambda c: L.diffz_(P(c), *args, P(dz), fd(dt), P(rkver), P(rho), P(work))),
+        ("diffx", lambda c: L.diffx_(P(c), *args, fd(dx), fd(dt), P(rkh), P(rho), P(work))),
+        ("diffy", lambda c: L.diffy_(P(c), *args, fd(dy), fd(dt), P(rkh), P(rho), P(work))),
+    ]:
+        c = cb.copy(order="F")
+        call(c)
+        rec[nm] = c
+    cdry = F(rng.uniform(0.0, 5.0, (nx, ny, 3)))
+    rec["cdry0"] = cdry.copy(order="F")
+    L.accum_(P(cb), P(cdry), fi(2), fi(nx), fi(ny), fi(nz), fi(nz), fi(1), P(vz), fd(dt),
+             P(vset), P(vdep))
+    rec["accum"] = cdry
+    # cmass2d / cmass3d on two classes
+    c2 = F(np.stack([cb, cb[::-1, :, :] * 0.5], axis=-1))
+    vset2 = F(np.stack([vset, vset * 0.3], axis=-1))
+    om = dd(1.25)
+    L.cmass2d_(P(c2), P(vx), P(vy), P(vz), P(vset2), fi(nx), fi(ny), fi(nz), fi(2), fi(1),
+               fi(2), fi(1), fi(nz), fd(dx), fd(dy), P(dz), fd(dt), ctypes.byref(om))
+    rec["c2"], rec["vset2"], rec["cmass2d_outmass"] = c2, vset2, om.value
+    cload = F(rng.uniform(0.0, 3.0, (nx, ny, 3)))
+    cdr = F(rng.uniform(0.0, 3.0, (nx, ny, 3)))
+    cwt = F(rng.uniform(0.0, 1.0, (nx, ny, 3)))
+    outs = [dd() for _ in range(4)]
+    L.cmass3d_(P(c2), P(cload), P(cdr), P(cwt), fi(nx), fi(ny), fi(nz), fi(2), fd(dx), fd(dy),
+               P(dz), *[ctypes.byref(o) for o in outs])
+    rec.update(cload=cload, cdr=cdr, cwt=cwt,
+               cmass3d=np.array([o.value for o in outs]))
+    np.savez_compressed(os.path.join(OUT, "routines_9x7x6.npz"), **rec)
+
+
+if __name__ == "__main__":
+    if not os.path.exists(REF_PATH):
+        sys.exit(f"{REF_PATH} missing: build it with `make -C oracle ref`")
+    if len(sys.argv) > 1:
+        which = sys.argv[1]
+        make_routines() if which == "routines" else make_run(which)
+    else:
+        for name in ["routines"] + list(RUNS):
+            subprocess.run([sys.executable, os.path.abspath(__file__), name], check=True)
+            print("wrote", name)

@@ -1,0 +1,30 @@
+"""Rebuild a Scenario from a golden fixture (tests/golden/*.npz)."""
+import glob
+import os
+
+import numpy as np
+
+from fall3d_amd import scenario as S
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def run_fixtures():
+    return sorted(glob.glob(os.path.join(GOLDEN, "run_*.npz")))
+
+
+def load(path):
+    return dict(np.load(path, allow_pickle=False))
+
+
+def scenario_from(g):
+    nx, ny, nz, nc = (int(g[k]) for k in ("nx", "ny", "nz", "nc"))
+    return S.Scenario(
+        name=os.path.basename(str(g.get("name", "golden"))), nx=nx, ny=ny, nz=nz, nc=nc,
+        dx=float(g["dx"]), dy=float(g["dy"]), zlayer=g["zlayer"], diam=g["diam"],
+        rhop=g["rhop"], sphe=g["sphe"], fc=np.ones(nc) / nc, vx=g["vx_in"], vy=g["vy"],
+        vz=g["vz"], tempe=g["tempe"], vptemp=g["vptemp"], rho=g["rho_in"], ustar=g["ustar"],
+        zi=g["zi"], rl=g["rl"], topg=g["topg"], vdep=g["vdep"], isrc=g["isrc"],
+        jsrc=g["jsrc"], ksrc=g["ksrc"], tmrat=g["tmrat"], flowrate=float(g["flowrate"]),
+        modkv=int(g["modkv"]), modkh=int(g["modkh"]), modv=int(g["modv"]),
+        rkv0=float(g["rkv0"]), rkh0=float(g["rkh0"]), csh=float(g["csh"]))
