@@ -1,0 +1,190 @@
+"""Benchmark of the FALL3D transport step on one MI355X.
+
+Workload: the precision-form scenario, 400x400x40 grid, 20 particle classes
+(2 um - 4 mm), Fourier-mode meteorology with sinusoidal terrain (synthetic,
+seeded).  A step is one pass of the transport core over all 20 classes:
+setsrc + sizloop (advection, diffusion, boundaries, deposit, clip, outflux).
+
+value     = cell*class updates per second of the FP32 fused path (inputs
+            resident in HBM, meteo stage excluded and reported separately)
+roofline  = the fused step kernel's algorithmic HBM bytes per launch over its
+            HIP-event average launch time, against 8 TB/s
+fp64_plain= the routine-by-routine FP64 drop-in on the same workload
+cpu_baseline = the oracle's FP64 C restatement (class-parallel threads) on
+            this host, on a bounded number of steps of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def alg_bytes_per_update(nc, n_share=1, s_c=4, s_v=4, s_m=4):
+    """Compulsory HBM bytes per cell*class update of one fused step: read and
+    write c, read vset of the class, read the 7 shared meteo/coefficient
+    fields (vx, vy, vz, 1/rho, rho*rkh1, rho*rkhor, rho*rkver) once per tile
+    for all nc classes."""
+    return 2 * s_c + (s_v + 7 * s_m / nc) / n_share
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=400)
+    ap.add_argument("--nz", type=int, default=40)
+    ap.add_argument("--nc", type=int, default=20)
+    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--tile", default="32x8")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the path shards)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from fall3d_amd import lib as F
+    from fall3d_amd import scenario as S
+
+    t0 = time.time()
+    sc = S.config_c2(nx=args.nx, ny=args.nx, nz=args.nz, nc=args.nc)
+    log(f"scenario {sc.name} built in {time.time() - t0:.1f}s")
+    upd = sc.updates_per_step
+
+    # ---------------- FP32 fused path ----------------
+    r = F.Run(sc, precision=F.FP32)
+    tx, ty = (int(v) for v in args.tile.split("x"))
+    r.ctx.set_tile(tx, ty)
+    t0 = time.time()
+    r.meteo_update()
+    r.ctx.sync()
+    meteo_s = time.time() - t0
+    log(f"meteo stage (kappa3+setvset+finddt, FP64 on device) {meteo_s:.3f}s dt={r.dt:.3f}s")
+    r.step(args.warmup)
+    r.ctx.sync()
+    r.ctx.timing(True)
+    r.ctx.kernel_time(reset=True)
+    r.ctx.sync()
+    t0 = time.perf_counter()
+    r.step(args.steps)
+    r.ctx.sync()
+    el = time.perf_counter() - t0
+    kms, kcount = r.ctx.kernel_time()
+    r.ctx.timing(False)
+    value = upd * args.steps / el
+    ms_per_step = el * 1e3 / args.steps
+    log(f"fp32 fused: {ms_per_step:.3f} ms/step, {value:.3e} updates/s, "
+        f"kernel avg {kms:.3f} ms over {kcount} launches")
+    b_alg = alg_bytes_per_update(sc.nc)
+    achieved = b_alg * upd / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    budget = r.budget()
+    fp32_state = None
+    r.close()
+
+    # ---------------- FP64 plain drop-in ----------------
+    fp64 = None
+    if not args.skip_fp64:
+        r64 = F.Run(sc, precision=F.FP64)
+        r64.meteo_update()
+        r64.step(2)
+        r64.ctx.sync()
+        t0 = time.perf_counter()
+        r64.step(args.fp64_steps)
+        r64.ctx.sync()
+        el64 = time.perf_counter() - t0
+        fp64 = upd * args.fp64_steps / el64
+        log(f"fp64 plain: {el64 * 1e3 / args.fp64_steps:.3f} ms/step, {fp64:.3e} updates/s")
+        coef = r64.ctx.coef() if not args.skip_cpu else None
+        r64.close()
+
+    # ---------------- CPU baseline: the oracle on this host ----------------
+    cpu = None
+    if not args.skip_cpu:
+        from oracle.oracle import OracleRun
+
+        ncores = len(os.sched_getaffinity(0))
+        nthreads = max(1, min(ncores, sc.nc, 16))
+        o = OracleRun(sc, nthreads=nthreads)
+        if coef is None:
+            r64 = F.Run(sc, precision=F.FP64)
+            r64.meteo_update()
+            coef = r64.ctx.coef()
+            r64.close()
+        # coefficients from the device meteo stage (the step phase is timed)
+        o.vx, o.rho, o.rkhor, o.rkh1, o.rkver, o.vset = (coef[k] for k in
+                                                          ("vx", "rho", "rkhor", "rkh1", "rkver", "vset"))
+        o.vy = np.asfortranarray(sc.vy)
+        o.vz = np.asfortranarray(sc.vz)
+        o.vdep = np.asfortranarray(sc.vdep)
+        o.dt = r_dt(sc, coef)
+        nsteps, tcpu = 0, 0.0
+        while tcpu < args.cpu_seconds and nsteps < 50:
+            t0 = time.perf_counter()
+            o.step()
+            tcpu += time.perf_counter() - t0
+            nsteps += 1
+            log(f"cpu oracle step {nsteps}: {tcpu:.1f}s")
+        cpu = dict(value=upd * nsteps / tcpu, unit="cell*class updates/s", cores=nthreads,
+                   kind="port",
+                   sample=f"{nsteps} steps of the {sc.name} workload (FP64 C restatement, "
+                          f"{nthreads} class-parallel threads, coefficients from the device "
+                          f"meteo stage)")
+
+    out = {
+        "metric": "cell·class updates/s + % HBM peak; ensemble p50/p99 forecast latency",
+        "value": value,
+        "unit": "cell*class updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 state, f64 deposit/budgets/settling/CFL",
+        "data": "synthetic (seeded Fourier-mode meteorology, Suzuki source)",
+        "config": {"workload": f"precision form {sc.nx}x{sc.ny}x{sc.nz} grid, {sc.nc} classes, "
+                               f"fused FP32 step", "nx": sc.nx, "ny": sc.ny, "nz": sc.nz,
+                   "nc": sc.nc, "tile": args.tile, "dt_s": None},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_update": b_alg, "kernel_avg_ms": kms, "launches": kcount},
+        "fp64_plain_updates_per_s": fp64,
+        "speedup_vs_fp64_plain": (value / fp64) if fp64 else None,
+        "meteo_stage_s": meteo_s,
+        "cpu_baseline": cpu,
+        "hbm_peak_frac_wall": value * b_alg / 1e9 / HBM_PEAK_GBS,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def r_dt(sc, coef):
+    """finddt.f90:26-61 on the coefficient arrays (host, for the CPU leg)."""
+    import ctypes
+    from oracle.oracle import lib as olib, P, F as FF
+    dtc = ctypes.c_double()
+    return olib().or_finddt(sc.nx, sc.ny, sc.nz, sc.nc, P(FF(sc.dz)), sc.dx, sc.dy, P(coef["vx"]),
+                            P(FF(sc.vy)), P(FF(sc.vz)), P(coef["rkh1"]), P(coef["rkhor"]),
+                            P(coef["rkver"]), P(coef["vset"]), ctypes.byref(dtc))
+
+
+if __name__ == "__main__":
+    main()

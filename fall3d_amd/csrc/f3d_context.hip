@@ -1,0 +1,622 @@
+// Device-resident context: the f3d_* half of the C ABI.
+//
+// The context mirrors the reference driver's state (Master.f90 arrays,
+// Numeric grid scalars, the Fall3d.f90 clock) in HBM.  f3d_step runs the
+// loop body of Fall3d.f90:112-117 on the device; nothing is copied to the
+// host during a step.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "f3d_ctx.h"
+
+using namespace f3d;
+
+namespace {
+
+int upload(f3d::DeviceBuffer &b, const void *h, size_t n, hipStream_t s)
+{
+    F3D_HIP(hipMemcpyAsync(b.p, h, n, hipMemcpyHostToDevice, s));
+    return F3D_OK;
+}
+
+#define TRY(x)                   \
+    do {                         \
+        int rc_ = (x);           \
+        if (rc_) return rc_;     \
+    } while (0)
+
+bool check_ctx(const f3d_ctx *c)
+{
+    if (!c) {
+        set_error("null context");
+        return false;
+    }
+    return true;
+}
+
+void record_start(f3d_ctx *c)
+{
+    if (!c->timing) return;
+    (void)hipEventRecord(c->ev_start[c->ev_next], c->stream);
+}
+
+void flush_timing(f3d_ctx *c)
+{
+    if (c->ev_next == 0) return;
+    (void)hipEventSynchronize(c->ev_stop[c->ev_next - 1]);
+    for (int i = 0; i < c->ev_next; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_start[i], c->ev_stop[i]) == hipSuccess) {
+            c->t_ms += ms;
+            c->t_count += 1;
+        }
+    }
+    c->ev_next = 0;
+}
+
+void record_stop(f3d_ctx *c)
+{
+    if (!c->timing) return;
+    (void)hipEventRecord(c->ev_stop[c->ev_next], c->stream);
+    c->ev_next += 1;
+    if (c->ev_next == (int)c->ev_stop.size()) flush_timing(c);
+}
+
+int step_fp64(f3d_ctx *c)
+{
+    const Grid &g = c->g;
+    const int nc = g.nc;
+    hipStream_t s = c->stream;
+    double *cc = c->d_c64.as<double>();
+    const double *vx = c->d_vx.as<double>(), *vy = c->d_vy.as<double>(), *vz = c->d_vz.as<double>();
+    const double *vset = c->d_vset.as<double>(), *vdep = c->d_vdep.as<double>();
+    const double *rkver = c->d_rkver.as<double>(), *rkhor = c->d_rkhor.as<double>();
+    const double *rkh1 = c->d_rkh1.as<double>(), *rho = c->d_rho.as<double>();
+    const double *dz = c->d_dz.as<double>();
+    const double dt = c->dt;
+    record_start(c);
+    // setsrc.f90:20-35
+    TRY(p64::setsrc(s, g, cc, c->nsou, c->d_isrc.as<int>(), c->d_jsrc.as<int>(),
+                    c->d_ksrc.as<int>(), c->d_tmrat.as<double>(), c->d_zlayer.as<double>(), dz,
+                    c->d_hm1.as<double>(), dt));
+    // sizloop.f90:31-73, routine by routine over all classes
+    TRY(p64::setbcc(s, g, nc, cc, vx, vy, vz, vset, vdep, rkver, dz));
+    TRY(p64::advctzc(s, g, nc, cc, vz, vset, c->d_zc.as<ZCoef>(), c->ldz[g.nz], dt));
+    if (c->iiter % 2 == 0) {
+        TRY(p64::advctx(s, g, nc, cc, vx, g.dx, dt));
+        TRY(p64::advcty(s, g, nc, cc, vy, g.dy, dt));
+    } else {
+        TRY(p64::advcty(s, g, nc, cc, vy, g.dy, dt));
+        TRY(p64::advctx(s, g, nc, cc, vx, g.dx, dt));
+    }
+    TRY(p64::diffz(s, g, nc, cc, c->d_zc.as<ZCoef>(), c->d_ldz.as<double>(), dt, rkver, rho));
+    if (c->iiter % 2 == 0) {
+        TRY(p64::diffx(s, g, nc, cc, g.dx, dt, rkh1, rho));
+        TRY(p64::diffy(s, g, nc, cc, g.dy, dt, rkhor, rho));
+    } else {
+        TRY(p64::diffy(s, g, nc, cc, g.dy, dt, rkhor, rho));
+        TRY(p64::diffx(s, g, nc, cc, g.dx, dt, rkh1, rho));
+    }
+    TRY(p64::accum(s, g, nc, cc, c->d_cdry.as<double>(), 1, dt, vset, vdep));
+    // sizloop.f90:77-83
+    TRY(p64::clip(s, cc, g.G * nc));
+    TRY(red::cmass2d64(s, g, nc, cc, vx, vy, vz, vset, dz, dt, c->d_scratch.as<double>(),
+                       c->d_outmass.as<double>()));
+    record_stop(c);
+    return F3D_OK;
+}
+
+int step_fp32(f3d_ctx *c)
+{
+    const Grid &g = c->g;
+    hipStream_t s = c->stream;
+    float *cin = c->d_c32[c->cur].as<float>();
+    float *cout = c->d_c32[1 - c->cur].as<float>();
+    const f32::Fields f = f32::fields_view(g, c->d_f32.as<float>(), c->d_vsum1.as<double>());
+    TRY(f32::setsrc(s, g, cin, c->nsou, c->d_isrc.as<int>(), c->d_jsrc.as<int>(),
+                    c->d_ksrc.as<int>(), c->d_tmrat.as<double>(), c->d_zlayer.as<double>(),
+                    c->d_dz.as<double>(), c->d_hm1.as<double>(), c->dt));
+    f32::StepArgs a{g, cin, cout, c->d_cdry.as<double>(), c->dt, c->iiter, c->d_zc.as<ZCoef>()};
+    record_start(c);
+    TRY(f32::fused_step(s, a, f, c->tile));
+    record_stop(c);
+    TRY(f32::frame_and_outflux(s, a, f, c->d_vx.as<double>(), c->d_vy.as<double>(),
+                               c->d_vz.as<double>(), c->d_vset.as<double>(), c->d_dz.as<double>(),
+                               c->d_scratch.as<double>(), c->d_outmass.as<double>()));
+    c->cur = 1 - c->cur;
+    return F3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int f3d_create(f3d_ctx **out, const f3d_grid *gr, int precision)
+{
+    if (!out || !gr) {
+        set_error("f3d_create: null argument");
+        return F3D_EINVAL;
+    }
+    *out = nullptr;
+    if (gr->nx < 2 || gr->ny < 2 || gr->nz < 2 || gr->nc < 1 || !gr->zlayer) {
+        set_error("f3d_create: need nx, ny, nz >= 2, nc >= 1 and zlayer (got %d %d %d %d)", gr->nx,
+                  gr->ny, gr->nz, gr->nc);
+        return F3D_EINVAL;
+    }
+    if (precision != F3D_FP64 && precision != F3D_FP32) {
+        set_error("f3d_create: unknown precision %d", precision);
+        return F3D_EINVAL;
+    }
+    if (precision == F3D_FP32 && gr->nz > 64) {
+        set_error("f3d_create: the FP32 fused path supports nz <= 64 (got %d)", gr->nz);
+        return F3D_EINVAL;
+    }
+    if (!(gr->dx > 0.0) || !(gr->dy > 0.0)) {
+        set_error("f3d_create: dx and dy must be positive");
+        return F3D_EINVAL;
+    }
+    f3d_ctx *c = new f3d_ctx();
+    c->precision = precision;
+    c->g = make_grid(gr->nx, gr->ny, gr->nz, gr->nc, gr->dx, gr->dy);
+    c->lonlat = gr->lonlat;
+    const Grid &g = c->g;
+    c->zlayer.assign(gr->zlayer, gr->zlayer + g.nz);
+    vertical_coefs(g.nz, gr->zlayer, c->dz, c->zc, c->ldz);
+    c->hm1.assign(g.NXY, 1.0);
+    if (gr->hm1) {
+        c->hm1.assign(gr->hm1, gr->hm1 + g.NXY);
+        for (double h : c->hm1)
+            if (h != 1.0) c->uniform_hm1 = false;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("f3d_create: hipStreamCreate failed");
+        delete c;
+        return F3D_EHIP;
+    }
+    long long *B = &c->bytes;
+    int rc = 0;
+    const size_t d8 = sizeof(double);
+    rc |= c->d_dz.alloc(d8 * (g.nz + 1), B);
+    rc |= c->d_ldz.alloc(d8 * (g.nz + 1), B);
+    rc |= c->d_zlayer.alloc(d8 * g.nz, B);
+    rc |= c->d_hm1.alloc(d8 * g.NXY, B);
+    rc |= c->d_kmh.alloc(d8 * g.NXY, B);
+    rc |= c->d_zc.alloc(sizeof(ZCoef) * (g.nz + 2), B);
+    for (DeviceBuffer *b : {&c->d_vx, &c->d_vy, &c->d_vz, &c->d_tempe, &c->d_vptemp, &c->d_rho,
+                            &c->d_rkhor, &c->d_rkh1, &c->d_rkver, &c->d_visc})
+        rc |= b->alloc(d8 * g.N, B);
+    for (DeviceBuffer *b : {&c->d_ustar, &c->d_zi, &c->d_rl}) rc |= b->alloc(d8 * g.NXY, B);
+    rc |= c->d_vdep.alloc(d8 * g.NXY * g.nc, B);
+    rc |= c->d_vset.alloc(d8 * g.N * g.nc, B);
+    rc |= c->d_cls.alloc(sizeof(met::ClassConst) * g.nc, B);
+    rc |= c->d_cd.alloc(d8 * 2, B);
+    rc |= c->d_status.alloc(sizeof(int) * 2, B);
+    rc |= c->d_dtcn.alloc(sizeof(unsigned long long), B);
+    rc |= c->d_cdry.alloc(d8 * g.NXY * (g.nc + 1), B);
+    rc |= c->d_cwet.alloc(d8 * g.NXY * (g.nc + 1), B);
+    rc |= c->d_scratch.alloc(d8 * 4 * red::kPartials, B);
+    rc |= c->d_outmass.alloc(d8, B);
+    rc |= c->d_budget.alloc(d8 * 4, B);
+    if (precision == F3D_FP64) {
+        rc |= c->d_c64.alloc(d8 * g.G * g.nc, B);
+    } else {
+        rc |= c->d_c32[0].alloc(sizeof(float) * g.G * g.nc, B);
+        rc |= c->d_c32[1].alloc(sizeof(float) * g.G * g.nc, B);
+        rc |= c->d_f32.alloc(sizeof(float) * f32::fields_floats(g), B);
+        rc |= c->d_vsum1.alloc(d8 * g.NXY * g.nc, B);
+    }
+    if (rc) {
+        f3d_destroy(c);
+        return F3D_ENOMEM;
+    }
+    // host-side per-column RAMS floor Kmh = 0.075*((Hm1*dX1*dX2)**(2/3)) (kappa3.f90:175)
+    c->kmh.resize(g.NXY);
+    for (long t = 0; t < g.NXY; ++t) {
+        const double dXX = c->hm1[t] * g.dx;
+        c->kmh[t] = 0.075 * (pow(dXX * g.dy, 2.0 / 3.0));
+    }
+    hipStream_t s = c->stream;
+    rc |= upload(c->d_dz, c->dz.data(), d8 * (g.nz + 1), s);
+    rc |= upload(c->d_ldz, c->ldz.data(), d8 * (g.nz + 1), s);
+    rc |= upload(c->d_zlayer, c->zlayer.data(), d8 * g.nz, s);
+    rc |= upload(c->d_hm1, c->hm1.data(), d8 * g.NXY, s);
+    rc |= upload(c->d_kmh, c->kmh.data(), d8 * g.NXY, s);
+    rc |= upload(c->d_zc, c->zc.data(), sizeof(ZCoef) * (g.nz + 2), s);
+    // zero state, coefficients and budgets (Master.f90:getmem1 initialises to 0)
+    for (DeviceBuffer *b : {&c->d_rkhor, &c->d_rkh1, &c->d_rkver, &c->d_vset, &c->d_vdep,
+                            &c->d_cdry, &c->d_cwet, &c->d_outmass, &c->d_c64, &c->d_c32[0],
+                            &c->d_c32[1]})
+        if (b->p) rc |= (hipMemsetAsync(b->p, 0, b->bytes, s) != hipSuccess);
+    if (hipStreamSynchronize(s) != hipSuccess || rc) {
+        set_error("f3d_create: initial upload failed");
+        f3d_destroy(c);
+        return F3D_EHIP;
+    }
+    *out = c;
+    return F3D_OK;
+}
+
+void f3d_destroy(f3d_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto e : c->ev_start) (void)hipEventDestroy(e);
+    for (auto e : c->ev_stop) (void)hipEventDestroy(e);
+    c->ev_start.clear();
+    c->ev_stop.clear();
+    hipStream_t s = c->stream;
+    c->stream = nullptr;
+    delete c;  // DeviceBuffer destructors free device memory
+    if (s) (void)hipStreamDestroy(s);
+}
+
+int f3d_set_classes(f3d_ctx *c, const double *diam, const double *rhop, const double *sphe,
+                    int modv)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (!diam || !rhop || !sphe) {
+        set_error("f3d_set_classes: null argument");
+        return F3D_EINVAL;
+    }
+    std::vector<met::ClassConst> cls(c->g.nc);
+    for (int q = 0; q < c->g.nc; ++q) TRY(class_constants(modv, diam[q], rhop[q], sphe[q], &cls[q]));
+    c->cls = cls;
+    c->modv = modv;
+    TRY(upload(c->d_cls, cls.data(), sizeof(met::ClassConst) * c->g.nc, c->stream));
+    F3D_HIP(hipStreamSynchronize(c->stream));
+    c->classes_set = true;
+    return F3D_OK;
+}
+
+int f3d_set_models(f3d_ctx *c, int modkv, int modkh, double rkv0, double rkh0, double csh)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (modkv < 0 || modkv > 2 || modkh < 0 || modkh > 2) {
+        set_error("f3d_set_models: modkv/modkh out of range (%d, %d)", modkv, modkh);
+        return F3D_EINVAL;
+    }
+    c->modkv = modkv;
+    c->modkh = modkh;
+    c->rkv0 = rkv0;
+    c->rkh0 = rkh0;
+    c->csh = csh;
+    return F3D_OK;
+}
+
+int f3d_set_meteo(f3d_ctx *c, const double *vx, const double *vy, const double *vz,
+                  const double *tempe, const double *vptemp, const double *rho,
+                  const double *ustar, const double *zi, const double *rl, const double *vdep)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (!c->classes_set) {
+        set_error("f3d_set_meteo: f3d_set_classes must come first");
+        return F3D_ESTATE;
+    }
+    if (!vx || !vy || !vz || !tempe || !rho) {
+        set_error("f3d_set_meteo: null field");
+        return F3D_EINVAL;
+    }
+    const Grid &g = c->g;
+    hipStream_t s = c->stream;
+    const size_t d8 = sizeof(double);
+    std::vector<double> zeros;
+    auto up_or_zero = [&](DeviceBuffer &b, const double *h, size_t n) -> int {
+        if (h) return upload(b, h, n * d8, s);
+        F3D_HIP(hipMemsetAsync(b.p, 0, n * d8, s));
+        return F3D_OK;
+    };
+    TRY(up_or_zero(c->d_vx, vx, g.N));
+    TRY(up_or_zero(c->d_vy, vy, g.N));
+    TRY(up_or_zero(c->d_vz, vz, g.N));
+    TRY(up_or_zero(c->d_tempe, tempe, g.N));
+    TRY(up_or_zero(c->d_vptemp, vptemp, g.N));
+    TRY(up_or_zero(c->d_rho, rho, g.N));
+    TRY(up_or_zero(c->d_ustar, ustar, g.NXY));
+    TRY(up_or_zero(c->d_zi, zi, g.NXY));
+    TRY(up_or_zero(c->d_rl, rl, g.NXY));
+    TRY(up_or_zero(c->d_vdep, vdep, g.NXY * g.nc));
+    // meteo.f90:27 kappa3 (on the unscaled wind)
+    TRY(met::kappa3(s, g, c->modkv, c->modkh, c->d_dz.as<double>(), c->d_zlayer.as<double>(),
+                    c->d_vx.as<double>(), c->d_vy.as<double>(), c->d_vptemp.as<double>(),
+                    c->d_zi.as<double>(), c->d_rl.as<double>(), c->d_ustar.as<double>(),
+                    c->d_hm1.as<double>(), c->d_kmh.as<double>(), g.dx, g.dy, c->rkv0, c->rkh0,
+                    c->csh, c->lonlat, c->d_rkver.as<double>(), c->d_rkhor.as<double>()));
+    // meteo.f90:81 setvset, continuing the SAVEd drag-coefficient chain
+    double cd2[2] = {c->cd_state, c->cd_state};
+    int st0[2] = {0, 0};
+    TRY(upload(c->d_cd, cd2, sizeof cd2, s));
+    TRY(upload(c->d_status, st0, sizeof st0, s));
+    TRY(met::setvset(s, g, g.nc, c->modv, c->d_cls.as<met::ClassConst>(), c->d_tempe.as<double>(),
+                     c->d_rho.as<double>(), c->d_visc.as<double>(), c->d_vset.as<double>(),
+                     c->d_cd.as<double>(), c->d_status.as<int>(), 256, 64));
+    // meteo.f90:108-112 Hm1 scaling
+    TRY(met::scale(s, g, c->d_hm1.as<double>(), c->d_vx.as<double>(), c->d_rho.as<double>(),
+                   c->d_rkhor.as<double>(), c->d_rkh1.as<double>()));
+    // meteo.f90:149 finddt
+    F3D_HIP(hipMemsetAsync(c->d_dtcn.p, 0, sizeof(unsigned long long), s));
+    TRY(met::finddt(s, g, c->d_dz.as<double>(), g.dx, g.dy, c->d_vx.as<double>(),
+                    c->d_vy.as<double>(), c->d_vz.as<double>(), c->d_rkh1.as<double>(),
+                    c->d_rkhor.as<double>(), c->d_rkver.as<double>(), c->d_vset.as<double>(),
+                    c->d_dtcn.as<unsigned long long>()));
+    if (c->precision == F3D_FP32) {
+        TRY(f32::build_fields(s, g, c->d_vx.as<double>(), c->d_vy.as<double>(), c->d_vz.as<double>(),
+                              c->d_rho.as<double>(), c->d_rkh1.as<double>(), c->d_rkhor.as<double>(),
+                              c->d_rkver.as<double>(), c->d_vset.as<double>(), c->d_vdep.as<double>(),
+                              c->d_dz.as<double>(), c->d_f32.as<float>(), c->d_vsum1.as<double>()));
+    }
+    unsigned long long bits = 0;
+    int status[2] = {0, 0};
+    F3D_HIP(hipMemcpyAsync(&bits, c->d_dtcn.p, sizeof bits, hipMemcpyDeviceToHost, s));
+    F3D_HIP(hipMemcpyAsync(cd2, c->d_cd.p, sizeof cd2, hipMemcpyDeviceToHost, s));
+    F3D_HIP(hipMemcpyAsync(status, c->d_status.p, sizeof status, hipMemcpyDeviceToHost, s));
+    F3D_HIP(hipStreamSynchronize(s));
+    if (status[0]) {
+        set_error("No convergence in vsettl routine");  // setvset.f90:40
+        return F3D_ECONV;
+    }
+    c->cd_state = cd2[1];
+    double dtcn;
+    std::memcpy(&dtcn, &bits, sizeof dtcn);
+    // finddt.f90:250-258
+    dtcn = dtcn > 1e-13 ? dtcn : 1e-13;
+    c->dtc = 1.0 / dtcn;
+    c->dt = 0.9 * c->dtc;
+    c->meteo_ready = true;
+    return F3D_OK;
+}
+
+int f3d_get_dt(f3d_ctx *c, double *dt, double *dtc)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (dt) *dt = c->dt;
+    if (dtc) *dtc = c->dtc;
+    return F3D_OK;
+}
+
+int f3d_set_dt(f3d_ctx *c, double dt)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (!(dt > 0.0)) {
+        set_error("f3d_set_dt: dt must be positive");
+        return F3D_EINVAL;
+    }
+    c->dt = dt;
+    return F3D_OK;
+}
+
+int f3d_set_source(f3d_ctx *c, int nsou, const int *isrc, const int *jsrc, const int *ksrc,
+                   const double *tmrat, double flowrate)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (nsou < 0 || (nsou > 0 && (!isrc || !jsrc || !ksrc || !tmrat))) {
+        set_error("f3d_set_source: bad arguments");
+        return F3D_EINVAL;
+    }
+    const Grid &g = c->g;
+    for (int is = 0; is < nsou; ++is) {
+        if (isrc[is] < 1 || isrc[is] > g.nx || jsrc[is] < 1 || jsrc[is] > g.ny) {
+            // reasrc.f90:76-80
+            set_error("f3d_set_source: source point %d outside the grid", is + 1);
+            return F3D_EINVAL;
+        }
+    }
+    long long *B = &c->bytes;
+    int rc = 0;
+    const size_t n = nsou > 0 ? nsou : 1;
+    rc |= c->d_isrc.alloc(sizeof(int) * n, B);
+    rc |= c->d_jsrc.alloc(sizeof(int) * n, B);
+    rc |= c->d_ksrc.alloc(sizeof(int) * n, B);
+    rc |= c->d_tmrat.alloc(sizeof(double) * n * g.nc, B);
+    if (rc) return F3D_ENOMEM;
+    if (nsou > 0) {
+        TRY(upload(c->d_isrc, isrc, sizeof(int) * nsou, c->stream));
+        TRY(upload(c->d_jsrc, jsrc, sizeof(int) * nsou, c->stream));
+        TRY(upload(c->d_ksrc, ksrc, sizeof(int) * nsou, c->stream));
+        TRY(upload(c->d_tmrat, tmrat, sizeof(double) * nsou * g.nc, c->stream));
+    }
+    F3D_HIP(hipStreamSynchronize(c->stream));
+    c->nsou = nsou;
+    c->flowrate = flowrate;
+    return F3D_OK;
+}
+
+int f3d_step(f3d_ctx *c, int nsteps)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (!c->meteo_ready) {
+        set_error("f3d_step: no meteo read yet (f3d_set_meteo)");
+        return F3D_ESTATE;
+    }
+    for (int n = 0; n < nsteps; ++n) {
+        // Fall3d.f90:112-114
+        c->time = c->time + c->dt;
+        c->iiter = c->iiter + 1;
+        c->erumass = c->erumass + c->flowrate * c->dt;
+        TRY(c->precision == F3D_FP64 ? step_fp64(c) : step_fp32(c));
+    }
+    return F3D_OK;
+}
+
+int f3d_budget(f3d_ctx *c, double *out)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    const Grid &g = c->g;
+    hipStream_t s = c->stream;
+    double *cdry = c->d_cdry.as<double>(), *cwet = c->d_cwet.as<double>();
+    TRY(red::getc0(s, g, cdry, cwet));
+    if (c->precision == F3D_FP64)
+        TRY(red::cmass3d(s, g, c->d_c64.as<double>(), nullptr, cdry, cwet, c->d_dz.as<double>(),
+                         c->d_scratch.as<double>(), c->d_budget.as<double>()));
+    else
+        TRY(red::cmass3d(s, g, c->d_c32[c->cur].as<float>(), nullptr, cdry, cwet,
+                         c->d_dz.as<double>(), c->d_scratch.as<double>(), c->d_budget.as<double>()));
+    double sums[4], om;
+    F3D_HIP(hipMemcpyAsync(sums, c->d_budget.p, sizeof sums, hipMemcpyDeviceToHost, s));
+    F3D_HIP(hipMemcpyAsync(&om, c->d_outmass.p, sizeof om, hipMemcpyDeviceToHost, s));
+    F3D_HIP(hipStreamSynchronize(s));
+    if (out) {
+        for (int m = 0; m < 4; ++m) out[m] = sums[m] * g.dx * g.dy;  // cmass3d.f90:191-194
+        out[4] = om;
+        out[5] = c->erumass;
+    }
+    return F3D_OK;
+}
+
+int f3d_get_state(f3d_ctx *c, double *ch, double *cdry)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    const Grid &g = c->g;
+    hipStream_t s = c->stream;
+    const size_t n = (size_t)g.G * g.nc;
+    if (ch) {
+        if (c->precision == F3D_FP64) {
+            F3D_HIP(hipMemcpyAsync(ch, c->d_c64.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        } else {
+            std::vector<float> tmp(n);
+            F3D_HIP(hipMemcpyAsync(tmp.data(), c->d_c32[c->cur].p, n * sizeof(float),
+                                   hipMemcpyDeviceToHost, s));
+            F3D_HIP(hipStreamSynchronize(s));
+            for (size_t i = 0; i < n; ++i) ch[i] = (double)tmp[i];
+        }
+    }
+    if (cdry)
+        F3D_HIP(hipMemcpyAsync(cdry, c->d_cdry.p, sizeof(double) * g.NXY * (g.nc + 1),
+                               hipMemcpyDeviceToHost, s));
+    F3D_HIP(hipStreamSynchronize(s));
+    return F3D_OK;
+}
+
+int f3d_set_state(f3d_ctx *c, const double *ch, const double *cdry)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    const Grid &g = c->g;
+    hipStream_t s = c->stream;
+    const size_t n = (size_t)g.G * g.nc;
+    if (ch) {
+        if (c->precision == F3D_FP64) {
+            F3D_HIP(hipMemcpyAsync(c->d_c64.p, ch, n * sizeof(double), hipMemcpyHostToDevice, s));
+        } else {
+            std::vector<float> tmp(n);
+            for (size_t i = 0; i < n; ++i) tmp[i] = (float)ch[i];
+            F3D_HIP(hipMemcpyAsync(c->d_c32[c->cur].p, tmp.data(), n * sizeof(float),
+                                   hipMemcpyHostToDevice, s));
+            F3D_HIP(hipStreamSynchronize(s));
+        }
+    }
+    if (cdry)
+        F3D_HIP(hipMemcpyAsync(c->d_cdry.p, cdry, sizeof(double) * g.NXY * (g.nc + 1),
+                               hipMemcpyHostToDevice, s));
+    F3D_HIP(hipStreamSynchronize(s));
+    return F3D_OK;
+}
+
+int f3d_get_coef(f3d_ctx *c, double *vx, double *rho, double *rkhor, double *rkh1,
+                 double *rkver, double *vset)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    const Grid &g = c->g;
+    hipStream_t s = c->stream;
+    const size_t b = sizeof(double) * g.N;
+    if (vx) F3D_HIP(hipMemcpyAsync(vx, c->d_vx.p, b, hipMemcpyDeviceToHost, s));
+    if (rho) F3D_HIP(hipMemcpyAsync(rho, c->d_rho.p, b, hipMemcpyDeviceToHost, s));
+    if (rkhor) F3D_HIP(hipMemcpyAsync(rkhor, c->d_rkhor.p, b, hipMemcpyDeviceToHost, s));
+    if (rkh1) F3D_HIP(hipMemcpyAsync(rkh1, c->d_rkh1.p, b, hipMemcpyDeviceToHost, s));
+    if (rkver) F3D_HIP(hipMemcpyAsync(rkver, c->d_rkver.p, b, hipMemcpyDeviceToHost, s));
+    if (vset) F3D_HIP(hipMemcpyAsync(vset, c->d_vset.p, b * g.nc, hipMemcpyDeviceToHost, s));
+    F3D_HIP(hipStreamSynchronize(s));
+    return F3D_OK;
+}
+
+int f3d_get_clock(f3d_ctx *c, double *time, int *iiter, double *erumass, double *outmass)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (time) *time = c->time;
+    if (iiter) *iiter = c->iiter;
+    if (erumass) *erumass = c->erumass;
+    if (outmass) {
+        F3D_HIP(hipMemcpyAsync(outmass, c->d_outmass.p, sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+        F3D_HIP(hipStreamSynchronize(c->stream));
+    }
+    return F3D_OK;
+}
+
+int f3d_set_clock(f3d_ctx *c, double time, int iiter, double erumass, double outmass)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    c->time = time;
+    c->iiter = iiter;
+    c->erumass = erumass;
+    F3D_HIP(hipMemcpyAsync(c->d_outmass.p, &outmass, sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    F3D_HIP(hipStreamSynchronize(c->stream));
+    return F3D_OK;
+}
+
+int f3d_get_cd_state(f3d_ctx *c, double *cd)
+{
+    if (!check_ctx(c) || !cd) return F3D_EINVAL;
+    *cd = c->cd_state;
+    return F3D_OK;
+}
+
+int f3d_set_cd_state(f3d_ctx *c, double cd)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    c->cd_state = cd;
+    return F3D_OK;
+}
+
+int f3d_sync(f3d_ctx *c)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    F3D_HIP(hipStreamSynchronize(c->stream));
+    return F3D_OK;
+}
+
+int f3d_set_tile(f3d_ctx *c, int tx, int ty)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    const int ok[][2] = {{32, 8}, {32, 16}, {64, 8}, {64, 4}, {16, 16}};
+    for (auto &t : ok)
+        if (t[0] == tx && t[1] == ty) {
+            c->tile = f32::TileShape{tx, ty};
+            return F3D_OK;
+        }
+    set_error("f3d_set_tile: unsupported tile %dx%d", tx, ty);
+    return F3D_EINVAL;
+}
+
+int f3d_timing(f3d_ctx *c, int enable)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (enable && c->ev_start.empty()) {
+        c->ev_start.resize(512);
+        c->ev_stop.resize(512);
+        for (size_t i = 0; i < c->ev_start.size(); ++i) {
+            F3D_HIP(hipEventCreate(&c->ev_start[i]));
+            F3D_HIP(hipEventCreate(&c->ev_stop[i]));
+        }
+    }
+    if (!enable) flush_timing(c);
+    c->timing = enable != 0;
+    return F3D_OK;
+}
+
+int f3d_kernel_time(f3d_ctx *c, int reset, double *avg_ms, long *count)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    flush_timing(c);
+    if (avg_ms) *avg_ms = c->t_count ? c->t_ms / (double)c->t_count : 0.0;
+    if (count) *count = c->t_count;
+    if (reset) {
+        c->t_ms = 0.0;
+        c->t_count = 0;
+    }
+    return F3D_OK;
+}
+
+long long f3d_device_bytes(const f3d_ctx *c) { return c ? c->bytes : 0; }
+
+}  // extern "C"

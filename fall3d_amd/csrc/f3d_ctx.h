@@ -1,0 +1,144 @@
+// Device-resident context behind the f3d_* C ABI.
+#pragma once
+
+#include <vector>
+
+#include "f3d_internal.h"
+
+namespace f3d {
+
+struct DeviceBuffer {
+    void *p = nullptr;
+    size_t bytes = 0;
+    long long *cnt = nullptr;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() { release(); }
+    int alloc(size_t n, long long *counter);
+    void release();
+    template <typename T>
+    T *as() const
+    {
+        return static_cast<T *>(p);
+    }
+};
+
+int get_gama(double diam, double sphe, double *gama_out);
+int class_constants(int modv, double diam, double rhop, double sphe, met::ClassConst *out);
+Grid make_grid(int nx, int ny, int nz, int nc, double dx, double dy);
+void vertical_coefs(int nz, const double *zlayer, std::vector<double> &dz,
+                    std::vector<ZCoef> &zc, std::vector<double> &ldz);
+
+// ---------------- FP32 fused path (f3d_fused32.hip) ----------------
+namespace f32 {
+
+// FP32 step fields, built from the FP64 coefficients after a meteo read.
+// 3-D fields share the (nx,ny,nz) layout; per-class fields are stacked.
+struct Fields {
+    const float *vx, *vy, *vz;  // scaled velocities (vx/Hm1)
+    const float *lr;            // 1/rho (rho scaled by Hm1)
+    const float *pkx;           // rho*rkh1  (diffx face products)
+    const float *pky;           // rho*rkhor (diffy face products)
+    const float *pkz;           // rho*rkver (diffz face products)
+    const float *vs;            // vset (nx,ny,nz,nc)
+    const float *bfac;          // bottom ghost factor (nx,ny,nc): 0, or 1+dz0*vdep/rkver(1)
+    const float *tfac;          // top ghost factor (nx,ny,nc): 0 or 1 (setbcc.f90:126-130)
+    const double *vsum1;        // min(0, vset(:,:,1)+vdep) (nx,ny,nc), FP64 deposit factor
+};
+
+// Build Fields from the FP64 coefficient arrays (meteo.f90:108-112 applied).
+int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *vy,
+                 const double *vz, const double *rho, const double *rkh1, const double *rkhor,
+                 const double *rkver, const double *vset, const double *vdep, const double *dz,
+                 float *f32buf, double *vsum1);
+size_t fields_floats(const Grid &g);
+Fields fields_view(const Grid &g, const float *f32buf, const double *vsum1);
+
+struct StepArgs {
+    Grid g;
+    const float *cin;
+    float *cout;
+    double *cdry;       // (nx,ny,0:nc); class q accumulates into slot q+1
+    double dt;
+    int iiter;
+    const ZCoef *zc;    // per level (FP64, converted in kernel)
+};
+
+// Tile geometry of the fused kernel (chosen by the autotuner)
+struct TileShape {
+    int tx, ty;
+};
+
+// One fused launch: setbcc + advctzc + advctx/advcty (parity order) + diffz
+// + diffx/diffy (parity order) + accum + clip for every class, c_in -> c_out.
+int fused_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts);
+// Ghost frame of c_out (setbcc values of c_in, clipped) and cmass2d partials;
+// then outmass += sum (deterministic).
+int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,
+                      const double *vy64, const double *vz64, const double *vset64,
+                      const double *dz, double *scratch, double *outmass);
+int setsrc(hipStream_t s, const Grid &g, float *c, int nsou, const int *isrc, const int *jsrc,
+           const int *ksrc, const double *tmrat, const double *zlayer, const double *dz,
+           const double *hm1, double dt);
+int to_fp64(hipStream_t s, const float *c32, double *c64, long n);
+int from_fp64(hipStream_t s, const double *c64, float *c32, long n);
+
+}  // namespace f32
+
+}  // namespace f3d
+
+// The context
+struct f3d_ctx {
+    int precision = F3D_FP64;
+    f3d::Grid g{};
+    int lonlat = 0;
+    hipStream_t stream = nullptr;
+    long long bytes = 0;
+
+    // host copies
+    std::vector<double> zlayer, dz, ldz, hm1, kmh;
+    std::vector<f3d::ZCoef> zc;
+    bool uniform_hm1 = true;
+
+    // classes and models
+    int modv = 2;
+    bool classes_set = false;
+    std::vector<f3d::met::ClassConst> cls;
+    int modkv = 0, modkh = 0;
+    double rkv0 = 0.0, rkh0 = 0.0, csh = 0.2275;
+
+    // device: grid
+    f3d::DeviceBuffer d_dz, d_ldz, d_zlayer, d_hm1, d_kmh, d_zc;
+    // device: meteo and coefficients (FP64)
+    f3d::DeviceBuffer d_vx, d_vy, d_vz, d_tempe, d_vptemp, d_rho, d_ustar, d_zi, d_rl, d_vdep;
+    f3d::DeviceBuffer d_rkhor, d_rkh1, d_rkver, d_vset, d_visc;
+    f3d::DeviceBuffer d_cls, d_cd, d_status, d_dtcn;
+    // device: state
+    f3d::DeviceBuffer d_c64;          // FP64 path
+    f3d::DeviceBuffer d_c32[2];       // FP32 path ping-pong
+    int cur = 0;                      // FP32: index of the current state buffer
+    f3d::DeviceBuffer d_cdry, d_cwet;
+    f3d::DeviceBuffer d_scratch, d_outmass, d_budget;
+    // FP32 step fields
+    f3d::DeviceBuffer d_f32, d_vsum1;
+    f3d::f32::TileShape tile{32, 8};
+
+    // source
+    int nsou = 0;
+    f3d::DeviceBuffer d_isrc, d_jsrc, d_ksrc, d_tmrat;
+    double flowrate = 0.0;
+
+    // clock
+    double time = 0.0, dt = 0.0, dtc = 0.0, erumass = 0.0;
+    int iiter = 0;
+    bool meteo_ready = false;
+    double cd_state = 1.0;
+
+    // timing of the dominant step kernel
+    bool timing = false;
+    std::vector<hipEvent_t> ev_start, ev_stop;
+    int ev_next = 0;
+    double t_ms = 0.0;
+    long t_count = 0;
+};

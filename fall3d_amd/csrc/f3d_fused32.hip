@@ -1,0 +1,708 @@
+// Fused FP32 transport step for MI355X (gfx950).
+//
+// One launch performs, for every class, the whole class step of
+// Sources/sizloop.f90:31-78 -- setbcc, advctzc, advctx/advcty in iiter
+// parity order, diffz, diffx/diffy in parity order, accum and the clip --
+// reading c_in and writing c_out (ping-pong buffers, so neighbouring tiles
+// always read pre-step values).  A second small kernel writes the ghost frame
+// of c_out and the cmass2d outflux.  Deposit and budgets accumulate in FP64.
+//
+// Work decomposition: one workgroup (256 lanes, 4 waves) per (tile, class).
+// A tile is TX x TY columns; the workgroup holds a (TX+6) x (TY+6) region
+// (halo 3 = advection radius 2 + diffusion radius 1) and marches up the
+// column one level at a time as a software pipeline:
+//   A  advctzc at level s        per column, register window along k
+//   B1 first horizontal advection at level s        LDS plane
+//   B2 second horizontal advection at level s       LDS plane -> registers
+//   C  diffz at level s-1        per column, register window along k
+//   D1 first horizontal diffusion at level s-1      LDS plane
+//   D2 second horizontal diffusion at level s-1     -> c_out, deposit
+// Each region column is owned by one lane for the whole kernel (lane =
+// column % 256), so the z windows live in registers and only one plane per
+// stage is exchanged through LDS (four barriers per level).  Ghost cells
+// never change during a step (the reference refreshes them once per class
+// step, before advection, sizloop.f90:35), so ghost columns pass through
+// every stage with their setbcc value.
+//
+// Block order is XCD-aware: consecutive logical blocks (the classes of one
+// tile) are placed on the same XCD so the tile's meteorology is fetched
+// from HBM once and served to its classes from that XCD's L2.
+#include "f3d_ctx.h"
+
+namespace f3d {
+namespace f32 {
+namespace {
+
+constexpr int kNT = 256;
+constexpr int kXCD = 8;
+
+__device__ __forceinline__ float minmodf(float dr, float dl)
+{
+    const float m = fminf(fabsf(dr), fabsf(dl));
+    return (dr > 0.f) == (dl > 0.f) && dr != 0.f && dl != 0.f ? copysignf(m, dr) : 0.f;
+}
+
+struct LevelCoef {
+    float dtdzz;  // dt*2/(dz(k)+dz(k-1))
+    float a2;     // dt*0.5*dz(k-1)*dzz(k)
+    float bc;     // dt/dz(k-1)
+    float dzk;    // 1/dz(k)
+    float dzk_1;  // 1/dz(k-1)
+    float ldz;    // 1/dz(k)
+};
+
+struct Params {
+    Grid g;
+    const float *cin;
+    float *cout;
+    double *cdry;
+    Fields f;
+    const ZCoef *zc;
+    double dt64;
+    float dt;
+    int odd;  // iiter odd: y sweeps first (sizloop.f90:42-61)
+    int ntx, nty;
+    int nlogical;
+    int per_xcd;
+};
+
+// advctzc W2(k) (advctzc.f90:164-169), FP32
+__device__ __forceinline__ float w2z(float cm, float c0, float cp, float vk, float vk_1,
+                                     const LevelCoef &z)
+{
+    const float v12 = 0.5f * (vk + vk_1);
+    const float av = fabsf(v12);
+    const float sigma = minmodf((cp - c0) * z.dzk, (c0 - cm) * z.dzk_1);
+    return z.a2 * (1.f - av * z.bc) * sigma * av;
+}
+
+// advctx/advcty update at a line position held in an LDS plane with stride
+// st: P plane values, V velocity (clamped to the domain at the ends),
+// auxv = dt/h, kh = 0.5*dt/h; lo/hi flag the first/last domain cell.
+__device__ __forceinline__ float adv_update(const float *P, const float *V, int st, float auxv,
+                                            float kh, bool lo, bool hi)
+{
+    auto W2 = [&](int o) -> float {
+        const float v12 = 0.5f * (V[o] + V[o - st]);
+        const float av = fabsf(v12);
+        const float sigma = minmodf(P[o + st] - P[o], P[o] - P[o - st]);
+        return kh * (1.f - auxv * av) * sigma * av;
+    };
+    const float c0 = P[0];
+    const float w2i = W2(0);
+    if (V[0] > 0.f) {
+        const float w1 = auxv * (V[0] * P[0] - V[-st] * P[-st]);
+        const float w2m = lo ? w2i : W2(-st);
+        return c0 - w1 - (w2i - w2m);
+    }
+    const float w1 = auxv * (V[st] * P[st] - V[0] * P[0]);
+    const float w2p = hi ? w2i : W2(st);
+    return c0 - w1 - (w2p - w2i);
+}
+
+// diffx/diffy update (diffx.f90:20-53), FP32 with reciprocal density
+__device__ __forceinline__ float diff_update(const float *P, const float *LR, const float *PK,
+                                             int st, float dtdh2, bool lo, bool hi)
+{
+    float gr, gl;
+    if (lo) {
+        gr = PK[st] * (P[st] * LR[st] - P[0] * LR[0]);
+        gl = fmaxf(0.f, PK[0] * (P[0] * LR[0] - P[-st] * LR[0]));
+    } else if (hi) {
+        gr = fminf(0.f, PK[0] * (P[st] * LR[0] - P[0] * LR[0]));
+        gl = PK[-st] * (P[0] * LR[0] - P[-st] * LR[-st]);
+    } else {
+        gr = 0.5f * (PK[st] + PK[0]) * (P[st] * LR[st] - P[0] * LR[0]);
+        gl = 0.5f * (PK[0] + PK[-st]) * (P[0] * LR[0] - P[-st] * LR[-st]);
+    }
+    return P[0] + dtdh2 * (gr - gl);
+}
+
+template <int TX, int TY>
+__global__ void __launch_bounds__(kNT) k_fused(Params p)
+{
+    constexpr int RX = TX + 6, RY = TY + 6, RN = RX * RY;
+    constexpr int NSLOT = (RN + kNT - 1) / kNT;
+
+    // XCD-aware logical block id
+    const int b = blockIdx.x;
+    const int L = (b % kXCD) * p.per_xcd + b / kXCD;
+    if (L >= p.nlogical) return;
+    const int nc = p.g.nc;
+    const int q = L % nc;
+    const int tile = L / nc;
+    const int bx = tile % p.ntx, by = tile / p.ntx;
+
+    const int nx = p.g.nx, ny = p.g.ny, nz = p.g.nz;
+    const long NXY = p.g.NXY, N = p.g.N;
+    const long SZ = (long)(nx + 2) * (ny + 2);
+    const int gi0 = 1 + bx * TX - 3, gj0 = 1 + by * TY - 3;
+    const int tid = threadIdx.x;
+
+    __shared__ float sA[RN], sB[RN], sC[RN], sD[RN];
+    __shared__ float sVX[RN], sVY[RN], sLR[RN], sPX[RN], sPY[RN];
+    __shared__ LevelCoef zt[66];
+
+    const float dt = p.dt;
+    for (int k = tid; k <= nz + 1 && k < 66; k += kNT) {
+        LevelCoef z{};
+        if (k >= 1 && k <= nz) {
+            const ZCoef c = p.zc[k];
+            z.dtdzz = (float)(p.dt64 * c.dzz);
+            z.a2 = (float)(p.dt64 * 0.5 * c.dzm1 * c.dzz);
+            z.bc = (float)(p.dt64 * c.dzk_1);
+            z.dzk = (float)c.dzk;
+            z.dzk_1 = (float)c.dzk_1;
+            z.ldz = (float)c.dzk;
+        }
+        if (k == 0) z.ldz = (float)p.zc[1].dzk_1;
+        zt[k] = z;
+    }
+
+    const float *cin = p.cin + (long)q * p.g.G;
+    float *cout = p.cout + (long)q * p.g.G;
+    const float *vs = p.f.vs + (long)q * N;
+
+    // per-slot column classification
+    int kind[NSLOT];   // 0 other, 1 interior, 2 x-ghost, 3 y-ghost
+    int ctr[NSLOT];    // 1 if inside the diffz region (halo 1)
+    long m2[NSLOT];    // 2-D offset (i-1)+nx*(j-1) of the column (or its face neighbour)
+    long coff[NSLOT];  // c offset of the column at k=0 (or of its face neighbour)
+#pragma unroll
+    for (int sl = 0; sl < NSLOT; ++sl) {
+        const int r = tid + kNT * sl;
+        const int li = r % RX, lj = r / RX;
+        const int gi = gi0 + li, gj = gj0 + lj;
+        kind[sl] = 0;
+        ctr[sl] = 0;
+        m2[sl] = 0;
+        coff[sl] = 0;
+        if (r < RN) {
+            const bool inx = gi >= 1 && gi <= nx, iny = gj >= 1 && gj <= ny;
+            if (inx && iny) {
+                kind[sl] = 1;
+            } else if (iny && (gi == 0 || gi == nx + 1)) {
+                kind[sl] = 2;
+            } else if (inx && (gj == 0 || gj == ny + 1)) {
+                kind[sl] = 3;
+            }
+            const int ci = min(max(gi, 1), nx), cj = min(max(gj, 1), ny);
+            m2[sl] = (ci - 1) + (long)nx * (cj - 1);
+            coff[sl] = cidx(nx, ny, ci, cj, 0);
+            ctr[sl] = (li >= 2 && li < RX - 2 && lj >= 2 && lj < RY - 2) ? 1 : 0;
+        }
+    }
+    __syncthreads();
+
+    // stage A state (old column window) and stage C state
+    float cm1[NSLOT], c0[NSLOT], cp1[NSLOT], vk[NSLOT], W1k[NSLOT], W2km1[NSLOT], W2k[NSLOT];
+    float gbot[NSLOT], gtopm[NSLOT];
+    float bm2[NSLOT], bm1[NSLOT], lrm2[NSLOT], lrm1[NSLOT], pzm2[NSLOT], pzm1[NSLOT];
+    float bcur[NSLOT];
+
+#pragma unroll
+    for (int sl = 0; sl < NSLOT; ++sl) {
+        cm1[sl] = c0[sl] = cp1[sl] = vk[sl] = W1k[sl] = W2km1[sl] = W2k[sl] = 0.f;
+        gbot[sl] = gtopm[sl] = 0.f;
+        bm2[sl] = bm1[sl] = lrm2[sl] = lrm1[sl] = pzm2[sl] = pzm1[sl] = bcur[sl] = 0.f;
+        if (kind[sl] == 1) {
+            const long o = coff[sl];
+            const float c1 = cin[o + SZ];
+            const float c2 = cin[o + 2 * SZ];
+            const float bf = p.f.bfac[(long)q * NXY + m2[sl]];
+            gtopm[sl] = p.f.tfac[(long)q * NXY + m2[sl]];
+            gbot[sl] = c1 * bf;  // setbcc.f90:108-113
+            cm1[sl] = gbot[sl];
+            c0[sl] = c1;
+            cp1[sl] = c2;
+            const float v1 = p.f.vz[m2[sl]] + vs[m2[sl]];
+            vk[sl] = v1;
+            const LevelCoef z1 = zt[1];
+            W1k[sl] = z1.dtdzz * (v1 * c0[sl] - v1 * cm1[sl]);
+            W2k[sl] = w2z(cm1[sl], c0[sl], cp1[sl], v1, v1, z1);
+            W2km1[sl] = W2k[sl];
+            bm2[sl] = gbot[sl];
+        }
+    }
+
+    const float auxx = (float)(p.dt64 / p.g.dx), khx = 0.5f * auxx;
+    const float auxy = (float)(p.dt64 / p.g.dy), khy = 0.5f * auxy;
+    const float dtdx2 = (float)(p.dt64 / (p.g.dx * p.g.dx));
+    const float dtdy2 = (float)(p.dt64 / (p.g.dy * p.g.dy));
+    const int odd = p.odd;
+
+    for (int s = 1; s <= nz + 1; ++s) {
+        // ---------------- stage A: advctzc at level s ----------------
+        if (s <= nz) {
+#pragma unroll
+            for (int sl = 0; sl < NSLOT; ++sl) {
+                const int r = tid + kNT * sl;
+                if (r >= RN) continue;
+                const long lev = (long)(s - 1) * NXY + m2[sl];
+                float a = 0.f;
+                if (kind[sl] == 1) {
+                    float W1n, W2n, vkn = 0.f, cp2 = 0.f;
+                    if (s < nz) {
+                        cp2 = (s + 2 <= nz) ? cin[coff[sl] + (long)(s + 2) * SZ]
+                                            : gtopm[sl] * cp1[sl];  // top ghost (setbcc.f90:126-130)
+                        vkn = p.f.vz[lev + NXY] + vs[lev + NXY];
+                        const LevelCoef z = zt[s + 1];
+                        W1n = z.dtdzz * (vkn * cp1[sl] - vk[sl] * c0[sl]);
+                        W2n = w2z(c0[sl], cp1[sl], cp2, vkn, vk[sl], z);
+                    } else {
+                        W1n = dt * zt[nz].ldz * vk[sl] * (cp1[sl] - c0[sl]);  // advctzc.f90:175-181
+                        W2n = W2k[sl];
+                    }
+                    if (vk[sl] > 0.f)
+                        a = c0[sl] - W1k[sl] - (W2k[sl] - W2km1[sl]);
+                    else
+                        a = c0[sl] - W1n - (W2n - W2k[sl]);
+                    cm1[sl] = c0[sl];
+                    c0[sl] = cp1[sl];
+                    cp1[sl] = cp2;
+                    vk[sl] = vkn;
+                    W1k[sl] = W1n;
+                    W2km1[sl] = W2k[sl];
+                    W2k[sl] = W2n;
+                } else if (kind[sl] == 2) {
+                    // x face ghost (setbcc.f90:139-152), from the adjacent cell
+                    const int li = r % RX;
+                    const int gi = gi0 + li;
+                    const float v = p.f.vx[lev];
+                    const bool zero = (gi == 0) ? (v >= 0.f) : (v <= 0.f);
+                    a = zero ? 0.f : cin[coff[sl] + (long)s * SZ];
+                } else if (kind[sl] == 3) {
+                    const int lj = r / RX;
+                    const int gj = gj0 + lj;
+                    const float v = p.f.vy[lev];
+                    const bool zero = (gj == 0) ? (v >= 0.f) : (v <= 0.f);
+                    a = zero ? 0.f : cin[coff[sl] + (long)s * SZ];
+                }
+                sA[r] = a;
+                sVX[r] = p.f.vx[lev];
+                sVY[r] = p.f.vy[lev];
+            }
+        }
+        __syncthreads();
+
+        // ---------------- stage B1 + B2: horizontal advection at level s ----------------
+        if (s <= nz) {
+            // B1: odd -> advcty on rows lj in [2,RY-2) ... output region covers what B2 reads
+            for (int r = tid; r < RN; r += kNT) {
+                const int li = r % RX, lj = r / RX;
+                const int gi = gi0 + li, gj = gj0 + lj;
+                const bool interior = gi >= 1 && gi <= nx && gj >= 1 && gj <= ny;
+                float out = sA[r];
+                if (odd) {
+                    if (lj >= 2 && lj < RY - 2 && interior)
+                        out = adv_update(&sA[r], &sVY[r], RX, auxy, khy, gj == 1, gj == ny);
+                } else {
+                    if (li >= 2 && li < RX - 2 && interior)
+                        out = adv_update(&sA[r], &sVX[r], 1, auxx, khx, gi == 1, gi == nx);
+                }
+                sB[r] = out;
+            }
+        }
+        __syncthreads();
+
+        // B2 (owner lanes of the diffz region) then stage C: diffz at level s-1
+#pragma unroll
+        for (int sl = 0; sl < NSLOT; ++sl) {
+            const int r = tid + kNT * sl;
+            if (r >= RN || !ctr[sl]) continue;
+            const int li = r % RX, lj = r / RX;
+            const int gi = gi0 + li, gj = gj0 + lj;
+            float bnew;
+            if (s <= nz) {
+                bnew = sB[r];
+                if (kind[sl] == 1) {
+                    if (odd)
+                        bnew = adv_update(&sB[r], &sVX[r], 1, auxx, khx, gi == 1, gi == nx);
+                    else
+                        bnew = adv_update(&sB[r], &sVY[r], RX, auxy, khy, gj == 1, gj == ny);
+                }
+            } else {
+                bnew = (kind[sl] == 1) ? gtopm[sl] * cm1[sl] : bm1[sl];  // b(nz+1) = top ghost
+            }
+            if (kind[sl] == 1) {
+                float lrn = 0.f, pzn = 0.f;
+                if (s <= nz) {
+                    const long lev = (long)(s - 1) * NXY + m2[sl];
+                    lrn = p.f.lr[lev];
+                    pzn = p.f.pkz[lev];
+                }
+                if (s >= 2) {
+                    const int k = s - 1;
+                    float dnew;
+                    if (k == 1) {
+                        const float gt = pzn * (bnew * lrn - bm1[sl] * lrm1[sl]) * zt[1].ldz;
+                        const float gb =
+                            fmaxf(0.f, pzm1[sl] * (bm1[sl] * lrm1[sl] - bm2[sl] * lrm1[sl]) * zt[0].ldz);
+                        dnew = bm1[sl] + zt[1].dtdzz * (gt - gb);
+                    } else if (k < nz) {
+                        const float gt = 0.5f * (pzn + pzm1[sl]) * (bnew * lrn - bm1[sl] * lrm1[sl]) * zt[k].ldz;
+                        const float gb = 0.5f * (pzm1[sl] + pzm2[sl]) *
+                                         (bm1[sl] * lrm1[sl] - bm2[sl] * lrm2[sl]) * zt[k - 1].ldz;
+                        dnew = bm1[sl] + zt[k].dtdzz * (gt - gb);
+                    } else {
+                        const float gt =
+                            fminf(0.f, pzm1[sl] * (bnew * lrm1[sl] - bm1[sl] * lrm1[sl]) * zt[nz].ldz);
+                        const float gb = pzm2[sl] * (bm1[sl] * lrm1[sl] - bm2[sl] * lrm2[sl]) * zt[nz - 1].ldz;
+                        dnew = bm1[sl] + dt * zt[nz].ldz * (gt - gb);
+                    }
+                    sC[r] = dnew;
+                    sLR[r] = lrm1[sl];
+                    const long lev = (long)(k - 1) * NXY + m2[sl];
+                    sPX[r] = p.f.pkx[lev];
+                    sPY[r] = p.f.pky[lev];
+                }
+                bm2[sl] = bm1[sl];
+                bm1[sl] = bnew;
+                lrm2[sl] = lrm1[sl];
+                lrm1[sl] = lrn;
+                pzm2[sl] = pzm1[sl];
+                pzm1[sl] = pzn;
+            } else {
+                if (s >= 2) {
+                    sC[r] = bm1[sl];
+                    sLR[r] = 0.f;
+                    sPX[r] = 0.f;
+                    sPY[r] = 0.f;
+                }
+                bm1[sl] = bnew;
+            }
+        }
+        __syncthreads();
+        if (s < 2) continue;
+
+        // ---------------- stage D1: first horizontal diffusion at level s-1 ----------------
+        {
+            // odd: diffy first on li in [2,RX-2), lj in [3,RY-3); even: diffx first on
+            // li in [3,RX-3), lj in [2,RY-2)
+            const int nli = odd ? RX - 4 : TX, nlj = odd ? TY : RY - 4;
+            const int li0 = odd ? 2 : 3, lj0 = odd ? 3 : 2;
+            for (int e = tid; e < nli * nlj; e += kNT) {
+                const int li = li0 + e % nli, lj = lj0 + e / nli;
+                const int r = lj * RX + li;
+                const int gi = gi0 + li, gj = gj0 + lj;
+                const bool interior = gi >= 1 && gi <= nx && gj >= 1 && gj <= ny;
+                float out = sC[r];
+                if (interior) {
+                    if (odd)
+                        out = diff_update(&sC[r], &sLR[r], &sPY[r], RX, dtdy2, gj == 1, gj == ny);
+                    else
+                        out = diff_update(&sC[r], &sLR[r], &sPX[r], 1, dtdx2, gi == 1, gi == nx);
+                }
+                sD[r] = out;
+            }
+        }
+        __syncthreads();
+
+        // ---------------- stage D2: last sweep, output, deposit ----------------
+        {
+            const int k = s - 1;
+            for (int e = tid; e < TX * TY; e += kNT) {
+                const int li = 3 + e % TX, lj = 3 + e / TX;
+                const int gi = gi0 + li, gj = gj0 + lj;
+                if (gi > nx || gj > ny) continue;
+                const int r = lj * RX + li;
+                float out;
+                if (odd)
+                    out = diff_update(&sD[r], &sLR[r], &sPX[r], 1, dtdx2, gi == 1, gi == nx);
+                else
+                    out = diff_update(&sD[r], &sLR[r], &sPY[r], RX, dtdy2, gj == 1, gj == ny);
+                cout[cidx(nx, ny, gi, gj, k)] = fmaxf(0.f, out);  // sizloop.f90:77-78
+                if (k == 1) {
+                    // accum.f90:22, deposit from c(:,:,1) before the clip, FP64
+                    const long t2 = (gi - 1) + (long)nx * (gj - 1);
+                    double *cd = p.cdry + (long)(q + 1) * NXY + t2;
+                    *cd = *cd - p.dt64 * (double)out * p.f.vsum1[(long)q * NXY + t2];
+                }
+            }
+        }
+        // the next level's stage A writes sA/sVX/sVY, last read in B2 before the
+        // barrier after stage C; stage D planes are rewritten only after two
+        // more barriers
+    }
+}
+
+// ---------------- ghost frame + cmass2d outflux ----------------
+// setbcc (setbcc.f90:19-109) values of c_in written to c_out's frame, then
+// clipped; cmass2d terms (cmass2d.f90:68-127) of the clipped ghosts, FP64.
+__global__ void __launch_bounds__(256) k_frame(Grid g, const float *cin, float *cout, Fields f,
+                                               const double *vx, const double *vy,
+                                               const double *vz, const double *vset,
+                                               const double *dz, double dt, double *part)
+{
+    __shared__ double sh[256];
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long nxf = (long)ny * nz, nyf = (long)nx * nz, nxy = g.NXY;
+    const long nedge = 4L * nz + 4L * ny + 4L * nx + 8;
+    const long per = nxf + nyf + nxy + nedge;
+    const long total = per * g.nc;
+    double acc = 0.0;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < total; t += (long)gridDim.x * 256) {
+        const int q = (int)(t / per);
+        long r = t - (long)q * per;
+        const float *ci = cin + (long)q * g.G;
+        float *co = cout + (long)q * g.G;
+        if (r < nxf) {
+            const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
+            const float gl = (vx[midx(nx, ny, 1, j, k)] >= 0.0) ? 0.f : fmaxf(0.f, ci[cidx(nx, ny, 1, j, k)]);
+            const float gr = (vx[midx(nx, ny, nx, j, k)] <= 0.0) ? 0.f : fmaxf(0.f, ci[cidx(nx, ny, nx, j, k)]);
+            co[cidx(nx, ny, 0, j, k)] = gl;
+            co[cidx(nx, ny, nx + 1, j, k)] = gr;
+            const double faty = (j == 1 || j == ny) ? 0.5 : 1.0;
+            const double area = faty * g.dy * 0.5 * (dz[k] + dz[k - 1]);
+            acc += -(dt * area * vx[midx(nx, ny, 1, j, k)] * (double)gl);
+            acc += dt * area * vx[midx(nx, ny, nx, j, k)] * (double)gr;
+            continue;
+        }
+        r -= nxf;
+        if (r < nyf) {
+            const int i = (int)(r % nx) + 1, k = (int)(r / nx) + 1;
+            const float gd = (vy[midx(nx, ny, i, 1, k)] >= 0.0) ? 0.f : fmaxf(0.f, ci[cidx(nx, ny, i, 1, k)]);
+            const float gu = (vy[midx(nx, ny, i, ny, k)] <= 0.0) ? 0.f : fmaxf(0.f, ci[cidx(nx, ny, i, ny, k)]);
+            co[cidx(nx, ny, i, 0, k)] = gd;
+            co[cidx(nx, ny, i, ny + 1, k)] = gu;
+            const double fatx = (i == 1 || i == nx) ? 0.5 : 1.0;
+            const double area = fatx * g.dx * 0.5 * (dz[k] + dz[k - 1]);
+            acc += -(dt * area * vy[midx(nx, ny, i, 1, k)] * (double)gd);
+            acc += dt * area * vy[midx(nx, ny, i, ny, k)] * (double)gu;
+            continue;
+        }
+        r -= nyf;
+        if (r < nxy) {
+            const int i = (int)(r % nx) + 1, j = (int)(r / nx) + 1;
+            const float gb = fmaxf(0.f, ci[cidx(nx, ny, i, j, 1)] * f.bfac[(long)q * nxy + r]);
+            const float gt = fmaxf(0.f, f.tfac[(long)q * nxy + r] * ci[cidx(nx, ny, i, j, nz)]);
+            co[cidx(nx, ny, i, j, 0)] = gb;
+            co[cidx(nx, ny, i, j, nz + 1)] = gt;
+            const double fatx = (i == 1 || i == nx) ? 0.5 : 1.0;
+            const double faty = (j == 1 || j == ny) ? 0.5 : 1.0;
+            const double area = fatx * faty * g.dx * g.dy;
+            const double velo = vz[midx(nx, ny, i, j, nz)] + vset[(long)q * g.N + midx(nx, ny, i, j, nz)];
+            acc += dt * area * velo * (double)gt;
+            continue;
+        }
+        r -= nxy;
+        // frame edges are never set by setbcc: carried over (clipped);
+        // corners copy the adjacent interior corner (setbcc.f90:173-195)
+        long o;
+        float v;
+        if (r < 4L * nz) {
+            const int e = (int)(r / nz), k = (int)(r % nz) + 1;
+            const int i = (e & 1) ? nx + 1 : 0, j = (e & 2) ? ny + 1 : 0;
+            o = cidx(nx, ny, i, j, k);
+            v = fmaxf(0.f, ci[o]);
+        } else if ((r -= 4L * nz) < 4L * ny) {
+            const int e = (int)(r / ny), j = (int)(r % ny) + 1;
+            const int i = (e & 1) ? nx + 1 : 0, k = (e & 2) ? nz + 1 : 0;
+            o = cidx(nx, ny, i, j, k);
+            v = fmaxf(0.f, ci[o]);
+        } else if ((r -= 4L * ny) < 4L * nx) {
+            const int e = (int)(r / nx), i = (int)(r % nx) + 1;
+            const int j = (e & 1) ? ny + 1 : 0, k = (e & 2) ? nz + 1 : 0;
+            o = cidx(nx, ny, i, j, k);
+            v = fmaxf(0.f, ci[o]);
+        } else {
+            r -= 4L * nx;
+            const int e = (int)r;
+            const int i = (e & 1) ? nx + 1 : 0, j = (e & 2) ? ny + 1 : 0, k = (e & 4) ? nz + 1 : 0;
+            o = cidx(nx, ny, i, j, k);
+            v = fmaxf(0.f, ci[cidx(nx, ny, i ? nx : 1, j ? ny : 1, k ? nz : 1)]);
+        }
+        co[o] = v;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// FP32 step fields from the FP64 coefficients
+__global__ void k_fields3d(long N, const double *vx, const double *vy, const double *vz,
+                           const double *rho, const double *rkh1, const double *rkhor,
+                           const double *rkver, float *o)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const double r = rho[t];
+    o[t] = (float)vx[t];
+    o[N + t] = (float)vy[t];
+    o[2 * N + t] = (float)vz[t];
+    o[3 * N + t] = (float)(1.0 / r);
+    o[4 * N + t] = (float)(r * rkh1[t]);
+    o[5 * N + t] = (float)(r * rkhor[t]);
+    o[6 * N + t] = (float)(r * rkver[t]);
+}
+
+__global__ void k_fields_class(Grid g, const double *vz, const double *vset, const double *vdep,
+                               const double *rkver, const double *dz, float *vs32, float *bfac,
+                               float *tfac, double *vsum1)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (t < g.N) vs32[(long)q * g.N + t] = (float)vset[(long)q * g.N + t];
+    if (t < g.NXY) {
+        const long top = (long)(g.nz - 1) * g.NXY + t;
+        const double vs1 = vset[(long)q * g.N + t];
+        const double vd = vdep[(long)q * g.NXY + t];
+        // setbcc.f90:108-113 and 126-130
+        bfac[(long)q * g.NXY + t] =
+            ((vz[t] + vs1) >= 0.0) ? 0.f : (float)(1.0 + dz[0] * vd / rkver[t]);
+        tfac[(long)q * g.NXY + t] = ((vz[top] + vset[(long)q * g.N + top]) <= 0.0) ? 0.f : 1.f;
+        const double s = vs1 + vd;  // accum.f90:22
+        vsum1[(long)q * g.NXY + t] = s < 0.0 ? s : 0.0;
+    }
+}
+
+__global__ void k_setsrc32(Grid g, float *c, int nsou, const int *isrc, const int *jsrc,
+                           const int *ksrc, const double *tmrat, const double *zlayer,
+                           const double *dz, const double *hm1, double dt)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nc) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    for (int is = 0; is < nsou; ++is) {
+        const int i = isrc[is], j = jsrc[is], k = ksrc[is];
+        if (k >= 1 && k <= nz) {
+            const double dzeta = (k == 1) ? 0.5 * dz[k] + zlayer[k - 1] : 0.5 * (dz[k - 1] + dz[k]);
+            const double vol0 = hm1[(i - 1) + (long)nx * (j - 1)] * g.dx * g.dy * dzeta;
+            float *cc = c + (long)q * g.G + cidx(nx, ny, i, j, k);
+            *cc = (float)((double)*cc + dt * tmrat[is + (long)nsou * q] / vol0);
+        }
+    }
+}
+
+__global__ void k_to64(const float *a, double *b, long n)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        b[t] = (double)a[t];
+}
+
+__global__ void k_from64(const double *a, float *b, long n)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        b[t] = (float)a[t];
+}
+
+template <int TX, int TY>
+int launch_fused(hipStream_t s, const Params &p0)
+{
+    Params p = p0;
+    p.ntx = (p.g.nx + TX - 1) / TX;
+    p.nty = (p.g.ny + TY - 1) / TY;
+    p.nlogical = p.ntx * p.nty * p.g.nc;
+    p.per_xcd = (p.nlogical + kXCD - 1) / kXCD;
+    const unsigned nb = (unsigned)(p.per_xcd * kXCD);
+    hipLaunchKernelGGL((k_fused<TX, TY>), dim3(nb), dim3(kNT), 0, s, p);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+}  // namespace
+
+size_t fields_floats(const Grid &g)
+{
+    return (size_t)7 * g.N + (size_t)g.nc * g.N + (size_t)2 * g.nc * g.NXY;
+}
+
+Fields fields_view(const Grid &g, const float *b, const double *vsum1)
+{
+    Fields f;
+    f.vx = b;
+    f.vy = b + g.N;
+    f.vz = b + 2 * g.N;
+    f.lr = b + 3 * g.N;
+    f.pkx = b + 4 * g.N;
+    f.pky = b + 5 * g.N;
+    f.pkz = b + 6 * g.N;
+    f.vs = b + 7 * g.N;
+    f.bfac = b + 7 * g.N + (long)g.nc * g.N;
+    f.tfac = f.bfac + (long)g.nc * g.NXY;
+    f.vsum1 = vsum1;
+    return f;
+}
+
+int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *vy,
+                 const double *vz, const double *rho, const double *rkh1, const double *rkhor,
+                 const double *rkver, const double *vset, const double *vdep, const double *dz,
+                 float *f32buf, double *vsum1)
+{
+    hipLaunchKernelGGL(k_fields3d, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g.N, vx,
+                       vy, vz, rho, rkh1, rkhor, rkver, f32buf);
+    F3D_CHECK_LAUNCH();
+    Fields f = fields_view(g, f32buf, vsum1);
+    hipLaunchKernelGGL(k_fields_class, dim3((unsigned)((g.N + 255) / 256), g.nc), dim3(256), 0, s, g,
+                       vz, vset, vdep, rkver, dz, const_cast<float *>(f.vs),
+                       const_cast<float *>(f.bfac), const_cast<float *>(f.tfac), vsum1);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+int fused_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
+{
+    if (a.g.nz + 1 >= 66) {
+        set_error("fused step: nz=%d exceeds the supported 64 levels", a.g.nz);
+        return F3D_EINVAL;
+    }
+    Params p{};
+    p.g = a.g;
+    p.cin = a.cin;
+    p.cout = a.cout;
+    p.cdry = a.cdry;
+    p.f = f;
+    p.zc = a.zc;
+    p.dt64 = a.dt;
+    p.dt = (float)a.dt;
+    p.odd = (a.iiter % 2) != 0;
+    if (ts.tx == 32 && ts.ty == 8) return launch_fused<32, 8>(s, p);
+    if (ts.tx == 32 && ts.ty == 16) return launch_fused<32, 16>(s, p);
+    if (ts.tx == 64 && ts.ty == 8) return launch_fused<64, 8>(s, p);
+    if (ts.tx == 16 && ts.ty == 16) return launch_fused<16, 16>(s, p);
+    if (ts.tx == 64 && ts.ty == 4) return launch_fused<64, 4>(s, p);
+    set_error("fused step: unsupported tile %dx%d", ts.tx, ts.ty);
+    return F3D_EINVAL;
+}
+
+int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,
+                      const double *vy64, const double *vz64, const double *vset64,
+                      const double *dz, double *scratch, double *outmass)
+{
+    hipLaunchKernelGGL(k_frame, dim3(red::kPartials), dim3(256), 0, s, a.g, a.cin, a.cout, f, vx64,
+                       vy64, vz64, vset64, dz, a.dt, scratch);
+    F3D_CHECK_LAUNCH();
+    return red::finish_sum(s, scratch, red::kPartials, outmass, 1);
+}
+
+int setsrc(hipStream_t s, const Grid &g, float *c, int nsou, const int *isrc, const int *jsrc,
+           const int *ksrc, const double *tmrat, const double *zlayer, const double *dz,
+           const double *hm1, double dt)
+{
+    if (nsou <= 0) return F3D_OK;
+    hipLaunchKernelGGL(k_setsrc32, dim3((g.nc + 63) / 64), dim3(64), 0, s, g, c, nsou, isrc, jsrc,
+                       ksrc, tmrat, zlayer, dz, hm1, dt);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+int to_fp64(hipStream_t s, const float *c32, double *c64, long n)
+{
+    hipLaunchKernelGGL(k_to64, dim3(1024), dim3(256), 0, s, c32, c64, n);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+int from_fp64(hipStream_t s, const double *c64, float *c32, long n)
+{
+    hipLaunchKernelGGL(k_from64, dim3(1024), dim3(256), 0, s, c64, c32, n);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+}  // namespace f32
+}  // namespace f3d

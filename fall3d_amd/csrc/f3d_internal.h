@@ -1,0 +1,148 @@
+// Internal declarations shared by the HIP translation units of libfall3d_amd.
+//
+// Layout conventions (reference: Sources/Master.f90:92-136):
+//   c    (0:nx+1,0:ny+1,0:nz+1,nc)  column-major with a one-cell ghost frame
+//   3-D  (nx,ny,nz)                  meteo and coefficients, no ghosts
+//   vset (nx,ny,nz,nc)   vdep (nx,ny,nc)   cdry (nx,ny,0:nc)   dz (0:nz)
+// Device buffers keep exactly this layout so that host arrays of the
+// reference can be copied in and out without reshaping.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <algorithm>
+#include <cstdint>
+
+#include "../../include/fall3d_amd.h"
+
+namespace f3d {
+
+void set_error(const char *fmt, ...);
+
+#define F3D_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            ::f3d::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),       \
+                             __FILE__, __LINE__);                                         \
+            return F3D_EHIP;                                                              \
+        }                                                                                 \
+    } while (0)
+
+#define F3D_CHECK_LAUNCH()                                                                \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess) {                                                           \
+            ::f3d::set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_),   \
+                             __FILE__, __LINE__);                                         \
+            return F3D_EHIP;                                                              \
+        }                                                                                 \
+    } while (0)
+
+// Grid of one scenario as the kernels see it.
+struct Grid {
+    int nx, ny, nz, nc;
+    long G;       // ghosted slab (nx+2)(ny+2)(nz+2)
+    long N;       // nx*ny*nz
+    long NXY;     // nx*ny
+    double dx, dy;
+};
+
+__host__ __device__ inline long cidx(int nx, int ny, int i, int j, int k)
+{
+    return (long)i + (long)(nx + 2) * ((long)j + (long)(ny + 2) * (long)k);
+}
+__host__ __device__ inline long midx(int nx, int ny, int i, int j, int k)
+{
+    return (long)(i - 1) + (long)nx * ((long)(j - 1) + (long)ny * (long)(k - 1));
+}
+
+// Per-level vertical coefficients of advctzc/diffz (advctzc.f90:151-153,
+// diffz.f90:82-85), computed once on the host with IEEE division.
+struct ZCoef {
+    double dzz;    // 2/(dz(k)+dz(k-1))
+    double dzk;    // 1/dz(k)
+    double dzk_1;  // 1/dz(k-1)
+    double dzm1;   // dz(k-1)
+};
+
+// ---------------- plain FP64 kernels (f3d_plain64.hip) ----------------
+// Every launcher processes `ncls` classes in one launch; class q uses
+// c + q*G, vset + q*N, vdep + q*NXY.  Within a class the arithmetic is the
+// reference routine's, operation for operation.
+namespace p64 {
+int setbcc(hipStream_t s, const Grid &g, int ncls, double *c, const double *vx, const double *vy,
+           const double *vz, const double *vset, const double *vdep, const double *rkver,
+           const double *dz);
+int advctzc(hipStream_t s, const Grid &g, int ncls, double *c, const double *vz,
+            const double *vset, const ZCoef *zc, double dznz, double dt);
+int advctx(hipStream_t s, const Grid &g, int ncls, double *c, const double *vx, double dx,
+           double dt);
+int advcty(hipStream_t s, const Grid &g, int ncls, double *c, const double *vy, double dy,
+           double dt);
+int diffz(hipStream_t s, const Grid &g, int ncls, double *c, const ZCoef *zc, const double *ldz,
+          double dt, const double *rkver, const double *rho);
+int diffx(hipStream_t s, const Grid &g, int ncls, double *c, double dx, double dt,
+          const double *rkh, const double *rho);
+int diffy(hipStream_t s, const Grid &g, int ncls, double *c, double dy, double dt,
+          const double *rkh, const double *rho);
+// cdry slot of class q is cdry + (ic0 + q)*NXY
+int accum(hipStream_t s, const Grid &g, int ncls, const double *c, double *cdry, int ic0,
+          double dt, const double *vset, const double *vdep);
+int clip(hipStream_t s, double *c, long n);
+int setsrc(hipStream_t s, const Grid &g, double *c, int nsou, const int *isrc, const int *jsrc,
+           const int *ksrc, const double *tmrat, const double *zlayer, const double *dz,
+           const double *hm1, double dt);
+}  // namespace p64
+
+// ---------------- deterministic FP64 reductions (f3d_reduce.hip) ----------------
+namespace red {
+constexpr int kPartials = 1024;  // fixed number of per-block partials
+// outmass += cmass2d over ncls classes (cmass2d.f90:38-105); scratch holds kPartials doubles
+int cmass2d64(hipStream_t s, const Grid &g, int ncls, const double *c, const double *vx,
+              const double *vy, const double *vz, const double *vset, const double *dz,
+              double dt, double *scratch, double *outmass);
+// getc0 (endstep.f90:80-106): slot 0 of cdry/cwet = sum of the class slots
+int getc0(hipStream_t s, const Grid &g, double *cdry, double *cwet);
+// cmass3d (cmass3d.f90:24-58) without the final dx*dy scaling:
+// out[0..3] = sums for massvol, massdep, massdry, masswet.  cload0 may be
+// null, then cload(:,:,0) = cdry0 + cwet0 (endstep.f90:35).
+int cmass3d(hipStream_t s, const Grid &g, const double *c, const double *cload0,
+            const double *cdry0, const double *cwet0, const double *dz, double *scratch,
+            double *out);
+int cmass3d(hipStream_t s, const Grid &g, const float *c, const double *cload0,
+            const double *cdry0, const double *cwet0, const double *dz, double *scratch,
+            double *out);
+// final stage: dst[0] += sum(partials[0..n)) in a fixed order (one block)
+int finish_sum(hipStream_t s, const double *partials, int n, double *dst, int accumulate);
+}  // namespace red
+
+// ---------------- meteo stage (f3d_meteo.hip), FP64 ----------------
+namespace met {
+int kappa3(hipStream_t s, const Grid &g, int modkv, int modkh, const double *dz,
+           const double *zlayer, const double *vx, const double *vy, const double *vptemp,
+           const double *zi, const double *rl, const double *ustar, const double *hm1,
+           const double *kmh, double dx_num, double dy_num, double rkv0, double rkh0,
+           double csh, int lonlat, double *rkver, double *rkhor);
+// Per-class constants prepared on the host: rk1, rk2 (Ganser) and psi.
+struct ClassConst {
+    double diam, rhop, psi, rk1, rk2;
+};
+// setvset chain (setvset.f90:25-60) with warm-started chunks.
+// visc is an N-double scratch for the per-cell Sutherland viscosity;
+// cd_io[0] is the carried drag coefficient, cd_io[1] receives the last one.
+int setvset(hipStream_t s, const Grid &g, int np, int modv, const ClassConst *cls,
+            const double *tempe, const double *rho, double *visc, double *vset, double *cd_io,
+            int *status, int chunk, int warm);
+int scale(hipStream_t s, const Grid &g, const double *hm1, double *vx, double *rho,
+          const double *rkhor, double *rkh1);
+// dtcn = max over cells and classes of the CFL rate (finddt.f90:26-48);
+// dtcn_bits is a uint64 holding the bit pattern of a non-negative double.
+int finddt(hipStream_t s, const Grid &g, const double *dz, double dX1, double dX2,
+           const double *vx, const double *vy, const double *vz, const double *rkh1,
+           const double *rkhor, const double *rkver, const double *vset,
+           unsigned long long *dtcn_bits);
+}  // namespace met
+
+}  // namespace f3d

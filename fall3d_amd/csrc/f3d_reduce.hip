@@ -1,0 +1,215 @@
+// Deterministic FP64 reductions for the mass budgets.
+//
+// Every reduction runs on a fixed grid of red::kPartials blocks: each thread
+// sums its grid-stride share in index order, each block folds its threads by
+// a fixed LDS tree into one partial, and a single-block kernel folds the
+// partials by a fixed tree.  The order depends only on the problem size, so
+// a scenario's budget is bit-identical whatever else runs on the device.
+// The summation order differs from the reference's serial loops (cmass2d.f90,
+// cmass3d.f90), which changes the sums by O(1e-16) relative.
+#include "f3d_internal.h"
+
+namespace f3d {
+namespace {
+
+constexpr int kBS = 256;
+
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// cmass2d.f90:38-105: one "site" per x-face (j,k) pair, y-face (i,k) pair
+// and top cell (i,j), for every class.  Terms are dt*area*velo*conc in the
+// reference's association; influx terms enter with a minus sign.
+__global__ void __launch_bounds__(kBS) k_cmass2d64(Grid g, int ncls, const double *c,
+                                                    const double *vx, const double *vy,
+                                                    const double *vz, const double *vset,
+                                                    const double *dz, double dt, double *part)
+{
+    __shared__ double sh[kBS];
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long nxf = (long)ny * nz, nyf = (long)nx * nz, ntop = g.NXY;
+    const long per = nxf + nyf + ntop;
+    const long total = per * ncls;
+    double acc = 0.0;
+    for (long t = (long)blockIdx.x * kBS + threadIdx.x; t < total; t += (long)gridDim.x * kBS) {
+        const int q = (int)(t / per);
+        long r = t - (long)q * per;
+        const double *cc = c + q * g.G;
+        if (r < nxf) {
+            const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
+            const double faty = (j == 1 || j == ny) ? 0.5 : 1.0;
+            const double area = faty * g.dy * 0.5 * (dz[k] + dz[k - 1]);
+            acc += -(dt * area * vx[midx(nx, ny, 1, j, k)] * cc[cidx(nx, ny, 0, j, k)]);
+            acc += dt * area * vx[midx(nx, ny, nx, j, k)] * cc[cidx(nx, ny, nx + 1, j, k)];
+            continue;
+        }
+        r -= nxf;
+        if (r < nyf) {
+            const int i = (int)(r % nx) + 1, k = (int)(r / nx) + 1;
+            const double fatx = (i == 1 || i == nx) ? 0.5 : 1.0;
+            const double area = fatx * g.dx * 0.5 * (dz[k] + dz[k - 1]);
+            acc += -(dt * area * vy[midx(nx, ny, i, 1, k)] * cc[cidx(nx, ny, i, 0, k)]);
+            acc += dt * area * vy[midx(nx, ny, i, ny, k)] * cc[cidx(nx, ny, i, ny + 1, k)];
+            continue;
+        }
+        r -= nyf;
+        {
+            const int i = (int)(r % nx) + 1, j = (int)(r / nx) + 1;
+            const double fatx = (i == 1 || i == nx) ? 0.5 : 1.0;
+            const double faty = (j == 1 || j == ny) ? 0.5 : 1.0;
+            const double area = fatx * faty * g.dx * g.dy;
+            const double velo = vz[midx(nx, ny, i, j, nz)] + vset[q * g.N + midx(nx, ny, i, j, nz)];
+            acc += dt * area * velo * cc[cidx(nx, ny, i, j, nz + 1)];
+        }
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// getc0, endstep.f90:80-106: slot 0 = sum of the particle slots, in order
+__global__ void k_getc0(Grid g, double *cdry, double *cwet)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.NXY) return;
+    double d = 0.0, w = 0.0;
+    for (int ic = 1; ic <= g.nc; ++ic) {
+        d = d + cdry[t + ic * g.NXY];
+        w = w + cwet[t + ic * g.NXY];
+    }
+    cdry[t] = d;
+    cwet[t] = w;
+}
+
+// cmass3d.f90:24-58 with cload(:,:,0) = cdry + cwet (endstep.f90:35) when
+// cload0 is null; four partials per block: vol, dep, dry, wet.
+template <typename T>
+__global__ void __launch_bounds__(kBS) k_cmass3d(Grid g, const T *c, const double *cload0,
+                                                 const double *cdry0, const double *cwet0,
+                                                 const double *dz, double *part)
+{
+    __shared__ double sh[kBS];
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long SZ = (long)(nx + 2) * (ny + 2);
+    double av = 0.0, ad = 0.0, adr = 0.0, aw = 0.0;
+    for (long t = (long)blockIdx.x * kBS + threadIdx.x; t < g.NXY; t += (long)gridDim.x * kBS) {
+        const int i = (int)(t % nx) + 1, j = (int)(t / nx) + 1;
+        const double fatx = (i == 1 || i == nx) ? 0.5 : 1.0;
+        const double faty = (j == 1 || j == ny) ? 0.5 : 1.0;
+        const double ff = fatx * faty;
+        for (int q = 0; q < g.nc; ++q) {
+            const T *cc = c + q * g.G + cidx(nx, ny, i, j, 0);
+            double zmass = 0.0;
+            for (int k = 1; k <= nz - 1; ++k)
+                zmass = zmass + 0.5 * dz[k] * ((double)cc[k * SZ] + (double)cc[(k + 1) * SZ]);
+            av += ff * zmass;
+        }
+        const double d = cdry0[t], w = cwet0[t];
+        const double cl = cload0 ? cload0[t] : d + w;
+        ad += ff * cl;
+        adr += ff * d;
+        aw += ff * w;
+    }
+    const double sv = block_sum(av, sh);
+    const double sd = block_sum(ad, sh);
+    const double sdr = block_sum(adr, sh);
+    const double sw = block_sum(aw, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sv;
+        part[blockIdx.x + gridDim.x] = sd;
+        part[blockIdx.x + 2 * gridDim.x] = sdr;
+        part[blockIdx.x + 3 * gridDim.x] = sw;
+    }
+}
+
+// dst[m] (+)= sum of part[m*n .. m*n+n), for m = 0..nsum-1, fixed tree
+__global__ void __launch_bounds__(1024) k_finish(const double *part, int n, int nsum, double *dst,
+                                                 int accumulate, double scale)
+{
+    __shared__ double sh[1024];
+    for (int m = 0; m < nsum; ++m) {
+        double v = 0.0;
+        for (int t = threadIdx.x; t < n; t += 1024) v += part[(long)m * n + t];
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int s = 512; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const double r = sh[0] * scale;
+            dst[m] = accumulate ? dst[m] + r : r;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+namespace red {
+
+int cmass2d64(hipStream_t s, const Grid &g, int ncls, const double *c, const double *vx,
+              const double *vy, const double *vz, const double *vset, const double *dz,
+              double dt, double *scratch, double *outmass)
+{
+    hipLaunchKernelGGL(k_cmass2d64, dim3(kPartials), dim3(kBS), 0, s, g, ncls, c, vx, vy, vz,
+                       vset, dz, dt, scratch);
+    F3D_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, scratch, kPartials, 1, outmass, 1, 1.0);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+int finish_sum(hipStream_t s, const double *partials, int n, double *dst, int accumulate)
+{
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, partials, n, 1, dst, accumulate, 1.0);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+int getc0(hipStream_t s, const Grid &g, double *cdry, double *cwet)
+{
+    hipLaunchKernelGGL(k_getc0, dim3((unsigned)((g.NXY + 255) / 256)), dim3(256), 0, s, g, cdry,
+                       cwet);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+template <typename T>
+static int cmass3d_impl(hipStream_t s, const Grid &g, const T *c, const double *cload0,
+                        const double *cdry0, const double *cwet0, const double *dz,
+                        double *scratch, double *out)
+{
+    hipLaunchKernelGGL(k_cmass3d<T>, dim3(kPartials), dim3(kBS), 0, s, g, c, cload0, cdry0, cwet0,
+                       dz, scratch);
+    F3D_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, scratch, kPartials, 4, out, 0, 1.0);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+int cmass3d(hipStream_t s, const Grid &g, const double *c, const double *cload0,
+            const double *cdry0, const double *cwet0, const double *dz, double *scratch,
+            double *out)
+{
+    return cmass3d_impl<double>(s, g, c, cload0, cdry0, cwet0, dz, scratch, out);
+}
+
+int cmass3d(hipStream_t s, const Grid &g, const float *c, const double *cload0,
+            const double *cdry0, const double *cwet0, const double *dz, double *scratch,
+            double *out)
+{
+    return cmass3d_impl<float>(s, g, c, cload0, cdry0, cwet0, dz, scratch, out);
+}
+
+}  // namespace red
+}  // namespace f3d

@@ -1,0 +1,287 @@
+"""ctypes binding of libfall3d_amd.so (the C ABI in include/fall3d_amd.h).
+
+`Context` wraps one device-resident f3d_ctx.  `Run` mirrors the reference
+driver (Sources/Fall3d.f90:79-121) over a `Scenario`: one meteo read
+(meteo.f90), then steps (setsrc + sizloop), and budgets (endstep/writim).
+
+The library must be built (`python -m fall3d_amd.build`); importing this
+module does not fall back to anything when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfall3d_amd.so")
+
+FP64, FP32 = 0, 1
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_L = None
+
+# every symbol include/fall3d_amd.h declares
+DROPIN_SYMBOLS = ["setbcc_", "advctzc_", "advctx_", "advcty_", "diffz_", "diffx_", "diffy_",
+                  "accum_", "cmass2d_", "cmass3d_"]
+CTX_SYMBOLS = ["f3d_create", "f3d_destroy", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
+               "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
+               "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_get_clock",
+               "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
+               "f3d_set_tile", "f3d_timing", "f3d_kernel_time", "f3d_device_bytes",
+               "f3d_last_error"]
+
+
+class F3DError(RuntimeError):
+    pass
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("nc", ctypes.c_int), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("zlayer", _dp), ("hm1", _dp), ("lonlat", ctypes.c_int)]
+
+
+def lib():
+    """Load libfall3d_amd.so; raises if it has not been built."""
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            raise F3DError(f"{LIB_PATH} is not built: run `python -m fall3d_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        c, d, i, dp, ip, vp = (ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _ip,
+                               ctypes.c_void_p)
+        L.f3d_last_error.restype = ctypes.c_char_p
+        L.f3d_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(_Grid), i]
+        L.f3d_destroy.argtypes = [vp]
+        L.f3d_destroy.restype = None
+        L.f3d_set_classes.argtypes = [vp, dp, dp, dp, i]
+        L.f3d_set_models.argtypes = [vp, i, i, d, d, d]
+        L.f3d_set_meteo.argtypes = [vp] + [dp] * 10
+        L.f3d_get_dt.argtypes = [vp, dp, dp]
+        L.f3d_set_dt.argtypes = [vp, d]
+        L.f3d_set_source.argtypes = [vp, i, ip, ip, ip, dp, d]
+        L.f3d_step.argtypes = [vp, i]
+        L.f3d_budget.argtypes = [vp, dp]
+        L.f3d_get_state.argtypes = [vp, dp, dp]
+        L.f3d_set_state.argtypes = [vp, dp, dp]
+        L.f3d_get_coef.argtypes = [vp] + [dp] * 6
+        L.f3d_get_clock.argtypes = [vp, dp, ip, dp, dp]
+        L.f3d_set_clock.argtypes = [vp, d, i, d, d]
+        L.f3d_get_cd_state.argtypes = [vp, dp]
+        L.f3d_set_cd_state.argtypes = [vp, d]
+        L.f3d_sync.argtypes = [vp]
+        L.f3d_set_tile.argtypes = [vp, i, i]
+        L.f3d_timing.argtypes = [vp, i]
+        L.f3d_kernel_time.argtypes = [vp, i, dp, ctypes.POINTER(ctypes.c_long)]
+        L.f3d_device_bytes.argtypes = [vp]
+        L.f3d_device_bytes.restype = ctypes.c_longlong
+        for name in ("f3d_create", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
+                     "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
+                     "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_get_clock",
+                     "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
+                     "f3d_set_tile", "f3d_timing", "f3d_kernel_time"):
+            getattr(L, name).restype = c
+        _L = L
+    return _L
+
+
+def last_error():
+    return lib().f3d_last_error().decode(errors="replace")
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise F3DError(f"{what} failed ({rc}): {last_error()}")
+
+
+def _F(a, dtype=np.float64):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def _P(a):
+    if a is None:
+        return None
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_ip)
+    assert a.dtype == np.float64
+    assert a.flags.f_contiguous if a.ndim > 1 else a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """One device-resident scenario (f3d_ctx)."""
+
+    def __init__(self, nx, ny, nz, nc, dx, dy, zlayer, precision=FP32, hm1=None, lonlat=0):
+        L = lib()
+        self.nx, self.ny, self.nz, self.nc = nx, ny, nz, nc
+        self.precision = precision
+        self._zl = _F(zlayer)
+        self._hm1 = None if hm1 is None else _F(hm1)
+        g = _Grid(nx, ny, nz, nc, float(dx), float(dy), _P(self._zl), _P(self._hm1), lonlat)
+        h = ctypes.c_void_p()
+        _check(L.f3d_create(ctypes.byref(h), ctypes.byref(g), precision), "f3d_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().f3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_classes(self, diam, rhop, sphe, modv):
+        self._cls = [_F(diam), _F(rhop), _F(sphe)]
+        _check(lib().f3d_set_classes(self.h, *[_P(a) for a in self._cls], int(modv)),
+               "f3d_set_classes")
+
+    def set_models(self, modkv, modkh, rkv0, rkh0, csh=0.2275):
+        _check(lib().f3d_set_models(self.h, int(modkv), int(modkh), float(rkv0), float(rkh0),
+                                    float(csh)), "f3d_set_models")
+
+    def set_meteo(self, vx, vy, vz, tempe, vptemp, rho, ustar, zi, rl, vdep):
+        arrs = [_F(a) for a in (vx, vy, vz, tempe, vptemp, rho, ustar, zi, rl, vdep)]
+        _check(lib().f3d_set_meteo(self.h, *[_P(a) for a in arrs]), "f3d_set_meteo")
+        return self.dt
+
+    @property
+    def dt(self):
+        dt, dtc = ctypes.c_double(), ctypes.c_double()
+        _check(lib().f3d_get_dt(self.h, ctypes.byref(dt), ctypes.byref(dtc)), "f3d_get_dt")
+        return dt.value
+
+    @property
+    def dtc(self):
+        dt, dtc = ctypes.c_double(), ctypes.c_double()
+        _check(lib().f3d_get_dt(self.h, ctypes.byref(dt), ctypes.byref(dtc)), "f3d_get_dt")
+        return dtc.value
+
+    def set_dt(self, dt):
+        _check(lib().f3d_set_dt(self.h, float(dt)), "f3d_set_dt")
+
+    def set_source(self, isrc, jsrc, ksrc, tmrat, flowrate):
+        self._src = [np.ascontiguousarray(a, dtype=np.int32) for a in (isrc, jsrc, ksrc)]
+        self._tm = _F(tmrat)
+        _check(lib().f3d_set_source(self.h, len(self._src[0]), *[_P(a) for a in self._src],
+                                    _P(self._tm), float(flowrate)), "f3d_set_source")
+
+    def step(self, n=1):
+        _check(lib().f3d_step(self.h, int(n)), "f3d_step")
+
+    def sync(self):
+        _check(lib().f3d_sync(self.h), "f3d_sync")
+
+    def budget(self):
+        out = np.zeros(6)
+        _check(lib().f3d_budget(self.h, _P(out)), "f3d_budget")
+        keys = ("massvol", "massdep", "massdry", "masswet", "outmass", "erumass")
+        return dict(zip(keys, out.tolist()))
+
+    def state(self):
+        nx, ny, nz, nc = self.nx, self.ny, self.nz, self.nc
+        c = np.zeros((nx + 2, ny + 2, nz + 2, nc), order="F")
+        cdry = np.zeros((nx, ny, nc + 1), order="F")
+        _check(lib().f3d_get_state(self.h, _P(c), _P(cdry)), "f3d_get_state")
+        return c, cdry
+
+    def set_state(self, c=None, cdry=None):
+        c = None if c is None else _F(c)
+        cdry = None if cdry is None else _F(cdry)
+        _check(lib().f3d_set_state(self.h, _P(c), _P(cdry)), "f3d_set_state")
+
+    def coef(self):
+        nx, ny, nz, nc = self.nx, self.ny, self.nz, self.nc
+        a = [np.zeros((nx, ny, nz), order="F") for _ in range(5)]
+        vset = np.zeros((nx, ny, nz, nc), order="F")
+        _check(lib().f3d_get_coef(self.h, *[_P(x) for x in a], _P(vset)), "f3d_get_coef")
+        return dict(vx=a[0], rho=a[1], rkhor=a[2], rkh1=a[3], rkver=a[4], vset=vset)
+
+    def clock(self):
+        t, it, em, om = ctypes.c_double(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().f3d_get_clock(self.h, ctypes.byref(t), ctypes.byref(it), ctypes.byref(em),
+                                   ctypes.byref(om)), "f3d_get_clock")
+        return dict(time=t.value, iiter=it.value, erumass=em.value, outmass=om.value)
+
+    def set_clock(self, time, iiter, erumass, outmass):
+        _check(lib().f3d_set_clock(self.h, float(time), int(iiter), float(erumass),
+                                   float(outmass)), "f3d_set_clock")
+
+    @property
+    def cd_state(self):
+        v = ctypes.c_double()
+        _check(lib().f3d_get_cd_state(self.h, ctypes.byref(v)), "f3d_get_cd_state")
+        return v.value
+
+    @cd_state.setter
+    def cd_state(self, v):
+        _check(lib().f3d_set_cd_state(self.h, float(v)), "f3d_set_cd_state")
+
+    def set_tile(self, tx, ty):
+        _check(lib().f3d_set_tile(self.h, int(tx), int(ty)), "f3d_set_tile")
+
+    def timing(self, enable=True):
+        _check(lib().f3d_timing(self.h, 1 if enable else 0), "f3d_timing")
+
+    def kernel_time(self, reset=False):
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        _check(lib().f3d_kernel_time(self.h, 1 if reset else 0, ctypes.byref(ms),
+                                     ctypes.byref(n)), "f3d_kernel_time")
+        return ms.value, n.value
+
+    @property
+    def device_bytes(self):
+        return int(lib().f3d_device_bytes(self.h))
+
+
+class Run:
+    """Driver over one device context with the reference's control flow:
+    meteo read, then time += dt; iiter += 1; erumass += flowrate*dt;
+    setsrc; sizloop (Fall3d.f90:79-121)."""
+
+    def __init__(self, sc, precision=FP32, cd_state=1.0):
+        self.sc = sc
+        self.ctx = Context(sc.nx, sc.ny, sc.nz, sc.nc, sc.dx, sc.dy, sc.zlayer, precision)
+        self.ctx.set_classes(sc.diam, sc.rhop, sc.sphe, sc.modv)
+        self.ctx.set_models(sc.modkv, sc.modkh, sc.rkv0, sc.rkh0, sc.csh)
+        self.ctx.set_source(sc.isrc, sc.jsrc, sc.ksrc, sc.tmrat, sc.flowrate)
+        self.ctx.cd_state = cd_state
+        self.nsteps = 0
+
+    def meteo_update(self):
+        sc = self.sc
+        return self.ctx.set_meteo(sc.vx, sc.vy, sc.vz, sc.tempe, sc.vptemp, sc.rho, sc.ustar,
+                                  sc.zi, sc.rl, sc.vdep)
+
+    @property
+    def dt(self):
+        return self.ctx.dt
+
+    def step(self, n=1):
+        self.ctx.step(n)
+        self.nsteps += n
+
+    def run_to(self, end_time):
+        """Steps until time >= end_time (Fall3d.f90:120; the last step may overshoot)."""
+        dt = self.ctx.dt
+        t = self.ctx.clock()["time"]
+        n = 0
+        while t < end_time:
+            t += dt
+            n += 1
+        if n:
+            self.step(n)
+        return n
+
+    def state(self):
+        return self.ctx.state()
+
+    def budget(self):
+        return self.ctx.budget()
+
+    def close(self):
+        self.ctx.close()

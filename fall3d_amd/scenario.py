@@ -96,6 +96,24 @@ class Scenario:
     def updates_per_step(self) -> int:
         return self.nx * self.ny * self.nz * self.nc
 
+    def write_binary(self, path):
+        """Raw little-endian stream read by the Fortran driver
+        (fall3d_amd/fortran/f3d_driver.f90)."""
+        i4 = lambda *v: np.asarray(v, dtype="<i4").tobytes()  # noqa: E731
+        f8 = lambda a: np.asarray(a, dtype="<f8").ravel(order="F").tobytes()  # noqa: E731
+        with open(path, "wb") as fh:
+            fh.write(i4(self.nx, self.ny, self.nz, self.nc, self.nsou, self.modv, self.modkv,
+                        self.modkh))
+            fh.write(f8([self.dx, self.dy, self.rkv0, self.rkh0, self.csh, self.flowrate,
+                         self.end_time]))
+            for a in (self.zlayer, self.diam, self.rhop, self.sphe, self.vx, self.vy, self.vz,
+                      self.tempe, self.vptemp, self.rho, self.ustar, self.zi, self.rl, self.vdep):
+                fh.write(f8(a))
+            if self.nsou:
+                for a in (self.isrc, self.jsrc, self.ksrc):
+                    fh.write(np.asarray(a, dtype="<i4").tobytes())
+                fh.write(f8(self.tmrat))
+
 
 def _q32(a) -> np.ndarray:
     return np.asfortranarray(np.asarray(a, dtype=np.float32).astype(np.float64))

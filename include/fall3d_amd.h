@@ -1,0 +1,165 @@
+/*
+ * fall3d_amd: MI355X-native transport core for FALL3D-7.0 (C ABI).
+ *
+ * Two layers, both plain C: pointers, sizes, no device or C++ types.
+ *
+ * 1. Per-routine drop-ins with the reference's own external-procedure ABI
+ *    (lower case + one trailing underscore, every argument by reference,
+ *    Fortran column-major arrays, no hidden lengths).  A reference build
+ *    that links libfall3d_amd.so ahead of its own objects gets the GPU
+ *    versions of these routines without a source change.  They stage host
+ *    arrays to the device, run the plain FP64 kernel, and copy back.  Only
+ *    the serial decomposition (nz_loc == nz, iz_loc == 1; ic_loc = 1,
+ *    nc_loc = nc) is accepted; anything else leaves the outputs untouched
+ *    and sets f3d_last_error().
+ *
+ * 2. A device-resident context (f3d_*) that holds grid, meteorology,
+ *    coefficients, concentration and budgets in HBM and runs the whole
+ *    per-step transport core there: the calls the driver loop of
+ *    Sources/Fall3d.f90:79-121 makes (meteo, setsrc, sizloop, endstep
+ *    budgets).  Precision F3D_FP64 replays the reference routine by
+ *    routine in FP64; F3D_FP32 runs one fused FP32 kernel per step over
+ *    all classes with FP64 deposit, budgets, CFL step and settling.
+ *
+ * Errors: every int-returning call returns 0 on success and a negative
+ * code otherwise (F3D_E*); f3d_last_error() gives the message.  All calls
+ * on one context must come from one host thread at a time.
+ */
+#ifndef FALL3D_AMD_H
+#define FALL3D_AMD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    F3D_OK = 0,
+    F3D_EINVAL = -1,   /* bad argument, size or unsupported decomposition */
+    F3D_EHIP = -2,     /* HIP runtime error (message carries the HIP error) */
+    F3D_ENOMEM = -3,   /* device allocation failed */
+    F3D_ECONV = -4,    /* vsettl / get_gama did not converge (runend in the reference) */
+    F3D_ESTATE = -5    /* call out of order (e.g. step before meteo) */
+};
+
+enum { F3D_FP64 = 0, F3D_FP32 = 1 };
+
+/* ---------------- layer 1: reference-ABI drop-ins (FP64) ---------------- */
+
+/* Sources/setbcc.f90:1-13 */
+void setbcc_(double *c, int *nx, int *ny, int *nz, int *nz_loc, int *iz_loc, const double *vx,
+             const double *vy, const double *vz, const double *vset, const double *vdep,
+             const double *rkver, const double *dz);
+/* Sources/advctzc.f90:1-17 (lmeth unused, as in the reference) */
+void advctzc_(double *c, int *nx, int *ny, int *nz, int *nz_loc, int *iz_loc, const double *vz,
+              const double *vset, const double *dz, double *dt, int *lmeth);
+/* Sources/advctx.f90:1-19 (work(nx+1,2) is not touched) */
+void advctx_(double *c, int *nx, int *ny, int *nz, int *nz_loc, int *iz_loc, const double *vx,
+             double *dx, double *dt, double *work, int *lmeth);
+/* Sources/advcty.f90:1-19 */
+void advcty_(double *c, int *nx, int *ny, int *nz, int *nz_loc, int *iz_loc, const double *vy,
+             double *dy, double *dt, double *work, int *lmeth);
+/* Sources/diffz.f90:1-13 */
+void diffz_(double *c, int *nx, int *ny, int *nz, int *nz_loc, int *iz_loc, const double *dz,
+            double *dt, const double *rkver, const double *rho, double *work);
+/* Sources/diffx.f90:1-13 (called with rkh1 by sizloop.f90:56,60) */
+void diffx_(double *c, int *nx, int *ny, int *nz, int *nz_loc, int *iz_loc, double *dx,
+            double *dt, const double *rkhor, const double *rho, double *work);
+/* Sources/diffy.f90:1-13 (called with rkhor by sizloop.f90:57,59) */
+void diffy_(double *c, int *nx, int *ny, int *nz, int *nz_loc, int *iz_loc, double *dy,
+            double *dt, const double *rkhor, const double *rho, double *work);
+/* Sources/accum.f90:1-15; cdry is (nx,ny,0:*) and ic selects the slot */
+void accum_(const double *c, double *cdry, int *ic, int *nx, int *ny, int *nz, int *nz_loc,
+            int *iz_loc, const double *vz, double *dt, const double *vset, const double *vdep);
+/* Sources/cmass2d.f90:1-30; outmass accumulates */
+void cmass2d_(const double *c, const double *vx, const double *vy, const double *vz,
+              const double *vset, int *nx, int *ny, int *nz, int *nc, int *ic_loc, int *nc_loc,
+              int *iz_loc, int *nz_loc, double *dx, double *dy, const double *dz, double *dt,
+              double *outmass);
+/* Sources/cmass3d.f90:1-24 */
+void cmass3d_(const double *c, const double *cload, const double *cdry, const double *cwet,
+              int *nx, int *ny, int *nz, int *nc, double *dx, double *dy, const double *dz,
+              double *massvol, double *massdep, double *massdry, double *masswet);
+
+/* ---------------- layer 2: device-resident context ---------------- */
+
+typedef struct f3d_ctx f3d_ctx;
+
+typedef struct {
+    int nx, ny, nz, nc;          /* grid (Numeric.f90:18) and classes (Master.f90:44) */
+    double dx, dy;               /* dX1, dX2 in m (reagrd.f90:194-200) */
+    const double *zlayer;        /* (nz) layer heights; dz follows reagrd.f90:216-220 */
+    const double *hm1;           /* (nx,ny) scale factor, NULL for UTM (Hm1 = 1) */
+    int lonlat;                  /* 0 UTM, 1 LON-LAT (CMAQ constant only) */
+} f3d_grid;
+
+/* Allocates every device buffer (Master.f90:getmem1 equivalent). */
+int f3d_create(f3d_ctx **ctx, const f3d_grid *grid, int precision);
+void f3d_destroy(f3d_ctx *ctx);
+
+/* Classes and settling model (reagrn.f90:20-87, readat.f90:218-248).
+ * modv: 1 Arastoopour, 2 Ganser, 3 Wilson, 4 Dellino. */
+int f3d_set_classes(f3d_ctx *ctx, const double *diam, const double *rhop, const double *sphe,
+                    int modv);
+/* Turbulence models (readat.f90:251-324): modkv 0 CONSTANT, 1 SIMILARITY,
+ * 2 SURFACE_LAYER; modkh 0 CONSTANT, 1 RAMS, 2 CMAQ. */
+int f3d_set_models(f3d_ctx *ctx, int modkv, int modkh, double rkv0, double rkh0, double csh);
+
+/* Meteo read (meteo.f90:19-149): uploads the fields (Fortran order, host)
+ * and runs kappa3, setvset, the Hm1 scaling and finddt on the device.
+ * 3-D fields are (nx,ny,nz); ustar, zi, rl are (nx,ny); vdep is (nx,ny,nc)
+ * (the setvdrydep output).  vz(:,:,1) must already be 0 (reamet.f90:153). */
+int f3d_set_meteo(f3d_ctx *ctx, const double *vx, const double *vy, const double *vz,
+                  const double *tempe, const double *vptemp, const double *rho,
+                  const double *ustar, const double *zi, const double *rl, const double *vdep);
+int f3d_get_dt(f3d_ctx *ctx, double *dt, double *dtc);
+/* Override dt (the reference keeps dt until the next meteo read). */
+int f3d_set_dt(f3d_ctx *ctx, double dt);
+
+/* Source nodes (source.f90, 1-based indices) with tmrat(nsou,nc) in kg/s,
+ * already multiplied by Hm1 (source.f90:75-77); flowrate for erumass. */
+int f3d_set_source(f3d_ctx *ctx, int nsou, const int *isrc, const int *jsrc, const int *ksrc,
+                   const double *tmrat, double flowrate);
+
+/* nsteps passes of the loop body Fall3d.f90:112-117: time += dt,
+ * iiter += 1, erumass += flowrate*dt, setsrc, sizloop (with accum, clip and
+ * cmass2d).  Asynchronous on the context's stream. */
+int f3d_step(f3d_ctx *ctx, int nsteps);
+
+/* getc0 + cload + cmass3d (endstep.f90:35,80-106; writim.f90:18).
+ * out[6] = massvol, massdep, massdry, masswet, outmass, erumass. */
+int f3d_budget(f3d_ctx *ctx, double *out);
+
+/* State transfer, FP64 host arrays in reference layout:
+ * c (0:nx+1,0:ny+1,0:nz+1,nc), cdry (nx,ny,0:nc) (slot 0 as last set by
+ * f3d_budget).  Either pointer may be NULL. */
+int f3d_get_state(f3d_ctx *ctx, double *c, double *cdry);
+int f3d_set_state(f3d_ctx *ctx, const double *c, const double *cdry);
+/* Coefficients after the meteo update (scaled as the reference holds them):
+ * vx/Hm1, rho*Hm1, rkhor, rkh1, rkver (nx,ny,nz) and vset (nx,ny,nz,nc). */
+int f3d_get_coef(f3d_ctx *ctx, double *vx, double *rho, double *rkhor, double *rkh1,
+                 double *rkver, double *vset);
+int f3d_get_clock(f3d_ctx *ctx, double *time, int *iiter, double *erumass, double *outmass);
+int f3d_set_clock(f3d_ctx *ctx, double time, int iiter, double erumass, double outmass);
+/* The SAVEd vsettl drag coefficient carried between meteo reads
+ * (setvset.f90:89); starts at 1.0. */
+int f3d_get_cd_state(f3d_ctx *ctx, double *cd);
+int f3d_set_cd_state(f3d_ctx *ctx, double cd);
+
+int f3d_sync(f3d_ctx *ctx);
+/* Tile shape (columns x rows) of the fused FP32 step kernel.  No
+ * counterpart in the reference: a tuning knob set by the autotuner.
+ * Supported: 32x8 (default), 32x16, 64x8, 64x4, 16x16. */
+int f3d_set_tile(f3d_ctx *ctx, int tx, int ty);
+/* HIP-event timing of the step kernels (enabled by f3d_timing(ctx,1)):
+ * average duration in ms and launch count of the dominant step kernel
+ * since the last reset. */
+int f3d_timing(f3d_ctx *ctx, int enable);
+int f3d_kernel_time(f3d_ctx *ctx, int reset, double *avg_ms, long *count);
+/* Device bytes held by the context. */
+long long f3d_device_bytes(const f3d_ctx *ctx);
+
+const char *f3d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,221 @@
+"""GPU parity of libfall3d_amd against the reference (golden vectors) and the
+oracle, through the C ABI.
+
+Tolerances:
+* per-routine FP64 drop-ins: bit-exact (same operation order, no
+  contraction, IEEE division);
+* FP64 context: coefficients without transcendental calls bit-exact;
+  settling velocity (device pow) and everything downstream within 1e-11
+  relative L-infinity;
+* FP32 fused context: relative L1 of concentration and ground load
+  <= 1e-4, mass budget terms <= 1e-4 relative.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from fall3d_amd import lib as F
+from fall3d_amd import scenario as S
+from oracle.oracle import OracleRun
+from tests.golden_util import GOLDEN, load, run_fixtures, scenario_from
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_linf(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    nan_a, nan_b = np.isnan(a), np.isnan(b)
+    assert np.array_equal(nan_a, nan_b), "NaN pattern differs"
+    m = ~nan_a
+    den = np.max(np.abs(b[m])) if m.any() else 0.0
+    return 0.0 if den == 0 else float(np.max(np.abs(a[m] - b[m])) / den)
+
+
+def rel_l1(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    den = np.sum(np.abs(b))
+    return 0.0 if den == 0 else float(np.sum(np.abs(a - b)) / den)
+
+
+def interior(c):
+    return c[1:-1, 1:-1, 1:-1, :]
+
+
+# ---------------- per-routine drop-ins: bit-exact ----------------
+
+def test_dropins_bitexact_vs_reference_routines():
+    g = load(os.path.join(GOLDEN, "routines_9x7x6.npz"))
+    L = F.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+
+    def P(a):
+        return a.ctypes.data_as(dp)
+
+    def i(x):
+        return ctypes.byref(ctypes.c_int(x))
+
+    def d(x):
+        return ctypes.byref(ctypes.c_double(x))
+
+    A = {k: np.asfortranarray(g[k]) for k in g.files}
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    dx, dy, dt = float(g["dx"]), float(g["dy"]), float(g["dt"])
+    args = (i(nx), i(ny), i(nz), i(nz), i(1))
+    c = A["c0"].copy(order="F")
+    L.setbcc_(P(c), *args, P(A["vx"]), P(A["vy"]), P(A["vz"]), P(A["vset"]), P(A["vdep"]),
+              P(A["rkver"]), P(A["dz"]))
+    assert np.array_equal(c, g["setbcc"]), F.last_error()
+    cb = A["setbcc"]
+    work = np.zeros(64)
+    calls = {
+        "advctzc": lambda c: L.advctzc_(P(c), *args, P(A["vz"]), P(A["vset"]), P(A["dz"]), d(dt), i(1)),
+        "advctx": lambda c: L.advctx_(P(c), *args, P(A["vx"]), d(dx), d(dt), P(work), i(1)),
+        "advcty": lambda c: L.advcty_(P(c), *args, P(A["vy"]), d(dy), d(dt), P(work), i(1)),
+        "diffz": lambda c: L.diffz_(P(c), *args, P(A["dz"]), d(dt), P(A["rkver"]), P(A["rho"]), P(work)),
+        "diffx": lambda c: L.diffx_(P(c), *args, d(dx), d(dt), P(A["rkh"]), P(A["rho"]), P(work)),
+        "diffy": lambda c: L.diffy_(P(c), *args, d(dy), d(dt), P(A["rkh"]), P(A["rho"]), P(work)),
+    }
+    for name, call in calls.items():
+        c = cb.copy(order="F")
+        call(c)
+        assert np.array_equal(c, g[name]), (name, F.last_error(), np.abs(c - g[name]).max())
+    cdry = A["cdry0"].copy(order="F")
+    L.accum_(P(cb), P(cdry), i(2), i(nx), i(ny), i(nz), i(nz), i(1), P(A["vz"]), d(dt),
+             P(A["vset"]), P(A["vdep"]))
+    assert np.array_equal(cdry, g["accum"])
+    om = ctypes.c_double(1.25)
+    L.cmass2d_(P(A["c2"]), P(A["vx"]), P(A["vy"]), P(A["vz"]), P(A["vset2"]), i(nx), i(ny), i(nz),
+               i(2), i(1), i(2), i(1), i(nz), d(dx), d(dy), P(A["dz"]), d(dt), ctypes.byref(om))
+    # tree summation order: within a few ulps of the serial sum
+    assert abs(om.value - float(g["cmass2d_outmass"])) <= 1e-13 * abs(float(g["cmass2d_outmass"]))
+    outs = [ctypes.c_double() for _ in range(4)]
+    L.cmass3d_(P(A["c2"]), P(A["cload"]), P(A["cdr"]), P(A["cwt"]), i(nx), i(ny), i(nz), i(2),
+               d(dx), d(dy), P(A["dz"]), *[ctypes.byref(o) for o in outs])
+    np.testing.assert_allclose([o.value for o in outs], g["cmass3d"], rtol=1e-13)
+
+
+# ---------------- FP64 context vs the reference's golden runs ----------------
+
+@pytest.mark.parametrize("path", run_fixtures(), ids=os.path.basename)
+def test_fp64_context_vs_golden(path):
+    g = load(path)
+    sc = scenario_from(g)
+    r = F.Run(sc, precision=F.FP64)
+    try:
+        r.meteo_update()
+        co = r.ctx.coef()
+        for name in ("vx", "rho", "rkhor", "rkh1", "rkver"):
+            assert np.array_equal(co[name], g[name], equal_nan=True), name
+        assert rel_linf(co["vset"], g["vset"]) <= 1e-12
+        assert abs(r.dt - float(g["dt"])) <= 1e-12 * float(g["dt"])
+        r.step(1)
+        c, cdry = r.state()
+        assert rel_linf(c, g["c_step1"]) <= 1e-11
+        assert rel_linf(cdry, g["cdry_step1"]) <= 1e-11
+        r.step(int(g["nsteps"]) - 1)
+        c, cdry = r.state()
+        assert rel_linf(c, g["c_final"]) <= 1e-11
+        assert rel_linf(cdry, g["cdry_final"]) <= 1e-11
+        b = r.budget()
+        for k in ("massvol", "massdep", "outmass", "erumass"):
+            ref = float(g["budget_" + k])
+            if np.isnan(ref):
+                assert np.isnan(b[k])
+            else:
+                assert abs(b[k] - ref) <= 1e-11 * max(abs(ref), 1e-300), k
+    finally:
+        r.close()
+
+
+# ---------------- FP32 fused context ----------------
+
+FINITE = [p for p in run_fixtures() if "similarity" not in p]
+
+
+@pytest.mark.parametrize("path", FINITE, ids=os.path.basename)
+def test_fp32_context_vs_golden(path):
+    g = load(path)
+    sc = scenario_from(g)
+    r = F.Run(sc, precision=F.FP32)
+    try:
+        r.meteo_update()
+        r.step(int(g["nsteps"]))
+        c, cdry = r.state()
+        ref_c, ref_d = g["c_final"], g["cdry_final"]
+        assert rel_l1(interior(c), interior(ref_c)) <= 1e-4
+        assert rel_l1(cdry[:, :, 1:].sum(-1), ref_d[:, :, 1:].sum(-1)) <= 1e-4
+        b = r.budget()
+        for k in ("massvol", "massdep", "outmass"):
+            ref = float(g["budget_" + k])
+            assert abs(b[k] - ref) <= 1e-4 * abs(ref) + 1e-12 * float(g["budget_erumass"]), k
+    finally:
+        r.close()
+
+
+def _compare_fp32_with_oracle(sc, nsteps):
+    o = OracleRun(sc)
+    o.meteo_update()
+    r = F.Run(sc, precision=F.FP32)
+    try:
+        r.meteo_update()
+        assert abs(r.dt - o.dt) <= 1e-12 * o.dt
+        for _ in range(nsteps):
+            o.step()
+        r.step(nsteps)
+        c, cdry = r.state()
+        bo, br = o.budget(), r.budget()
+        load_ref = o.cdry[:, :, 0]
+        return dict(c_l1=rel_l1(interior(c), interior(o.c)),
+                    load_l1=rel_l1(cdry[:, :, 1:].sum(-1), load_ref),
+                    rho_m=abs((br["massvol"] + br["massdep"] + br["outmass"]) -
+                              (bo["massvol"] + bo["massdep"] + bo["outmass"])) / bo["erumass"])
+    finally:
+        r.close()
+
+
+@pytest.mark.parametrize("parity_steps", [7, 30])
+def test_fp32_vs_oracle_moderate_grid(parity_steps):
+    sc = S.config_small(nx=45, ny=38, nz=12, nc=5, seed=21, mean_u=7.0, mean_v=-3.0)
+    m = _compare_fp32_with_oracle(sc, parity_steps)
+    assert m["c_l1"] <= 1e-4, m
+    assert m["load_l1"] <= 1e-4, m
+    assert m["rho_m"] <= 1e-5, m
+
+
+def test_fp32_tile_edges_partial_tiles():
+    # grid sizes that are not multiples of any tile edge, nz at the small end
+    sc = S.config_small(nx=37, ny=11, nz=2, nc=2, seed=5, mean_u=-4.0, mean_v=6.0)
+    m = _compare_fp32_with_oracle(sc, 12)
+    assert m["c_l1"] <= 1e-4 and m["load_l1"] <= 1e-4, m
+
+
+def test_fp32_deterministic():
+    sc = S.config_small(nx=33, ny=29, nz=9, nc=3, seed=9, mean_u=5.0)
+    outs = []
+    for _ in range(2):
+        r = F.Run(sc, precision=F.FP32)
+        r.meteo_update()
+        r.step(15)
+        c, cdry = r.state()
+        outs.append((c, cdry, r.budget()))
+        r.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert outs[0][2] == outs[1][2]
+
+
+def test_settling_chain_matches_serial_oracle():
+    """The warm-started chunked chain reproduces the serial SAVEd-cd chain."""
+    sc = S.config_small(nx=40, ny=30, nz=20, nc=6, seed=3)
+    o = OracleRun(sc)
+    o.meteo_update()
+    r = F.Run(sc, precision=F.FP64)
+    try:
+        r.meteo_update()
+        vs = r.ctx.coef()["vset"]
+        assert rel_linf(vs, o.vset) <= 1e-12
+        assert abs(r.ctx.cd_state - o.cd.value) <= 1e-12 * abs(o.cd.value)
+    finally:
+        r.close()
