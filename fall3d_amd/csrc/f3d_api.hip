@@ -99,6 +99,7 @@ Grid make_grid(int nx, int ny, int nz, int nc, double dx, double dy)
     g.NXY = (long)nx * ny;
     g.dx = dx;
     g.dy = dy;
+    g.nm = 1;
     return g;
 }
 
@@ -409,7 +410,7 @@ void cmass3d_(const double *c, const double *cload, const double *cdry, const do
         set_error("cmass3d: device staging failed");
         return;
     }
-    if (red::cmass3d(0, g, dc, dl, dd, dw, ddz, scr, out)) return;
+    if (red::cmass3d(0, g, dc, dl, dd, dw, 0, ddz, scr, out)) return;
     double o[4];
     if (copy_back(o, out, 4)) return;
     *massvol = o[0] * *dx * *dy;  // cmass3d.f90:191-194

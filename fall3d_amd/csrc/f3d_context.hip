@@ -114,7 +114,7 @@ int step_fp32(f3d_ctx *c)
     float *cin = c->d_c32[c->cur].as<float>();
     float *cout = c->d_c32[1 - c->cur].as<float>();
     const f32::Fields f = f32::fields_view(g, c->d_f32.as<float>(), c->d_vsum1.as<double>());
-    TRY(f32::setsrc(s, g, cin, c->nsou, c->d_isrc.as<int>(), c->d_jsrc.as<int>(),
+    TRY(f32::setsrc(s, g, cin, c->d_srcoff.as<int>(), c->d_isrc.as<int>(), c->d_jsrc.as<int>(),
                     c->d_ksrc.as<int>(), c->d_tmrat.as<double>(), c->d_zlayer.as<double>(),
                     c->d_dz.as<double>(), c->d_hm1.as<double>(), c->dt));
     f32::StepArgs a{g, cin, cout, c->d_cdry.as<double>(), c->dt, c->iiter, c->d_zc.as<ZCoef>()};
@@ -132,7 +132,7 @@ int step_fp32(f3d_ctx *c)
 
 extern "C" {
 
-int f3d_create(f3d_ctx **out, const f3d_grid *gr, int precision)
+static int create_impl(f3d_ctx **out, const f3d_grid *gr, int precision, int nm)
 {
     if (!out || !gr) {
         set_error("f3d_create: null argument");
@@ -156,9 +156,20 @@ int f3d_create(f3d_ctx **out, const f3d_grid *gr, int precision)
         set_error("f3d_create: dx and dy must be positive");
         return F3D_EINVAL;
     }
+    if (nm < 1 || (nm > 1 && precision != F3D_FP32)) {
+        set_error("f3d_create: ensembles (%d members) run on the FP32 fused path only", nm);
+        return F3D_EINVAL;
+    }
     f3d_ctx *c = new f3d_ctx();
     c->precision = precision;
     c->g = make_grid(gr->nx, gr->ny, gr->nz, gr->nc, gr->dx, gr->dy);
+    c->g.nm = nm;
+    c->src_i.assign(nm, {});
+    c->src_j.assign(nm, {});
+    c->src_k.assign(nm, {});
+    c->src_tm.assign(nm, {});
+    c->flowrate.assign(nm, 0.0);
+    c->erumass.assign(nm, 0.0);
     c->lonlat = gr->lonlat;
     const Grid &g = c->g;
     c->zlayer.assign(gr->zlayer, gr->zlayer + g.nz);
@@ -193,16 +204,20 @@ int f3d_create(f3d_ctx **out, const f3d_grid *gr, int precision)
     rc |= c->d_cd.alloc(d8 * 2, B);
     rc |= c->d_status.alloc(sizeof(int) * 2, B);
     rc |= c->d_dtcn.alloc(sizeof(unsigned long long), B);
-    rc |= c->d_cdry.alloc(d8 * g.NXY * (g.nc + 1), B);
-    rc |= c->d_cwet.alloc(d8 * g.NXY * (g.nc + 1), B);
-    rc |= c->d_scratch.alloc(d8 * 4 * red::kPartials, B);
-    rc |= c->d_outmass.alloc(d8, B);
-    rc |= c->d_budget.alloc(d8 * 4, B);
+    rc |= c->d_cdry.alloc(d8 * g.NXY * (g.nc + 1) * nm, B);
+    rc |= c->d_cwet.alloc(d8 * g.NXY * (g.nc + 1) * nm, B);
+    const size_t nscr = std::max<size_t>({4 * (size_t)red::kPartials,
+                                          4 * (size_t)nm * red::kMemberPartials,
+                                          (size_t)nm * red::kFramePartials});
+    rc |= c->d_scratch.alloc(d8 * nscr, B);
+    rc |= c->d_outmass.alloc(d8 * nm, B);
+    rc |= c->d_budget.alloc(d8 * 4 * nm, B);
+    rc |= c->d_srcoff.alloc(sizeof(int) * (nm + 1), B);
     if (precision == F3D_FP64) {
         rc |= c->d_c64.alloc(d8 * g.G * g.nc, B);
     } else {
-        rc |= c->d_c32[0].alloc(sizeof(float) * g.G * g.nc, B);
-        rc |= c->d_c32[1].alloc(sizeof(float) * g.G * g.nc, B);
+        rc |= c->d_c32[0].alloc(sizeof(float) * g.G * g.nc * nm, B);
+        rc |= c->d_c32[1].alloc(sizeof(float) * g.G * g.nc * nm, B);
         rc |= c->d_f32.alloc(sizeof(float) * f32::fields_floats(g), B);
         rc |= c->d_vsum1.alloc(d8 * g.NXY * g.nc, B);
     }
@@ -226,7 +241,7 @@ int f3d_create(f3d_ctx **out, const f3d_grid *gr, int precision)
     // zero state, coefficients and budgets (Master.f90:getmem1 initialises to 0)
     for (DeviceBuffer *b : {&c->d_rkhor, &c->d_rkh1, &c->d_rkver, &c->d_vset, &c->d_vdep,
                             &c->d_cdry, &c->d_cwet, &c->d_outmass, &c->d_c64, &c->d_c32[0],
-                            &c->d_c32[1]})
+                            &c->d_c32[1], &c->d_srcoff})
         if (b->p) rc |= (hipMemsetAsync(b->p, 0, b->bytes, s) != hipSuccess);
     if (hipStreamSynchronize(s) != hipSuccess || rc) {
         set_error("f3d_create: initial upload failed");
@@ -235,6 +250,16 @@ int f3d_create(f3d_ctx **out, const f3d_grid *gr, int precision)
     }
     *out = c;
     return F3D_OK;
+}
+
+int f3d_create(f3d_ctx **out, const f3d_grid *gr, int precision)
+{
+    return create_impl(out, gr, precision, 1);
+}
+
+int f3d_create_ensemble(f3d_ctx **out, const f3d_grid *gr, int nmembers)
+{
+    return create_impl(out, gr, F3D_FP32, nmembers);
 }
 
 void f3d_destroy(f3d_ctx *c)
@@ -385,15 +410,54 @@ int f3d_set_dt(f3d_ctx *c, double dt)
     return F3D_OK;
 }
 
-int f3d_set_source(f3d_ctx *c, int nsou, const int *isrc, const int *jsrc, const int *ksrc,
-                   const double *tmrat, double flowrate)
+}  // extern "C"
+
+// Concatenate the members' source points and upload them
+static int upload_sources(f3d_ctx *c)
 {
-    if (!check_ctx(c)) return F3D_EINVAL;
+    const Grid &g = c->g;
+    std::vector<int> off(g.nm + 1, 0), I, J, K;
+    std::vector<double> T;
+    for (int m = 0; m < g.nm; ++m) {
+        off[m + 1] = off[m] + (int)c->src_i[m].size();
+        I.insert(I.end(), c->src_i[m].begin(), c->src_i[m].end());
+        J.insert(J.end(), c->src_j[m].begin(), c->src_j[m].end());
+        K.insert(K.end(), c->src_k[m].begin(), c->src_k[m].end());
+        T.insert(T.end(), c->src_tm[m].begin(), c->src_tm[m].end());
+    }
+    const int n = off[g.nm];
+    long long *B = &c->bytes;
+    const size_t nn = n > 0 ? n : 1;
+    int rc = 0;
+    rc |= c->d_isrc.alloc(sizeof(int) * nn, B);
+    rc |= c->d_jsrc.alloc(sizeof(int) * nn, B);
+    rc |= c->d_ksrc.alloc(sizeof(int) * nn, B);
+    rc |= c->d_tmrat.alloc(sizeof(double) * nn * g.nc, B);
+    if (rc) return F3D_ENOMEM;
+    TRY(upload(c->d_srcoff, off.data(), sizeof(int) * (g.nm + 1), c->stream));
+    if (n > 0) {
+        TRY(upload(c->d_isrc, I.data(), sizeof(int) * n, c->stream));
+        TRY(upload(c->d_jsrc, J.data(), sizeof(int) * n, c->stream));
+        TRY(upload(c->d_ksrc, K.data(), sizeof(int) * n, c->stream));
+        TRY(upload(c->d_tmrat, T.data(), sizeof(double) * n * g.nc, c->stream));
+    }
+    F3D_HIP(hipStreamSynchronize(c->stream));
+    c->nsou = n;
+    return F3D_OK;
+}
+
+static int set_member_source(f3d_ctx *c, int m, int nsou, const int *isrc, const int *jsrc,
+                             const int *ksrc, const double *tmrat, double flowrate)
+{
+    const Grid &g = c->g;
+    if (m < 0 || m >= g.nm) {
+        set_error("member %d out of range (ensemble of %d)", m, g.nm);
+        return F3D_EINVAL;
+    }
     if (nsou < 0 || (nsou > 0 && (!isrc || !jsrc || !ksrc || !tmrat))) {
         set_error("f3d_set_source: bad arguments");
         return F3D_EINVAL;
     }
-    const Grid &g = c->g;
     for (int is = 0; is < nsou; ++is) {
         if (isrc[is] < 1 || isrc[is] > g.nx || jsrc[is] < 1 || jsrc[is] > g.ny) {
             // reasrc.f90:76-80
@@ -401,24 +465,31 @@ int f3d_set_source(f3d_ctx *c, int nsou, const int *isrc, const int *jsrc, const
             return F3D_EINVAL;
         }
     }
-    long long *B = &c->bytes;
-    int rc = 0;
-    const size_t n = nsou > 0 ? nsou : 1;
-    rc |= c->d_isrc.alloc(sizeof(int) * n, B);
-    rc |= c->d_jsrc.alloc(sizeof(int) * n, B);
-    rc |= c->d_ksrc.alloc(sizeof(int) * n, B);
-    rc |= c->d_tmrat.alloc(sizeof(double) * n * g.nc, B);
-    if (rc) return F3D_ENOMEM;
-    if (nsou > 0) {
-        TRY(upload(c->d_isrc, isrc, sizeof(int) * nsou, c->stream));
-        TRY(upload(c->d_jsrc, jsrc, sizeof(int) * nsou, c->stream));
-        TRY(upload(c->d_ksrc, ksrc, sizeof(int) * nsou, c->stream));
-        TRY(upload(c->d_tmrat, tmrat, sizeof(double) * nsou * g.nc, c->stream));
-    }
-    F3D_HIP(hipStreamSynchronize(c->stream));
-    c->nsou = nsou;
-    c->flowrate = flowrate;
+    c->src_i[m].assign(isrc, isrc + nsou);
+    c->src_j[m].assign(jsrc, jsrc + nsou);
+    c->src_k[m].assign(ksrc, ksrc + nsou);
+    c->src_tm[m].assign(tmrat, tmrat + (size_t)nsou * g.nc);
+    c->flowrate[m] = flowrate;
     return F3D_OK;
+}
+
+extern "C" {
+
+int f3d_set_source(f3d_ctx *c, int nsou, const int *isrc, const int *jsrc, const int *ksrc,
+                   const double *tmrat, double flowrate)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    for (int m = 0; m < c->g.nm; ++m)
+        TRY(set_member_source(c, m, nsou, isrc, jsrc, ksrc, tmrat, flowrate));
+    return upload_sources(c);
+}
+
+int f3d_set_member_source(f3d_ctx *c, int member, int nsou, const int *isrc, const int *jsrc,
+                          const int *ksrc, const double *tmrat, double flowrate)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    TRY(set_member_source(c, member, nsou, isrc, jsrc, ksrc, tmrat, flowrate));
+    return upload_sources(c);
 }
 
 int f3d_step(f3d_ctx *c, int nsteps)
@@ -432,41 +503,70 @@ int f3d_step(f3d_ctx *c, int nsteps)
         // Fall3d.f90:112-114
         c->time = c->time + c->dt;
         c->iiter = c->iiter + 1;
-        c->erumass = c->erumass + c->flowrate * c->dt;
+        for (int m = 0; m < c->g.nm; ++m) c->erumass[m] = c->erumass[m] + c->flowrate[m] * c->dt;
         TRY(c->precision == F3D_FP64 ? step_fp64(c) : step_fp32(c));
     }
     return F3D_OK;
 }
 
-int f3d_budget(f3d_ctx *c, double *out)
+}  // extern "C"
+
+// getc0 + cload + cmass3d for every member; out[6*m + ...]
+static int budgets(f3d_ctx *c, double *out)
 {
-    if (!check_ctx(c)) return F3D_EINVAL;
     const Grid &g = c->g;
     hipStream_t s = c->stream;
     double *cdry = c->d_cdry.as<double>(), *cwet = c->d_cwet.as<double>();
+    const long dstride = (long)(g.nc + 1) * g.NXY;
     TRY(red::getc0(s, g, cdry, cwet));
     if (c->precision == F3D_FP64)
-        TRY(red::cmass3d(s, g, c->d_c64.as<double>(), nullptr, cdry, cwet, c->d_dz.as<double>(),
-                         c->d_scratch.as<double>(), c->d_budget.as<double>()));
-    else
-        TRY(red::cmass3d(s, g, c->d_c32[c->cur].as<float>(), nullptr, cdry, cwet,
+        TRY(red::cmass3d(s, g, c->d_c64.as<double>(), nullptr, cdry, cwet, dstride,
                          c->d_dz.as<double>(), c->d_scratch.as<double>(), c->d_budget.as<double>()));
-    double sums[4], om;
-    F3D_HIP(hipMemcpyAsync(sums, c->d_budget.p, sizeof sums, hipMemcpyDeviceToHost, s));
-    F3D_HIP(hipMemcpyAsync(&om, c->d_outmass.p, sizeof om, hipMemcpyDeviceToHost, s));
+    else
+        TRY(red::cmass3d(s, g, c->d_c32[c->cur].as<float>(), nullptr, cdry, cwet, dstride,
+                         c->d_dz.as<double>(), c->d_scratch.as<double>(), c->d_budget.as<double>()));
+    std::vector<double> sums(4 * g.nm), om(g.nm);
+    F3D_HIP(hipMemcpyAsync(sums.data(), c->d_budget.p, sizeof(double) * 4 * g.nm,
+                           hipMemcpyDeviceToHost, s));
+    F3D_HIP(hipMemcpyAsync(om.data(), c->d_outmass.p, sizeof(double) * g.nm, hipMemcpyDeviceToHost,
+                           s));
     F3D_HIP(hipStreamSynchronize(s));
     if (out) {
-        for (int m = 0; m < 4; ++m) out[m] = sums[m] * g.dx * g.dy;  // cmass3d.f90:191-194
-        out[4] = om;
-        out[5] = c->erumass;
+        for (int m = 0; m < g.nm; ++m) {
+            for (int k = 0; k < 4; ++k) out[6 * m + k] = sums[4 * m + k] * g.dx * g.dy;  // cmass3d.f90:191-194
+            out[6 * m + 4] = om[m];
+            out[6 * m + 5] = c->erumass[m];
+        }
     }
     return F3D_OK;
 }
 
-int f3d_get_state(f3d_ctx *c, double *ch, double *cdry)
+extern "C" {
+
+int f3d_budget(f3d_ctx *c, double *out)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    std::vector<double> all(6 * c->g.nm);
+    TRY(budgets(c, all.data()));
+    if (out)
+        for (int k = 0; k < 6; ++k) out[k] = all[k];
+    return F3D_OK;
+}
+
+int f3d_ensemble_budget(f3d_ctx *c, double *out)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    return budgets(c, out);
+}
+
+int f3d_get_member_state(f3d_ctx *c, int m, double *ch, double *cdry)
 {
     if (!check_ctx(c)) return F3D_EINVAL;
     const Grid &g = c->g;
+    if (m < 0 || m >= g.nm) {
+        set_error("member %d out of range (ensemble of %d)", m, g.nm);
+        return F3D_EINVAL;
+    }
     hipStream_t s = c->stream;
     const size_t n = (size_t)g.G * g.nc;
     if (ch) {
@@ -474,23 +574,33 @@ int f3d_get_state(f3d_ctx *c, double *ch, double *cdry)
             F3D_HIP(hipMemcpyAsync(ch, c->d_c64.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
         } else {
             std::vector<float> tmp(n);
-            F3D_HIP(hipMemcpyAsync(tmp.data(), c->d_c32[c->cur].p, n * sizeof(float),
-                                   hipMemcpyDeviceToHost, s));
+            F3D_HIP(hipMemcpyAsync(tmp.data(), c->d_c32[c->cur].as<float>() + n * m,
+                                   n * sizeof(float), hipMemcpyDeviceToHost, s));
             F3D_HIP(hipStreamSynchronize(s));
             for (size_t i = 0; i < n; ++i) ch[i] = (double)tmp[i];
         }
     }
+    const size_t nd = (size_t)g.NXY * (g.nc + 1);
     if (cdry)
-        F3D_HIP(hipMemcpyAsync(cdry, c->d_cdry.p, sizeof(double) * g.NXY * (g.nc + 1),
+        F3D_HIP(hipMemcpyAsync(cdry, c->d_cdry.as<double>() + nd * m, sizeof(double) * nd,
                                hipMemcpyDeviceToHost, s));
     F3D_HIP(hipStreamSynchronize(s));
     return F3D_OK;
+}
+
+int f3d_get_state(f3d_ctx *c, double *ch, double *cdry)
+{
+    return f3d_get_member_state(c, 0, ch, cdry);
 }
 
 int f3d_set_state(f3d_ctx *c, const double *ch, const double *cdry)
 {
     if (!check_ctx(c)) return F3D_EINVAL;
     const Grid &g = c->g;
+    if (g.nm != 1) {
+        set_error("f3d_set_state: single-member contexts only");
+        return F3D_EINVAL;
+    }
     hipStream_t s = c->stream;
     const size_t n = (size_t)g.G * g.nc;
     if (ch) {
@@ -533,7 +643,7 @@ int f3d_get_clock(f3d_ctx *c, double *time, int *iiter, double *erumass, double 
     if (!check_ctx(c)) return F3D_EINVAL;
     if (time) *time = c->time;
     if (iiter) *iiter = c->iiter;
-    if (erumass) *erumass = c->erumass;
+    if (erumass) *erumass = c->erumass[0];
     if (outmass) {
         F3D_HIP(hipMemcpyAsync(outmass, c->d_outmass.p, sizeof(double), hipMemcpyDeviceToHost,
                                c->stream));
@@ -547,7 +657,7 @@ int f3d_set_clock(f3d_ctx *c, double time, int iiter, double erumass, double out
     if (!check_ctx(c)) return F3D_EINVAL;
     c->time = time;
     c->iiter = iiter;
-    c->erumass = erumass;
+    c->erumass.assign(c->g.nm, erumass);
     F3D_HIP(hipMemcpyAsync(c->d_outmass.p, &outmass, sizeof(double), hipMemcpyHostToDevice,
                            c->stream));
     F3D_HIP(hipStreamSynchronize(c->stream));

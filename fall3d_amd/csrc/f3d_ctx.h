@@ -73,14 +73,15 @@ struct TileShape {
 // One fused launch: setbcc + advctzc + advctx/advcty (parity order) + diffz
 // + diffx/diffy (parity order) + accum + clip for every class, c_in -> c_out.
 int fused_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts);
-// Ghost frame of c_out (setbcc values of c_in, clipped) and cmass2d partials;
-// then outmass += sum (deterministic).
+// Ghost frame of c_out (setbcc values of c_in, clipped) and cmass2d
+// outflux; outmass[m] += member m's outflux (deterministic per member).
 int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,
                       const double *vy64, const double *vz64, const double *vset64,
                       const double *dz, double *scratch, double *outmass);
-int setsrc(hipStream_t s, const Grid &g, float *c, int nsou, const int *isrc, const int *jsrc,
-           const int *ksrc, const double *tmrat, const double *zlayer, const double *dz,
-           const double *hm1, double dt);
+// setsrc for every (member, class); off[m]..off[m+1] are member m's points
+int setsrc(hipStream_t s, const Grid &g, float *c, const int *off, const int *isrc,
+           const int *jsrc, const int *ksrc, const double *tmrat, const double *zlayer,
+           const double *dz, const double *hm1, double dt);
 int to_fp64(hipStream_t s, const float *c32, double *c64, long n);
 int from_fp64(hipStream_t s, const double *c64, float *c32, long n);
 
@@ -124,13 +125,17 @@ struct f3d_ctx {
     f3d::DeviceBuffer d_f32, d_vsum1;
     f3d::f32::TileShape tile{32, 8};
 
-    // source
-    int nsou = 0;
-    f3d::DeviceBuffer d_isrc, d_jsrc, d_ksrc, d_tmrat;
-    double flowrate = 0.0;
+    // source points per member (host copies) and their device concatenation:
+    // member m owns points [src_off[m], src_off[m+1])
+    std::vector<std::vector<int>> src_i, src_j, src_k;
+    std::vector<std::vector<double>> src_tm;  // (nsou_m, nc) per member
+    std::vector<double> flowrate;             // per member
+    int nsou = 0;                             // total points
+    f3d::DeviceBuffer d_isrc, d_jsrc, d_ksrc, d_tmrat, d_srcoff;
 
-    // clock
-    double time = 0.0, dt = 0.0, dtc = 0.0, erumass = 0.0;
+    // clock (shared by the members of an ensemble: one dt)
+    double time = 0.0, dt = 0.0, dtc = 0.0;
+    std::vector<double> erumass;              // per member
     int iiter = 0;
     bool meteo_ready = false;
     double cd_state = 1.0;

@@ -129,8 +129,10 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
     const int L = (b % kXCD) * p.per_xcd + b / kXCD;
     if (L >= p.nlogical) return;
     const int nc = p.g.nc;
-    const int q = L % nc;
-    const int tile = L / nc;
+    const int nq = nc * p.g.nm;     // member-major slabs: q = member*nc + class
+    const int q = L % nq;
+    const int ic = q % nc, mem = q / nc;
+    const int tile = L / nq;
     const int bx = tile % p.ntx, by = tile / p.ntx;
 
     const int nx = p.g.nx, ny = p.g.ny, nz = p.g.nz;
@@ -161,7 +163,7 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
 
     const float *cin = p.cin + (long)q * p.g.G;
     float *cout = p.cout + (long)q * p.g.G;
-    const float *vs = p.f.vs + (long)q * N;
+    const float *vs = p.f.vs + (long)ic * N;
 
     // per-slot column classification
     int kind[NSLOT];   // 0 other, 1 interior, 2 x-ghost, 3 y-ghost
@@ -209,8 +211,8 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
             const long o = coff[sl];
             const float c1 = cin[o + SZ];
             const float c2 = cin[o + 2 * SZ];
-            const float bf = p.f.bfac[(long)q * NXY + m2[sl]];
-            gtopm[sl] = p.f.tfac[(long)q * NXY + m2[sl]];
+            const float bf = p.f.bfac[(long)ic * NXY + m2[sl]];
+            gtopm[sl] = p.f.tfac[(long)ic * NXY + m2[sl]];
             gbot[sl] = c1 * bf;  // setbcc.f90:108-113
             cm1[sl] = gbot[sl];
             c0[sl] = c1;
@@ -221,7 +223,7 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
             W1k[sl] = z1.dtdzz * (v1 * c0[sl] - v1 * cm1[sl]);
             W2k[sl] = w2z(cm1[sl], c0[sl], cp1[sl], v1, v1, z1);
             W2km1[sl] = W2k[sl];
-            bm2[sl] = gbot[sl];
+            bm1[sl] = gbot[sl];  // b(0): becomes bm2 after the level-1 shift
         }
     }
 
@@ -415,8 +417,8 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
                 if (k == 1) {
                     // accum.f90:22, deposit from c(:,:,1) before the clip, FP64
                     const long t2 = (gi - 1) + (long)nx * (gj - 1);
-                    double *cd = p.cdry + (long)(q + 1) * NXY + t2;
-                    *cd = *cd - p.dt64 * (double)out * p.f.vsum1[(long)q * NXY + t2];
+                    double *cd = p.cdry + ((long)mem * (nc + 1) + ic + 1) * NXY + t2;
+                    *cd = *cd - p.dt64 * (double)out * p.f.vsum1[(long)ic * NXY + t2];
                 }
             }
         }
@@ -440,12 +442,13 @@ __global__ void __launch_bounds__(256) k_frame(Grid g, const float *cin, float *
     const long nedge = 4L * nz + 4L * ny + 4L * nx + 8;
     const long per = nxf + nyf + nxy + nedge;
     const long total = per * g.nc;
+    const int mem = blockIdx.y;  // member: its own partials, its own outflux row
     double acc = 0.0;
     for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < total; t += (long)gridDim.x * 256) {
-        const int q = (int)(t / per);
+        const int q = (int)(t / per);  // class
         long r = t - (long)q * per;
-        const float *ci = cin + (long)q * g.G;
-        float *co = cout + (long)q * g.G;
+        const float *ci = cin + ((long)mem * g.nc + q) * g.G;
+        float *co = cout + ((long)mem * g.nc + q) * g.G;
         if (r < nxf) {
             const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
             const float gl = (vx[midx(nx, ny, 1, j, k)] >= 0.0) ? 0.f : fmaxf(0.f, ci[cidx(nx, ny, 1, j, k)]);
@@ -520,7 +523,7 @@ __global__ void __launch_bounds__(256) k_frame(Grid g, const float *cin, float *
         if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) part[(long)mem * gridDim.x + blockIdx.x] = sh[0];
 }
 
 // FP32 step fields from the FP64 coefficients
@@ -560,20 +563,25 @@ __global__ void k_fields_class(Grid g, const double *vz, const double *vset, con
     }
 }
 
-__global__ void k_setsrc32(Grid g, float *c, int nsou, const int *isrc, const int *jsrc,
+// setsrc.f90:20-35 per (member, class); member m owns source points
+// [off[m], off[m+1]) and its tmrat block (nsou_m, nc) starts at nc*off[m]
+__global__ void k_setsrc32(Grid g, float *c, const int *off, const int *isrc, const int *jsrc,
                            const int *ksrc, const double *tmrat, const double *zlayer,
                            const double *dz, const double *hm1, double dt)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.nc) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nc * g.nm) return;
+    const int q = t % g.nc, mem = t / g.nc;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int o = off[mem], nsou = off[mem + 1] - o;
+    const double *tm = tmrat + (long)g.nc * o;
     for (int is = 0; is < nsou; ++is) {
-        const int i = isrc[is], j = jsrc[is], k = ksrc[is];
+        const int i = isrc[o + is], j = jsrc[o + is], k = ksrc[o + is];
         if (k >= 1 && k <= nz) {
             const double dzeta = (k == 1) ? 0.5 * dz[k] + zlayer[k - 1] : 0.5 * (dz[k - 1] + dz[k]);
             const double vol0 = hm1[(i - 1) + (long)nx * (j - 1)] * g.dx * g.dy * dzeta;
-            float *cc = c + (long)q * g.G + cidx(nx, ny, i, j, k);
-            *cc = (float)((double)*cc + dt * tmrat[is + (long)nsou * q] / vol0);
+            float *cc = c + ((long)mem * g.nc + q) * g.G + cidx(nx, ny, i, j, k);
+            *cc = (float)((double)*cc + dt * tm[is + (long)nsou * q] / vol0);
         }
     }
 }
@@ -596,7 +604,7 @@ int launch_fused(hipStream_t s, const Params &p0)
     Params p = p0;
     p.ntx = (p.g.nx + TX - 1) / TX;
     p.nty = (p.g.ny + TY - 1) / TY;
-    p.nlogical = p.ntx * p.nty * p.g.nc;
+    p.nlogical = p.ntx * p.nty * p.g.nc * p.g.nm;
     p.per_xcd = (p.nlogical + kXCD - 1) / kXCD;
     const unsigned nb = (unsigned)(p.per_xcd * kXCD);
     hipLaunchKernelGGL((k_fused<TX, TY>), dim3(nb), dim3(kNT), 0, s, p);
@@ -673,18 +681,18 @@ int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const d
                       const double *vy64, const double *vz64, const double *vset64,
                       const double *dz, double *scratch, double *outmass)
 {
-    hipLaunchKernelGGL(k_frame, dim3(red::kPartials), dim3(256), 0, s, a.g, a.cin, a.cout, f, vx64,
-                       vy64, vz64, vset64, dz, a.dt, scratch);
+    hipLaunchKernelGGL(k_frame, dim3(red::kFramePartials, a.g.nm), dim3(256), 0, s, a.g, a.cin,
+                       a.cout, f, vx64, vy64, vz64, vset64, dz, a.dt, scratch);
     F3D_CHECK_LAUNCH();
-    return red::finish_sum(s, scratch, red::kPartials, outmass, 1);
+    return red::finish_rows(s, scratch, red::kFramePartials, a.g.nm, outmass, 1);
 }
 
-int setsrc(hipStream_t s, const Grid &g, float *c, int nsou, const int *isrc, const int *jsrc,
-           const int *ksrc, const double *tmrat, const double *zlayer, const double *dz,
-           const double *hm1, double dt)
+int setsrc(hipStream_t s, const Grid &g, float *c, const int *off, const int *isrc,
+           const int *jsrc, const int *ksrc, const double *tmrat, const double *zlayer,
+           const double *dz, const double *hm1, double dt)
 {
-    if (nsou <= 0) return F3D_OK;
-    hipLaunchKernelGGL(k_setsrc32, dim3((g.nc + 63) / 64), dim3(64), 0, s, g, c, nsou, isrc, jsrc,
+    const int n = g.nc * g.nm;
+    hipLaunchKernelGGL(k_setsrc32, dim3((n + 63) / 64), dim3(64), 0, s, g, c, off, isrc, jsrc,
                        ksrc, tmrat, zlayer, dz, hm1, dt);
     F3D_CHECK_LAUNCH();
     return F3D_OK;

@@ -47,6 +47,9 @@ struct Grid {
     long N;       // nx*ny*nz
     long NXY;     // nx*ny
     double dx, dy;
+    int nm;       // ensemble members sharing grid, meteo and dt (1 for a single run);
+                  // member m, class ic owns concentration slab m*nc + ic and
+                  // deposit slots cdry + m*(nc+1)*NXY
 };
 
 __host__ __device__ inline long cidx(int nx, int ny, int i, int j, int k)
@@ -103,19 +106,31 @@ constexpr int kPartials = 1024;  // fixed number of per-block partials
 int cmass2d64(hipStream_t s, const Grid &g, int ncls, const double *c, const double *vx,
               const double *vy, const double *vz, const double *vset, const double *dz,
               double dt, double *scratch, double *outmass);
-// getc0 (endstep.f90:80-106): slot 0 of cdry/cwet = sum of the class slots
+// per-member partial counts: a member's reduction tree depends only on its
+// own sizes, never on how many members share the launch
+constexpr int kMemberPartials = 256;
+constexpr int kFramePartials = 128;
+// getc0 (endstep.f90:80-106) for every member: slot 0 of cdry/cwet = sum of
+// the class slots; member m's deposit is cdry + m*(nc+1)*NXY
 int getc0(hipStream_t s, const Grid &g, double *cdry, double *cwet);
-// cmass3d (cmass3d.f90:24-58) without the final dx*dy scaling:
-// out[0..3] = sums for massvol, massdep, massdry, masswet.  cload0 may be
-// null, then cload(:,:,0) = cdry0 + cwet0 (endstep.f90:35).
+// cmass3d (cmass3d.f90:24-58) for every member, without the final dx*dy
+// scaling: out[4*m + 0..3] = sums for massvol, massdep, massdry, masswet.
+// Member m's slot-0 deposit arrays start at cdry0/cwet0 + m*dstride; cload0
+// may be null, then cload(:,:,0) = cdry0 + cwet0 (endstep.f90:35).
+// scratch holds 4*nm*kMemberPartials doubles.
 int cmass3d(hipStream_t s, const Grid &g, const double *c, const double *cload0,
-            const double *cdry0, const double *cwet0, const double *dz, double *scratch,
-            double *out);
+            const double *cdry0, const double *cwet0, long dstride, const double *dz,
+            double *scratch, double *out);
 int cmass3d(hipStream_t s, const Grid &g, const float *c, const double *cload0,
-            const double *cdry0, const double *cwet0, const double *dz, double *scratch,
-            double *out);
-// final stage: dst[0] += sum(partials[0..n)) in a fixed order (one block)
-int finish_sum(hipStream_t s, const double *partials, int n, double *dst, int accumulate);
+            const double *cdry0, const double *cwet0, long dstride, const double *dz,
+            double *scratch, double *out);
+// final stage: dst[r] (+)= sum(partials[r*n .. r*n+n)) for r < nrows, fixed tree
+int finish_rows(hipStream_t s, const double *partials, int n, int nrows, double *dst,
+                int accumulate);
+inline int finish_sum(hipStream_t s, const double *partials, int n, double *dst, int accumulate)
+{
+    return finish_rows(s, partials, n, 1, dst, accumulate);
+}
 }  // namespace red
 
 // ---------------- meteo stage (f3d_meteo.hip), FP64 ----------------

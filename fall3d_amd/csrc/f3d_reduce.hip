@@ -76,30 +76,37 @@ __global__ void __launch_bounds__(kBS) k_cmass2d64(Grid g, int ncls, const doubl
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// getc0, endstep.f90:80-106: slot 0 = sum of the particle slots, in order
+// getc0, endstep.f90:80-106: slot 0 = sum of the particle slots, in order;
+// grid.y = member
 __global__ void k_getc0(Grid g, double *cdry, double *cwet)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.NXY) return;
+    const long mo = (long)blockIdx.y * (g.nc + 1) * g.NXY;
     double d = 0.0, w = 0.0;
     for (int ic = 1; ic <= g.nc; ++ic) {
-        d = d + cdry[t + ic * g.NXY];
-        w = w + cwet[t + ic * g.NXY];
+        d = d + cdry[mo + t + ic * g.NXY];
+        w = w + cwet[mo + t + ic * g.NXY];
     }
-    cdry[t] = d;
-    cwet[t] = w;
+    cdry[mo + t] = d;
+    cwet[mo + t] = w;
 }
 
 // cmass3d.f90:24-58 with cload(:,:,0) = cdry + cwet (endstep.f90:35) when
-// cload0 is null; four partials per block: vol, dep, dry, wet.
+// cload0 is null; grid = (kMemberPartials, members); four partials per
+// block: vol, dep, dry, wet, stored at part[(4*m + quantity)*nb + block].
 template <typename T>
 __global__ void __launch_bounds__(kBS) k_cmass3d(Grid g, const T *c, const double *cload0,
                                                  const double *cdry0, const double *cwet0,
-                                                 const double *dz, double *part)
+                                                 long dstride, const double *dz, double *part)
 {
     __shared__ double sh[kBS];
     const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int m = blockIdx.y;
     const long SZ = (long)(nx + 2) * (ny + 2);
+    const T *cm = c + (long)m * g.nc * g.G;
+    const double *d0 = cdry0 + m * dstride, *w0 = cwet0 + m * dstride;
+    const double *l0 = cload0 ? cload0 + m * dstride : nullptr;
     double av = 0.0, ad = 0.0, adr = 0.0, aw = 0.0;
     for (long t = (long)blockIdx.x * kBS + threadIdx.x; t < g.NXY; t += (long)gridDim.x * kBS) {
         const int i = (int)(t % nx) + 1, j = (int)(t / nx) + 1;
@@ -107,14 +114,14 @@ __global__ void __launch_bounds__(kBS) k_cmass3d(Grid g, const T *c, const doubl
         const double faty = (j == 1 || j == ny) ? 0.5 : 1.0;
         const double ff = fatx * faty;
         for (int q = 0; q < g.nc; ++q) {
-            const T *cc = c + q * g.G + cidx(nx, ny, i, j, 0);
+            const T *cc = cm + q * g.G + cidx(nx, ny, i, j, 0);
             double zmass = 0.0;
             for (int k = 1; k <= nz - 1; ++k)
                 zmass = zmass + 0.5 * dz[k] * ((double)cc[k * SZ] + (double)cc[(k + 1) * SZ]);
             av += ff * zmass;
         }
-        const double d = cdry0[t], w = cwet0[t];
-        const double cl = cload0 ? cload0[t] : d + w;
+        const double d = d0[t], w = w0[t];
+        const double cl = l0 ? l0[t] : d + w;
         ad += ff * cl;
         adr += ff * d;
         aw += ff * w;
@@ -124,38 +131,42 @@ __global__ void __launch_bounds__(kBS) k_cmass3d(Grid g, const T *c, const doubl
     const double sdr = block_sum(adr, sh);
     const double sw = block_sum(aw, sh);
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = sv;
-        part[blockIdx.x + gridDim.x] = sd;
-        part[blockIdx.x + 2 * gridDim.x] = sdr;
-        part[blockIdx.x + 3 * gridDim.x] = sw;
+        const long nb = gridDim.x;
+        part[(4L * m + 0) * nb + blockIdx.x] = sv;
+        part[(4L * m + 1) * nb + blockIdx.x] = sd;
+        part[(4L * m + 2) * nb + blockIdx.x] = sdr;
+        part[(4L * m + 3) * nb + blockIdx.x] = sw;
     }
 }
 
-// dst[m] (+)= sum of part[m*n .. m*n+n), for m = 0..nsum-1, fixed tree
-__global__ void __launch_bounds__(1024) k_finish(const double *part, int n, int nsum, double *dst,
-                                                 int accumulate, double scale)
+// dst[r] (+)= sum of part[r*n .. r*n+n): one block per row, fixed tree
+__global__ void __launch_bounds__(1024) k_finish(const double *part, int n, double *dst,
+                                                 int accumulate)
 {
     __shared__ double sh[1024];
-    for (int m = 0; m < nsum; ++m) {
-        double v = 0.0;
-        for (int t = threadIdx.x; t < n; t += 1024) v += part[(long)m * n + t];
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        for (int s = 512; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            const double r = sh[0] * scale;
-            dst[m] = accumulate ? dst[m] + r : r;
-        }
+    const int r = blockIdx.x;
+    double v = 0.0;
+    for (int t = threadIdx.x; t < n; t += 1024) v += part[(long)r * n + t];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
         __syncthreads();
     }
+    if (threadIdx.x == 0) dst[r] = accumulate ? dst[r] + sh[0] : sh[0];
 }
 
 }  // namespace
 
 namespace red {
+
+int finish_rows(hipStream_t s, const double *partials, int n, int nrows, double *dst,
+                int accumulate)
+{
+    hipLaunchKernelGGL(k_finish, dim3(nrows), dim3(1024), 0, s, partials, n, dst, accumulate);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
 
 int cmass2d64(hipStream_t s, const Grid &g, int ncls, const double *c, const double *vx,
               const double *vy, const double *vz, const double *vset, const double *dz,
@@ -164,51 +175,40 @@ int cmass2d64(hipStream_t s, const Grid &g, int ncls, const double *c, const dou
     hipLaunchKernelGGL(k_cmass2d64, dim3(kPartials), dim3(kBS), 0, s, g, ncls, c, vx, vy, vz,
                        vset, dz, dt, scratch);
     F3D_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, scratch, kPartials, 1, outmass, 1, 1.0);
-    F3D_CHECK_LAUNCH();
-    return F3D_OK;
-}
-
-int finish_sum(hipStream_t s, const double *partials, int n, double *dst, int accumulate)
-{
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, partials, n, 1, dst, accumulate, 1.0);
-    F3D_CHECK_LAUNCH();
-    return F3D_OK;
+    return finish_rows(s, scratch, kPartials, 1, outmass, 1);
 }
 
 int getc0(hipStream_t s, const Grid &g, double *cdry, double *cwet)
 {
-    hipLaunchKernelGGL(k_getc0, dim3((unsigned)((g.NXY + 255) / 256)), dim3(256), 0, s, g, cdry,
-                       cwet);
+    hipLaunchKernelGGL(k_getc0, dim3((unsigned)((g.NXY + 255) / 256), g.nm), dim3(256), 0, s, g,
+                       cdry, cwet);
     F3D_CHECK_LAUNCH();
     return F3D_OK;
 }
 
 template <typename T>
 static int cmass3d_impl(hipStream_t s, const Grid &g, const T *c, const double *cload0,
-                        const double *cdry0, const double *cwet0, const double *dz,
+                        const double *cdry0, const double *cwet0, long dstride, const double *dz,
                         double *scratch, double *out)
 {
-    hipLaunchKernelGGL(k_cmass3d<T>, dim3(kPartials), dim3(kBS), 0, s, g, c, cload0, cdry0, cwet0,
-                       dz, scratch);
+    hipLaunchKernelGGL(k_cmass3d<T>, dim3(kMemberPartials, g.nm), dim3(kBS), 0, s, g, c, cload0,
+                       cdry0, cwet0, dstride, dz, scratch);
     F3D_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, scratch, kPartials, 4, out, 0, 1.0);
-    F3D_CHECK_LAUNCH();
-    return F3D_OK;
+    return finish_rows(s, scratch, kMemberPartials, 4 * g.nm, out, 0);
 }
 
 int cmass3d(hipStream_t s, const Grid &g, const double *c, const double *cload0,
-            const double *cdry0, const double *cwet0, const double *dz, double *scratch,
-            double *out)
+            const double *cdry0, const double *cwet0, long dstride, const double *dz,
+            double *scratch, double *out)
 {
-    return cmass3d_impl<double>(s, g, c, cload0, cdry0, cwet0, dz, scratch, out);
+    return cmass3d_impl<double>(s, g, c, cload0, cdry0, cwet0, dstride, dz, scratch, out);
 }
 
 int cmass3d(hipStream_t s, const Grid &g, const float *c, const double *cload0,
-            const double *cdry0, const double *cwet0, const double *dz, double *scratch,
-            double *out)
+            const double *cdry0, const double *cwet0, long dstride, const double *dz,
+            double *scratch, double *out)
 {
-    return cmass3d_impl<float>(s, g, c, cload0, cdry0, cwet0, dz, scratch, out);
+    return cmass3d_impl<float>(s, g, c, cload0, cdry0, cwet0, dstride, dz, scratch, out);
 }
 
 }  // namespace red

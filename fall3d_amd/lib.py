@@ -25,7 +25,8 @@ _L = None
 # every symbol include/fall3d_amd.h declares
 DROPIN_SYMBOLS = ["setbcc_", "advctzc_", "advctx_", "advcty_", "diffz_", "diffx_", "diffy_",
                   "accum_", "cmass2d_", "cmass3d_"]
-CTX_SYMBOLS = ["f3d_create", "f3d_destroy", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
+CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
+               "f3d_ensemble_budget", "f3d_get_member_state", "f3d_destroy", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
                "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_get_clock",
                "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
@@ -54,6 +55,10 @@ def lib():
                                ctypes.c_void_p)
         L.f3d_last_error.restype = ctypes.c_char_p
         L.f3d_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(_Grid), i]
+        L.f3d_create_ensemble.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(_Grid), i]
+        L.f3d_set_member_source.argtypes = [vp, i, i, ip, ip, ip, dp, d]
+        L.f3d_ensemble_budget.argtypes = [vp, dp]
+        L.f3d_get_member_state.argtypes = [vp, i, dp, dp]
         L.f3d_destroy.argtypes = [vp]
         L.f3d_destroy.restype = None
         L.f3d_set_classes.argtypes = [vp, dp, dp, dp, i]
@@ -77,7 +82,8 @@ def lib():
         L.f3d_kernel_time.argtypes = [vp, i, dp, ctypes.POINTER(ctypes.c_long)]
         L.f3d_device_bytes.argtypes = [vp]
         L.f3d_device_bytes.restype = ctypes.c_longlong
-        for name in ("f3d_create", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
+        for name in ("f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
+                     "f3d_ensemble_budget", "f3d_get_member_state", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
                      "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                      "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_get_clock",
                      "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
@@ -113,15 +119,21 @@ def _P(a):
 class Context:
     """One device-resident scenario (f3d_ctx)."""
 
-    def __init__(self, nx, ny, nz, nc, dx, dy, zlayer, precision=FP32, hm1=None, lonlat=0):
+    def __init__(self, nx, ny, nz, nc, dx, dy, zlayer, precision=FP32, hm1=None, lonlat=0,
+                 members=1):
         L = lib()
         self.nx, self.ny, self.nz, self.nc = nx, ny, nz, nc
         self.precision = precision
+        self.members = members
         self._zl = _F(zlayer)
         self._hm1 = None if hm1 is None else _F(hm1)
         g = _Grid(nx, ny, nz, nc, float(dx), float(dy), _P(self._zl), _P(self._hm1), lonlat)
         h = ctypes.c_void_p()
-        _check(L.f3d_create(ctypes.byref(h), ctypes.byref(g), precision), "f3d_create")
+        if members == 1:
+            _check(L.f3d_create(ctypes.byref(h), ctypes.byref(g), precision), "f3d_create")
+        else:
+            _check(L.f3d_create_ensemble(ctypes.byref(h), ctypes.byref(g), members),
+                   "f3d_create_ensemble")
         self.h = h
 
     def close(self):
@@ -170,6 +182,12 @@ class Context:
         _check(lib().f3d_set_source(self.h, len(self._src[0]), *[_P(a) for a in self._src],
                                     _P(self._tm), float(flowrate)), "f3d_set_source")
 
+    def set_member_source(self, m, isrc, jsrc, ksrc, tmrat, flowrate):
+        src = [np.ascontiguousarray(a, dtype=np.int32) for a in (isrc, jsrc, ksrc)]
+        tm = _F(tmrat)
+        _check(lib().f3d_set_member_source(self.h, int(m), len(src[0]), *[_P(a) for a in src],
+                                           _P(tm), float(flowrate)), "f3d_set_member_source")
+
     def step(self, n=1):
         _check(lib().f3d_step(self.h, int(n)), "f3d_step")
 
@@ -182,12 +200,19 @@ class Context:
         keys = ("massvol", "massdep", "massdry", "masswet", "outmass", "erumass")
         return dict(zip(keys, out.tolist()))
 
-    def state(self):
+    def state(self, member=0):
         nx, ny, nz, nc = self.nx, self.ny, self.nz, self.nc
         c = np.zeros((nx + 2, ny + 2, nz + 2, nc), order="F")
         cdry = np.zeros((nx, ny, nc + 1), order="F")
-        _check(lib().f3d_get_state(self.h, _P(c), _P(cdry)), "f3d_get_state")
+        _check(lib().f3d_get_member_state(self.h, int(member), _P(c), _P(cdry)),
+               "f3d_get_member_state")
         return c, cdry
+
+    def ensemble_budget(self):
+        out = np.zeros(6 * self.members)
+        _check(lib().f3d_ensemble_budget(self.h, _P(out)), "f3d_ensemble_budget")
+        keys = ("massvol", "massdep", "massdry", "masswet", "outmass", "erumass")
+        return [dict(zip(keys, out[6 * m:6 * m + 6].tolist())) for m in range(self.members)]
 
     def set_state(self, c=None, cdry=None):
         c = None if c is None else _F(c)
@@ -282,6 +307,39 @@ class Run:
 
     def budget(self):
         return self.ctx.budget()
+
+    def close(self):
+        self.ctx.close()
+
+
+class Ensemble:
+    """Members on one grid and meteorology (one dt) that differ only in their
+    source; one fused launch per step advances all of them."""
+
+    def __init__(self, base, members):
+        sc = base
+        self.sc = sc
+        self.members = list(members)
+        self.ctx = Context(sc.nx, sc.ny, sc.nz, sc.nc, sc.dx, sc.dy, sc.zlayer, FP32,
+                           members=len(self.members))
+        self.ctx.set_classes(sc.diam, sc.rhop, sc.sphe, sc.modv)
+        self.ctx.set_models(sc.modkv, sc.modkh, sc.rkv0, sc.rkh0, sc.csh)
+        for m, msc in enumerate(self.members):
+            self.ctx.set_member_source(m, msc.isrc, msc.jsrc, msc.ksrc, msc.tmrat, msc.flowrate)
+
+    def meteo_update(self):
+        sc = self.sc
+        return self.ctx.set_meteo(sc.vx, sc.vy, sc.vz, sc.tempe, sc.vptemp, sc.rho, sc.ustar,
+                                  sc.zi, sc.rl, sc.vdep)
+
+    def step(self, n=1):
+        self.ctx.step(n)
+
+    def state(self, m):
+        return self.ctx.state(m)
+
+    def budgets(self):
+        return self.ctx.ensemble_budget()
 
     def close(self):
         self.ctx.close()

@@ -93,6 +93,12 @@ typedef struct {
 
 /* Allocates every device buffer (Master.f90:getmem1 equivalent). */
 int f3d_create(f3d_ctx **ctx, const f3d_grid *grid, int precision);
+/* Ensemble of `nmembers` runs on one grid, one meteorology and one dt that
+ * differ only in their source (FP32 fused path): every member is a FALL3D
+ * run of its own (Fall3d.f90:79-121) and one step advances all of them in
+ * one launch.  No counterpart in the reference, which runs one scenario per
+ * process. */
+int f3d_create_ensemble(f3d_ctx **ctx, const f3d_grid *grid, int nmembers);
 void f3d_destroy(f3d_ctx *ctx);
 
 /* Classes and settling model (reagrn.f90:20-87, readat.f90:218-248).
@@ -118,6 +124,9 @@ int f3d_set_dt(f3d_ctx *ctx, double dt);
  * already multiplied by Hm1 (source.f90:75-77); flowrate for erumass. */
 int f3d_set_source(f3d_ctx *ctx, int nsou, const int *isrc, const int *jsrc, const int *ksrc,
                    const double *tmrat, double flowrate);
+/* Source of one ensemble member (0-based); f3d_set_source sets all members. */
+int f3d_set_member_source(f3d_ctx *ctx, int member, int nsou, const int *isrc, const int *jsrc,
+                          const int *ksrc, const double *tmrat, double flowrate);
 
 /* nsteps passes of the loop body Fall3d.f90:112-117: time += dt,
  * iiter += 1, erumass += flowrate*dt, setsrc, sizloop (with accum, clip and
@@ -127,11 +136,15 @@ int f3d_step(f3d_ctx *ctx, int nsteps);
 /* getc0 + cload + cmass3d (endstep.f90:35,80-106; writim.f90:18).
  * out[6] = massvol, massdep, massdry, masswet, outmass, erumass. */
 int f3d_budget(f3d_ctx *ctx, double *out);
+/* f3d_budget for every member: out[6*m + 0..5]. */
+int f3d_ensemble_budget(f3d_ctx *ctx, double *out);
 
 /* State transfer, FP64 host arrays in reference layout:
  * c (0:nx+1,0:ny+1,0:nz+1,nc), cdry (nx,ny,0:nc) (slot 0 as last set by
- * f3d_budget).  Either pointer may be NULL. */
+ * f3d_budget).  Either pointer may be NULL.  f3d_set_state: single runs only. */
 int f3d_get_state(f3d_ctx *ctx, double *c, double *cdry);
+/* State of one ensemble member; f3d_get_state is member 0. */
+int f3d_get_member_state(f3d_ctx *ctx, int member, double *c, double *cdry);
 int f3d_set_state(f3d_ctx *ctx, const double *c, const double *cdry);
 /* Coefficients after the meteo update (scaled as the reference holds them):
  * vx/Hm1, rho*Hm1, rkhor, rkh1, rkver (nx,ny,nz) and vset (nx,ny,nz,nc). */
