@@ -128,6 +128,21 @@ int step_fp32(f3d_ctx *c)
     return F3D_OK;
 }
 
+// FP32 step fields from the FP64 coefficients; the face coefficients
+// depend on dt, so they are rebuilt whenever dt changes
+int rebuild_fields(f3d_ctx *c)
+{
+    if (c->precision != F3D_FP32) return F3D_OK;
+    const Grid &g = c->g;
+    TRY(f32::build_fields(c->stream, g, c->d_vx.as<double>(), c->d_vy.as<double>(),
+                          c->d_vz.as<double>(), c->d_rho.as<double>(), c->d_rkh1.as<double>(),
+                          c->d_rkhor.as<double>(), c->d_rkver.as<double>(), c->d_vset.as<double>(),
+                          c->d_vdep.as<double>(), c->d_dz.as<double>(), c->dt, c->d_f32.as<float>(),
+                          c->d_vsum1.as<double>()));
+    F3D_HIP(hipStreamSynchronize(c->stream));
+    return F3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -364,12 +379,6 @@ int f3d_set_meteo(f3d_ctx *c, const double *vx, const double *vy, const double *
                     c->d_vy.as<double>(), c->d_vz.as<double>(), c->d_rkh1.as<double>(),
                     c->d_rkhor.as<double>(), c->d_rkver.as<double>(), c->d_vset.as<double>(),
                     c->d_dtcn.as<unsigned long long>()));
-    if (c->precision == F3D_FP32) {
-        TRY(f32::build_fields(s, g, c->d_vx.as<double>(), c->d_vy.as<double>(), c->d_vz.as<double>(),
-                              c->d_rho.as<double>(), c->d_rkh1.as<double>(), c->d_rkhor.as<double>(),
-                              c->d_rkver.as<double>(), c->d_vset.as<double>(), c->d_vdep.as<double>(),
-                              c->d_dz.as<double>(), c->d_f32.as<float>(), c->d_vsum1.as<double>()));
-    }
     unsigned long long bits = 0;
     int status[2] = {0, 0};
     F3D_HIP(hipMemcpyAsync(&bits, c->d_dtcn.p, sizeof bits, hipMemcpyDeviceToHost, s));
@@ -387,6 +396,7 @@ int f3d_set_meteo(f3d_ctx *c, const double *vx, const double *vy, const double *
     dtcn = dtcn > 1e-13 ? dtcn : 1e-13;
     c->dtc = 1.0 / dtcn;
     c->dt = 0.9 * c->dtc;
+    TRY(rebuild_fields(c));
     c->meteo_ready = true;
     return F3D_OK;
 }
@@ -407,6 +417,7 @@ int f3d_set_dt(f3d_ctx *c, double dt)
         return F3D_EINVAL;
     }
     c->dt = dt;
+    if (c->meteo_ready) TRY(rebuild_fields(c));
     return F3D_OK;
 }
 
@@ -688,7 +699,7 @@ int f3d_sync(f3d_ctx *c)
 int f3d_set_tile(f3d_ctx *c, int tx, int ty)
 {
     if (!check_ctx(c)) return F3D_EINVAL;
-    const int ok[][2] = {{32, 8}, {32, 16}, {64, 8}, {64, 4}, {16, 16}};
+    const int ok[][2] = {{32, 8}, {32, 16}, {64, 8}, {64, 4}, {16, 16}, {58, 10}, {58, 18}, {58, 26}};
     for (auto &t : ok)
         if (t[0] == tx && t[1] == ty) {
             c->tile = f32::TileShape{tx, ty};

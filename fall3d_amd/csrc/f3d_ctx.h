@@ -45,13 +45,15 @@ struct Fields {
     const float *bfac;          // bottom ghost factor (nx,ny,nc): 0, or 1+dz0*vdep/rkver(1)
     const float *tfac;          // top ghost factor (nx,ny,nc): 0 or 1 (setbcc.f90:126-130)
     const double *vsum1;        // min(0, vset(:,:,1)+vdep) (nx,ny,nc), FP64 deposit factor
+    // class-independent face coefficients (dt-dependent), see k_fields_face
+    const float *ax, *fx, *ay, *fy, *dfx, *dfy;
 };
 
 // Build Fields from the FP64 coefficient arrays (meteo.f90:108-112 applied).
 int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *vy,
                  const double *vz, const double *rho, const double *rkh1, const double *rkhor,
                  const double *rkver, const double *vset, const double *vdep, const double *dz,
-                 float *f32buf, double *vsum1);
+                 double dt, float *f32buf, double *vsum1);
 size_t fields_floats(const Grid &g);
 Fields fields_view(const Grid &g, const float *f32buf, const double *vsum1);
 
@@ -65,7 +67,8 @@ struct StepArgs {
     const ZCoef *zc;    // per level (FP64, converted in kernel)
 };
 
-// Tile geometry of the fused kernel (chosen by the autotuner)
+// Tile geometry of the fused kernel (chosen by the autotuner).  tx = 58
+// selects the row-per-wave kernel (64-lane region rows, ty interior rows).
 struct TileShape {
     int tx, ty;
 };
@@ -73,6 +76,10 @@ struct TileShape {
 // One fused launch: setbcc + advctzc + advctx/advcty (parity order) + diffz
 // + diffx/diffy (parity order) + accum + clip for every class, c_in -> c_out.
 int fused_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts);
+// Row-per-wavefront form of the same step (f3d_rows32.hip): tiles 58 x ty
+// with ty in {10, 18, 26}.
+bool rows_supported(TileShape ts);
+int rows_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts);
 // Ghost frame of c_out (setbcc values of c_in, clipped) and cmass2d
 // outflux; outmass[m] += member m's outflux (deterministic per member).
 int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,

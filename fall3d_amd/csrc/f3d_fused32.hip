@@ -118,141 +118,129 @@ __device__ __forceinline__ float diff_update(const float *P, const float *LR, co
     return P[0] + dtdh2 * (gr - gl);
 }
 
-template <int TX, int TY>
-__global__ void __launch_bounds__(kNT) k_fused(Params p)
+// One (tile, class) of the fused step.  ODD selects the sweep order of
+// the step (iiter odd: y sweeps first, sizloop.f90:42-61); EDGE is false for
+// tiles whose whole region lies inside the domain, which then need none of
+// the ghost-column and end-of-line cases.
+template <int TX, int TY, bool ODD, bool EDGE>
+__device__ __forceinline__ void fused_tile(const Params &p, const int q, const int ic,
+                                           const int mem, const int gi0, const int gj0,
+                                           float *sA, float *sB, float *sC, float *sD,
+                                           float *sVX, float *sVY, float *sLR, float *sPX,
+                                           float *sPY, const LevelCoef *zt)
 {
     constexpr int RX = TX + 6, RY = TY + 6, RN = RX * RY;
     constexpr int NSLOT = (RN + kNT - 1) / kNT;
-
-    // XCD-aware logical block id
-    const int b = blockIdx.x;
-    const int L = (b % kXCD) * p.per_xcd + b / kXCD;
-    if (L >= p.nlogical) return;
-    const int nc = p.g.nc;
-    const int nq = nc * p.g.nm;     // member-major slabs: q = member*nc + class
-    const int q = L % nq;
-    const int ic = q % nc, mem = q / nc;
-    const int tile = L / nq;
-    const int bx = tile % p.ntx, by = tile / p.ntx;
-
     const int nx = p.g.nx, ny = p.g.ny, nz = p.g.nz;
-    const long NXY = p.g.NXY, N = p.g.N;
-    const long SZ = (long)(nx + 2) * (ny + 2);
-    const int gi0 = 1 + bx * TX - 3, gj0 = 1 + by * TY - 3;
+    const unsigned NXY = (unsigned)p.g.NXY, N = (unsigned)p.g.N;
+    const unsigned SZ = (unsigned)((nx + 2) * (ny + 2));
     const int tid = threadIdx.x;
 
-    __shared__ float sA[RN], sB[RN], sC[RN], sD[RN];
-    __shared__ float sVX[RN], sVY[RN], sLR[RN], sPX[RN], sPY[RN];
-    __shared__ LevelCoef zt[66];
+    const float *cin = p.cin + (size_t)q * p.g.G;
+    float *cout = p.cout + (size_t)q * p.g.G;
+    const float *fvx = p.f.vx, *fvy = p.f.vy, *fvz = p.f.vz, *flr = p.f.lr;
+    const float *fpx = p.f.pkx, *fpy = p.f.pky, *fpz = p.f.pkz;
+    const float *vs = p.f.vs + (size_t)ic * N;
 
-    const float dt = p.dt;
-    for (int k = tid; k <= nz + 1 && k < 66; k += kNT) {
-        LevelCoef z{};
-        if (k >= 1 && k <= nz) {
-            const ZCoef c = p.zc[k];
-            z.dtdzz = (float)(p.dt64 * c.dzz);
-            z.a2 = (float)(p.dt64 * 0.5 * c.dzm1 * c.dzz);
-            z.bc = (float)(p.dt64 * c.dzk_1);
-            z.dzk = (float)c.dzk;
-            z.dzk_1 = (float)c.dzk_1;
-            z.ldz = (float)c.dzk;
-        }
-        if (k == 0) z.ldz = (float)p.zc[1].dzk_1;
-        zt[k] = z;
-    }
-
-    const float *cin = p.cin + (long)q * p.g.G;
-    float *cout = p.cout + (long)q * p.g.G;
-    const float *vs = p.f.vs + (long)ic * N;
-
-    // per-slot column classification
-    int kind[NSLOT];   // 0 other, 1 interior, 2 x-ghost, 3 y-ghost
-    int ctr[NSLOT];    // 1 if inside the diffz region (halo 1)
-    long m2[NSLOT];    // 2-D offset (i-1)+nx*(j-1) of the column (or its face neighbour)
-    long coff[NSLOT];  // c offset of the column at k=0 (or of its face neighbour)
+    // per-slot column data
+    int kind[NSLOT];       // EDGE only: 0 other, 1 interior, 2 x-ghost, 3 y-ghost
+    bool ctr[NSLOT];       // inside the diffz region (halo 1)
+    unsigned m2[NSLOT];    // 2-D offset (i-1)+nx*(j-1) (clamped into the domain)
+    unsigned coff[NSLOT];  // c offset at k=0 of the column (or of its face neighbour)
 #pragma unroll
     for (int sl = 0; sl < NSLOT; ++sl) {
         const int r = tid + kNT * sl;
         const int li = r % RX, lj = r / RX;
         const int gi = gi0 + li, gj = gj0 + lj;
-        kind[sl] = 0;
-        ctr[sl] = 0;
-        m2[sl] = 0;
-        coff[sl] = 0;
-        if (r < RN) {
+        kind[sl] = (r < RN) ? 1 : 0;
+        if (EDGE && r < RN) {
             const bool inx = gi >= 1 && gi <= nx, iny = gj >= 1 && gj <= ny;
-            if (inx && iny) {
-                kind[sl] = 1;
-            } else if (iny && (gi == 0 || gi == nx + 1)) {
-                kind[sl] = 2;
-            } else if (inx && (gj == 0 || gj == ny + 1)) {
-                kind[sl] = 3;
-            }
-            const int ci = min(max(gi, 1), nx), cj = min(max(gj, 1), ny);
-            m2[sl] = (ci - 1) + (long)nx * (cj - 1);
-            coff[sl] = cidx(nx, ny, ci, cj, 0);
-            ctr[sl] = (li >= 2 && li < RX - 2 && lj >= 2 && lj < RY - 2) ? 1 : 0;
+            kind[sl] = (inx && iny) ? 1
+                     : (iny && (gi == 0 || gi == nx + 1)) ? 2
+                     : (inx && (gj == 0 || gj == ny + 1)) ? 3 : 0;
         }
+        const int ci = EDGE ? min(max(gi, 1), nx) : gi;
+        const int cj = EDGE ? min(max(gj, 1), ny) : gj;
+        m2[sl] = (r < RN) ? (unsigned)((ci - 1) + nx * (cj - 1)) : 0u;
+        coff[sl] = (r < RN) ? (unsigned)cidx(nx, ny, ci, cj, 0) : 0u;
+        ctr[sl] = (r < RN) && li >= 2 && li < RX - 2 && lj >= 2 && lj < RY - 2;
     }
-    __syncthreads();
 
-    // stage A state (old column window) and stage C state
     float cm1[NSLOT], c0[NSLOT], cp1[NSLOT], vk[NSLOT], W1k[NSLOT], W2km1[NSLOT], W2k[NSLOT];
-    float gbot[NSLOT], gtopm[NSLOT];
+    float gtopm[NSLOT];
     float bm2[NSLOT], bm1[NSLOT], lrm2[NSLOT], lrm1[NSLOT], pzm2[NSLOT], pzm1[NSLOT];
-    float bcur[NSLOT];
+    // next level's global data, loaded one level ahead so that memory latency
+    // overlaps the current level's compute
+    float nc2[NSLOT], nvz[NSLOT], nvs[NSLOT], nvx[NSLOT], nvy[NSLOT];
+    float nlr[NSLOT], npz[NSLOT], npx[NSLOT], npy[NSLOT];
 
+    const LevelCoef z1 = zt[1];
 #pragma unroll
     for (int sl = 0; sl < NSLOT; ++sl) {
         cm1[sl] = c0[sl] = cp1[sl] = vk[sl] = W1k[sl] = W2km1[sl] = W2k[sl] = 0.f;
-        gbot[sl] = gtopm[sl] = 0.f;
-        bm2[sl] = bm1[sl] = lrm2[sl] = lrm1[sl] = pzm2[sl] = pzm1[sl] = bcur[sl] = 0.f;
+        gtopm[sl] = 0.f;
+        bm2[sl] = bm1[sl] = lrm2[sl] = lrm1[sl] = pzm2[sl] = pzm1[sl] = 0.f;
+        nc2[sl] = nvz[sl] = nvs[sl] = nvx[sl] = nvy[sl] = 0.f;
+        nlr[sl] = npz[sl] = npx[sl] = npy[sl] = 0.f;
         if (kind[sl] == 1) {
-            const long o = coff[sl];
+            const unsigned o = coff[sl], m = m2[sl];
             const float c1 = cin[o + SZ];
             const float c2 = cin[o + 2 * SZ];
-            const float bf = p.f.bfac[(long)ic * NXY + m2[sl]];
-            gtopm[sl] = p.f.tfac[(long)ic * NXY + m2[sl]];
-            gbot[sl] = c1 * bf;  // setbcc.f90:108-113
-            cm1[sl] = gbot[sl];
+            const float bf = p.f.bfac[(size_t)ic * NXY + m];
+            gtopm[sl] = p.f.tfac[(size_t)ic * NXY + m];
+            const float gb = c1 * bf;  // setbcc.f90:108-113
+            cm1[sl] = gb;
             c0[sl] = c1;
             cp1[sl] = c2;
-            const float v1 = p.f.vz[m2[sl]] + vs[m2[sl]];
+            const float v1 = fvz[m] + vs[m];
             vk[sl] = v1;
-            const LevelCoef z1 = zt[1];
-            W1k[sl] = z1.dtdzz * (v1 * c0[sl] - v1 * cm1[sl]);
-            W2k[sl] = w2z(cm1[sl], c0[sl], cp1[sl], v1, v1, z1);
+            W1k[sl] = z1.dtdzz * (v1 * c1 - v1 * gb);
+            W2k[sl] = w2z(gb, c1, c2, v1, v1, z1);
             W2km1[sl] = W2k[sl];
-            bm1[sl] = gbot[sl];  // b(0): becomes bm2 after the level-1 shift
+            bm1[sl] = gb;  // b(0): becomes bm2 after the level-1 shift
+            nc2[sl] = (3 <= nz) ? cin[o + 3 * SZ] : 0.f;
+            nvz[sl] = fvz[NXY + m];
+            nvs[sl] = vs[NXY + m];
+            if (ctr[sl]) {
+                nlr[sl] = flr[m];
+                npz[sl] = fpz[m];
+            }
+        } else if (EDGE && kind[sl] >= 2) {
+            nc2[sl] = cin[coff[sl] + SZ];  // face neighbour at level 1
+        }
+        if (EDGE || kind[sl] == 1) {
+            nvx[sl] = fvx[m2[sl]];
+            nvy[sl] = fvy[m2[sl]];
         }
     }
 
+    const float dt = p.dt;
     const float auxx = (float)(p.dt64 / p.g.dx), khx = 0.5f * auxx;
     const float auxy = (float)(p.dt64 / p.g.dy), khy = 0.5f * auxy;
     const float dtdx2 = (float)(p.dt64 / (p.g.dx * p.g.dx));
     const float dtdy2 = (float)(p.dt64 / (p.g.dy * p.g.dy));
-    const int odd = p.odd;
 
     for (int s = 1; s <= nz + 1; ++s) {
+        const unsigned lv = (unsigned)(s - 1) * NXY;  // level s offset in (nx,ny,nz) fields
         // ---------------- stage A: advctzc at level s ----------------
         if (s <= nz) {
+            const LevelCoef zn = zt[s + 1];
+            const float dtop = dt * zt[nz].ldz;
 #pragma unroll
             for (int sl = 0; sl < NSLOT; ++sl) {
                 const int r = tid + kNT * sl;
                 if (r >= RN) continue;
-                const long lev = (long)(s - 1) * NXY + m2[sl];
                 float a = 0.f;
+                const float vxs = nvx[sl], vys = nvy[sl];
                 if (kind[sl] == 1) {
                     float W1n, W2n, vkn = 0.f, cp2 = 0.f;
                     if (s < nz) {
-                        cp2 = (s + 2 <= nz) ? cin[coff[sl] + (long)(s + 2) * SZ]
-                                            : gtopm[sl] * cp1[sl];  // top ghost (setbcc.f90:126-130)
-                        vkn = p.f.vz[lev + NXY] + vs[lev + NXY];
-                        const LevelCoef z = zt[s + 1];
-                        W1n = z.dtdzz * (vkn * cp1[sl] - vk[sl] * c0[sl]);
-                        W2n = w2z(c0[sl], cp1[sl], cp2, vkn, vk[sl], z);
+                        cp2 = (s + 2 <= nz) ? nc2[sl] : gtopm[sl] * cp1[sl];  // top ghost (setbcc.f90:126-130)
+                        vkn = nvz[sl] + nvs[sl];
+                        W1n = zn.dtdzz * (vkn * cp1[sl] - vk[sl] * c0[sl]);
+                        W2n = w2z(c0[sl], cp1[sl], cp2, vkn, vk[sl], zn);
                     } else {
-                        W1n = dt * zt[nz].ldz * vk[sl] * (cp1[sl] - c0[sl]);  // advctzc.f90:175-181
+                        W1n = dtop * vk[sl] * (cp1[sl] - c0[sl]);  // advctzc.f90:175-181
                         W2n = W2k[sl];
                     }
                     if (vk[sl] > 0.f)
@@ -266,41 +254,50 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
                     W1k[sl] = W1n;
                     W2km1[sl] = W2k[sl];
                     W2k[sl] = W2n;
-                } else if (kind[sl] == 2) {
-                    // x face ghost (setbcc.f90:139-152), from the adjacent cell
-                    const int li = r % RX;
-                    const int gi = gi0 + li;
-                    const float v = p.f.vx[lev];
-                    const bool zero = (gi == 0) ? (v >= 0.f) : (v <= 0.f);
-                    a = zero ? 0.f : cin[coff[sl] + (long)s * SZ];
-                } else if (kind[sl] == 3) {
-                    const int lj = r / RX;
-                    const int gj = gj0 + lj;
-                    const float v = p.f.vy[lev];
-                    const bool zero = (gj == 0) ? (v >= 0.f) : (v <= 0.f);
-                    a = zero ? 0.f : cin[coff[sl] + (long)s * SZ];
+                    if (s + 3 <= nz) nc2[sl] = cin[coff[sl] + (unsigned)(s + 3) * SZ];
+                    if (s + 2 <= nz) {
+                        nvz[sl] = fvz[lv + 2 * NXY + m2[sl]];
+                        nvs[sl] = vs[lv + 2 * NXY + m2[sl]];
+                    }
+                } else if (EDGE && kind[sl] >= 2) {
+                    // face ghost (setbcc.f90:139-169) from the adjacent cell
+                    const int li = r % RX, lj = r / RX;
+                    const bool zero = (kind[sl] == 2)
+                                          ? ((gi0 + li == 0) ? (vxs >= 0.f) : (vxs <= 0.f))
+                                          : ((gj0 + lj == 0) ? (vys >= 0.f) : (vys <= 0.f));
+                    a = zero ? 0.f : nc2[sl];
+                    if (s + 1 <= nz) nc2[sl] = cin[coff[sl] + (unsigned)(s + 1) * SZ];
                 }
                 sA[r] = a;
-                sVX[r] = p.f.vx[lev];
-                sVY[r] = p.f.vy[lev];
+                sVX[r] = vxs;
+                sVY[r] = vys;
+                if (s + 1 <= nz && (EDGE || kind[sl] == 1)) {
+                    nvx[sl] = fvx[lv + NXY + m2[sl]];
+                    nvy[sl] = fvy[lv + NXY + m2[sl]];
+                }
             }
         }
         __syncthreads();
 
-        // ---------------- stage B1 + B2: horizontal advection at level s ----------------
+        // ---------------- stage B1: first horizontal advection at level s ----------------
         if (s <= nz) {
-            // B1: odd -> advcty on rows lj in [2,RY-2) ... output region covers what B2 reads
-            for (int r = tid; r < RN; r += kNT) {
+#pragma unroll
+            for (int sl = 0; sl < NSLOT; ++sl) {
+                const int r = tid + kNT * sl;
+                if (r >= RN) continue;
                 const int li = r % RX, lj = r / RX;
-                const int gi = gi0 + li, gj = gj0 + lj;
-                const bool interior = gi >= 1 && gi <= nx && gj >= 1 && gj <= ny;
                 float out = sA[r];
-                if (odd) {
-                    if (lj >= 2 && lj < RY - 2 && interior)
-                        out = adv_update(&sA[r], &sVY[r], RX, auxy, khy, gj == 1, gj == ny);
-                } else {
-                    if (li >= 2 && li < RX - 2 && interior)
-                        out = adv_update(&sA[r], &sVX[r], 1, auxx, khx, gi == 1, gi == nx);
+                if (kind[sl] == 1) {
+                    const int gi = gi0 + li, gj = gj0 + lj;
+                    if (ODD) {
+                        if (lj >= 2 && lj < RY - 2)
+                            out = adv_update(&sA[r], &sVY[r], RX, auxy, khy, EDGE && gj == 1,
+                                             EDGE && gj == ny);
+                    } else {
+                        if (li >= 2 && li < RX - 2)
+                            out = adv_update(&sA[r], &sVX[r], 1, auxx, khx, EDGE && gi == 1,
+                                             EDGE && gi == nx);
+                    }
                 }
                 sB[r] = out;
             }
@@ -311,28 +308,25 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
 #pragma unroll
         for (int sl = 0; sl < NSLOT; ++sl) {
             const int r = tid + kNT * sl;
-            if (r >= RN || !ctr[sl]) continue;
-            const int li = r % RX, lj = r / RX;
-            const int gi = gi0 + li, gj = gj0 + lj;
+            if (!ctr[sl]) continue;
             float bnew;
             if (s <= nz) {
                 bnew = sB[r];
                 if (kind[sl] == 1) {
-                    if (odd)
-                        bnew = adv_update(&sB[r], &sVX[r], 1, auxx, khx, gi == 1, gi == nx);
+                    const int li = r % RX, lj = r / RX;
+                    const int gi = gi0 + li, gj = gj0 + lj;
+                    if (ODD)
+                        bnew = adv_update(&sB[r], &sVX[r], 1, auxx, khx, EDGE && gi == 1,
+                                          EDGE && gi == nx);
                     else
-                        bnew = adv_update(&sB[r], &sVY[r], RX, auxy, khy, gj == 1, gj == ny);
+                        bnew = adv_update(&sB[r], &sVY[r], RX, auxy, khy, EDGE && gj == 1,
+                                          EDGE && gj == ny);
                 }
             } else {
                 bnew = (kind[sl] == 1) ? gtopm[sl] * cm1[sl] : bm1[sl];  // b(nz+1) = top ghost
             }
             if (kind[sl] == 1) {
-                float lrn = 0.f, pzn = 0.f;
-                if (s <= nz) {
-                    const long lev = (long)(s - 1) * NXY + m2[sl];
-                    lrn = p.f.lr[lev];
-                    pzn = p.f.pkz[lev];
-                }
+                const float lrn = nlr[sl], pzn = npz[sl];  // level s (unused at s = nz+1)
                 if (s >= 2) {
                     const int k = s - 1;
                     float dnew;
@@ -354,9 +348,8 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
                     }
                     sC[r] = dnew;
                     sLR[r] = lrm1[sl];
-                    const long lev = (long)(k - 1) * NXY + m2[sl];
-                    sPX[r] = p.f.pkx[lev];
-                    sPY[r] = p.f.pky[lev];
+                    sPX[r] = npx[sl];
+                    sPY[r] = npy[sl];
                 }
                 bm2[sl] = bm1[sl];
                 bm1[sl] = bnew;
@@ -364,6 +357,15 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
                 lrm1[sl] = lrn;
                 pzm2[sl] = pzm1[sl];
                 pzm1[sl] = pzn;
+                // prefetch for the next level: lr/pz(s+1), pkx/pky(s)
+                if (s + 1 <= nz) {
+                    nlr[sl] = flr[lv + NXY + m2[sl]];
+                    npz[sl] = fpz[lv + NXY + m2[sl]];
+                }
+                if (s <= nz) {
+                    npx[sl] = fpx[lv + m2[sl]];
+                    npy[sl] = fpy[lv + m2[sl]];
+                }
             } else {
                 if (s >= 2) {
                     sC[r] = bm1[sl];
@@ -379,21 +381,26 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
 
         // ---------------- stage D1: first horizontal diffusion at level s-1 ----------------
         {
-            // odd: diffy first on li in [2,RX-2), lj in [3,RY-3); even: diffx first on
+            // ODD: diffy first on li in [2,RX-2), lj in [3,RY-3); else diffx first on
             // li in [3,RX-3), lj in [2,RY-2)
-            const int nli = odd ? RX - 4 : TX, nlj = odd ? TY : RY - 4;
-            const int li0 = odd ? 2 : 3, lj0 = odd ? 3 : 2;
-            for (int e = tid; e < nli * nlj; e += kNT) {
+            constexpr int nli = ODD ? RX - 4 : TX, nlj = ODD ? TY : RY - 4;
+            constexpr int li0 = ODD ? 2 : 3, lj0 = ODD ? 3 : 2;
+#pragma unroll
+            for (int e0 = 0; e0 < nli * nlj; e0 += kNT) {
+                const int e = e0 + tid;
+                if (e >= nli * nlj) continue;
                 const int li = li0 + e % nli, lj = lj0 + e / nli;
                 const int r = lj * RX + li;
                 const int gi = gi0 + li, gj = gj0 + lj;
-                const bool interior = gi >= 1 && gi <= nx && gj >= 1 && gj <= ny;
+                const bool interior = !EDGE || (gi >= 1 && gi <= nx && gj >= 1 && gj <= ny);
                 float out = sC[r];
                 if (interior) {
-                    if (odd)
-                        out = diff_update(&sC[r], &sLR[r], &sPY[r], RX, dtdy2, gj == 1, gj == ny);
+                    if (ODD)
+                        out = diff_update(&sC[r], &sLR[r], &sPY[r], RX, dtdy2, EDGE && gj == 1,
+                                          EDGE && gj == ny);
                     else
-                        out = diff_update(&sC[r], &sLR[r], &sPX[r], 1, dtdx2, gi == 1, gi == nx);
+                        out = diff_update(&sC[r], &sLR[r], &sPX[r], 1, dtdx2, EDGE && gi == 1,
+                                          EDGE && gi == nx);
                 }
                 sD[r] = out;
             }
@@ -403,22 +410,28 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
         // ---------------- stage D2: last sweep, output, deposit ----------------
         {
             const int k = s - 1;
-            for (int e = tid; e < TX * TY; e += kNT) {
+            float *co = cout + (unsigned)k * SZ;
+#pragma unroll
+            for (int e0 = 0; e0 < TX * TY; e0 += kNT) {
+                const int e = e0 + tid;
+                if (e >= TX * TY) continue;
                 const int li = 3 + e % TX, lj = 3 + e / TX;
                 const int gi = gi0 + li, gj = gj0 + lj;
-                if (gi > nx || gj > ny) continue;
+                if (EDGE && (gi > nx || gj > ny)) continue;
                 const int r = lj * RX + li;
                 float out;
-                if (odd)
-                    out = diff_update(&sD[r], &sLR[r], &sPX[r], 1, dtdx2, gi == 1, gi == nx);
+                if (ODD)
+                    out = diff_update(&sD[r], &sLR[r], &sPX[r], 1, dtdx2, EDGE && gi == 1,
+                                      EDGE && gi == nx);
                 else
-                    out = diff_update(&sD[r], &sLR[r], &sPY[r], RX, dtdy2, gj == 1, gj == ny);
-                cout[cidx(nx, ny, gi, gj, k)] = fmaxf(0.f, out);  // sizloop.f90:77-78
+                    out = diff_update(&sD[r], &sLR[r], &sPY[r], RX, dtdy2, EDGE && gj == 1,
+                                      EDGE && gj == ny);
+                co[(unsigned)cidx(nx, ny, gi, gj, 0)] = fmaxf(0.f, out);  // sizloop.f90:77-78
                 if (k == 1) {
                     // accum.f90:22, deposit from c(:,:,1) before the clip, FP64
-                    const long t2 = (gi - 1) + (long)nx * (gj - 1);
-                    double *cd = p.cdry + ((long)mem * (nc + 1) + ic + 1) * NXY + t2;
-                    *cd = *cd - p.dt64 * (double)out * p.f.vsum1[(long)ic * NXY + t2];
+                    const size_t t2 = (gi - 1) + (size_t)nx * (gj - 1);
+                    double *cd = p.cdry + ((size_t)mem * (p.g.nc + 1) + ic + 1) * NXY + t2;
+                    *cd = *cd - p.dt64 * (double)out * p.f.vsum1[(size_t)ic * NXY + t2];
                 }
             }
         }
@@ -426,6 +439,52 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
         // barrier after stage C; stage D planes are rewritten only after two
         // more barriers
     }
+}
+
+template <int TX, int TY, bool ODD>
+__global__ void __launch_bounds__(kNT) k_fused(Params p)
+{
+    constexpr int RX = TX + 6, RY = TY + 6, RN = RX * RY;
+    // XCD-aware logical block id: consecutive logical ids (the classes and
+    // members of one tile) land on one XCD
+    const int b = blockIdx.x;
+    const int L = (b % kXCD) * p.per_xcd + b / kXCD;
+    if (L >= p.nlogical) return;
+    const int nc = p.g.nc;
+    const int nq = nc * p.g.nm;  // member-major slabs: q = member*nc + class
+    const int q = L % nq;
+    const int ic = q % nc, mem = q / nc;
+    const int tile = L / nq;
+    const int bx = tile % p.ntx, by = tile / p.ntx;
+    const int gi0 = 1 + bx * TX - 3, gj0 = 1 + by * TY - 3;
+
+    __shared__ float sA[RN], sB[RN], sC[RN], sD[RN];
+    __shared__ float sVX[RN], sVY[RN], sLR[RN], sPX[RN], sPY[RN];
+    __shared__ LevelCoef zt[66];
+
+    const int nz = p.g.nz;
+    for (int k = threadIdx.x; k <= nz + 1 && k < 66; k += kNT) {
+        LevelCoef z{};
+        if (k >= 1 && k <= nz) {
+            const ZCoef c = p.zc[k];
+            z.dtdzz = (float)(p.dt64 * c.dzz);
+            z.a2 = (float)(p.dt64 * 0.5 * c.dzm1 * c.dzz);
+            z.bc = (float)(p.dt64 * c.dzk_1);
+            z.dzk = (float)c.dzk;
+            z.dzk_1 = (float)c.dzk_1;
+            z.ldz = (float)c.dzk;
+        }
+        if (k == 0) z.ldz = (float)p.zc[1].dzk_1;
+        zt[k] = z;
+    }
+    __syncthreads();
+    const bool edge = gi0 < 1 || gj0 < 1 || gi0 + RX - 1 > p.g.nx || gj0 + RY - 1 > p.g.ny;
+    if (edge)
+        fused_tile<TX, TY, ODD, true>(p, q, ic, mem, gi0, gj0, sA, sB, sC, sD, sVX, sVY, sLR, sPX,
+                                      sPY, zt);
+    else
+        fused_tile<TX, TY, ODD, false>(p, q, ic, mem, gi0, gj0, sA, sB, sC, sD, sVX, sVY, sLR,
+                                       sPX, sPY, zt);
 }
 
 // ---------------- ghost frame + cmass2d outflux ----------------
@@ -543,6 +602,35 @@ __global__ void k_fields3d(long N, const double *vx, const double *vy, const dou
     o[6 * N + t] = (float)(r * rkver[t]);
 }
 
+// Class-independent face coefficients of the horizontal sweeps, formed once
+// per meteo read (they depend on the wind, the diffusivities and dt only):
+//   ax = dt/dx * vx                                   (advctx.f90:36)
+//   fx = 0.5*dt/dx * (1 - dt/dx*|v12|) * |v12|        at face i-1/2 (:42-43),
+//        v12 = 0.5*(vx(i) + vx(max(1,i-1)))
+//   dfx = dt/dx^2 * 0.5*(rho*rkh1)(i) + (rho*rkh1)(i-1)) at face i-1/2 (diffx.f90:35-36)
+// and the same along y with vy, rho*rkhor (advcty.f90, diffy.f90).
+__global__ void k_fields_face(Grid g, const double *vx, const double *vy, const double *rho,
+                              const double *rkh1, const double *rkhor, double dt, float *ax,
+                              float *fx, float *ay, float *fy, float *dfx, float *dfy)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.N) return;
+    const int nx = g.nx;
+    const int i = (int)(t % nx) + 1;
+    const int j = (int)((t / nx) % g.ny) + 1;
+    const long tx = (i > 1) ? t - 1 : t;
+    const long ty = (j > 1) ? t - nx : t;
+    const double auxx = dt / g.dx, auxy = dt / g.dy;
+    const double v12x = fabs(0.5 * (vx[t] + vx[tx]));
+    const double v12y = fabs(0.5 * (vy[t] + vy[ty]));
+    ax[t] = (float)(auxx * vx[t]);
+    fx[t] = (float)(0.5 * auxx * (1.0 - auxx * v12x) * v12x);
+    ay[t] = (float)(auxy * vy[t]);
+    fy[t] = (float)(0.5 * auxy * (1.0 - auxy * v12y) * v12y);
+    dfx[t] = (float)(dt / (g.dx * g.dx) * 0.5 * (rho[t] * rkh1[t] + rho[tx] * rkh1[tx]));
+    dfy[t] = (float)(dt / (g.dy * g.dy) * 0.5 * (rho[t] * rkhor[t] + rho[ty] * rkhor[ty]));
+}
+
 __global__ void k_fields_class(Grid g, const double *vz, const double *vset, const double *vdep,
                                const double *rkver, const double *dz, float *vs32, float *bfac,
                                float *tfac, double *vsum1)
@@ -607,7 +695,10 @@ int launch_fused(hipStream_t s, const Params &p0)
     p.nlogical = p.ntx * p.nty * p.g.nc * p.g.nm;
     p.per_xcd = (p.nlogical + kXCD - 1) / kXCD;
     const unsigned nb = (unsigned)(p.per_xcd * kXCD);
-    hipLaunchKernelGGL((k_fused<TX, TY>), dim3(nb), dim3(kNT), 0, s, p);
+    if (p.odd)
+        hipLaunchKernelGGL((k_fused<TX, TY, true>), dim3(nb), dim3(kNT), 0, s, p);
+    else
+        hipLaunchKernelGGL((k_fused<TX, TY, false>), dim3(nb), dim3(kNT), 0, s, p);
     F3D_CHECK_LAUNCH();
     return F3D_OK;
 }
@@ -616,7 +707,7 @@ int launch_fused(hipStream_t s, const Params &p0)
 
 size_t fields_floats(const Grid &g)
 {
-    return (size_t)7 * g.N + (size_t)g.nc * g.N + (size_t)2 * g.nc * g.NXY;
+    return (size_t)13 * g.N + (size_t)g.nc * g.N + (size_t)2 * g.nc * g.NXY;
 }
 
 Fields fields_view(const Grid &g, const float *b, const double *vsum1)
@@ -632,6 +723,13 @@ Fields fields_view(const Grid &g, const float *b, const double *vsum1)
     f.vs = b + 7 * g.N;
     f.bfac = b + 7 * g.N + (long)g.nc * g.N;
     f.tfac = f.bfac + (long)g.nc * g.NXY;
+    const float *face = f.tfac + (long)g.nc * g.NXY;
+    f.ax = face;
+    f.fx = face + g.N;
+    f.ay = face + 2 * g.N;
+    f.fy = face + 3 * g.N;
+    f.dfx = face + 4 * g.N;
+    f.dfy = face + 5 * g.N;
     f.vsum1 = vsum1;
     return f;
 }
@@ -639,7 +737,7 @@ Fields fields_view(const Grid &g, const float *b, const double *vsum1)
 int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *vy,
                  const double *vz, const double *rho, const double *rkh1, const double *rkhor,
                  const double *rkver, const double *vset, const double *vdep, const double *dz,
-                 float *f32buf, double *vsum1)
+                 double dt, float *f32buf, double *vsum1)
 {
     hipLaunchKernelGGL(k_fields3d, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g.N, vx,
                        vy, vz, rho, rkh1, rkhor, rkver, f32buf);
@@ -648,6 +746,12 @@ int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *v
     hipLaunchKernelGGL(k_fields_class, dim3((unsigned)((g.N + 255) / 256), g.nc), dim3(256), 0, s, g,
                        vz, vset, vdep, rkver, dz, const_cast<float *>(f.vs),
                        const_cast<float *>(f.bfac), const_cast<float *>(f.tfac), vsum1);
+    F3D_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_fields_face, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g, vx,
+                       vy, rho, rkh1, rkhor, dt, const_cast<float *>(f.ax),
+                       const_cast<float *>(f.fx), const_cast<float *>(f.ay),
+                       const_cast<float *>(f.fy), const_cast<float *>(f.dfx),
+                       const_cast<float *>(f.dfy));
     F3D_CHECK_LAUNCH();
     return F3D_OK;
 }
@@ -668,6 +772,7 @@ int fused_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
     p.dt64 = a.dt;
     p.dt = (float)a.dt;
     p.odd = (a.iiter % 2) != 0;
+    if (rows_supported(ts)) return rows_step(s, a, f, ts);
     if (ts.tx == 32 && ts.ty == 8) return launch_fused<32, 8>(s, p);
     if (ts.tx == 32 && ts.ty == 16) return launch_fused<32, 16>(s, p);
     if (ts.tx == 64 && ts.ty == 8) return launch_fused<64, 8>(s, p);

@@ -268,9 +268,11 @@ class Run:
     meteo read, then time += dt; iiter += 1; erumass += flowrate*dt;
     setsrc; sizloop (Fall3d.f90:79-121)."""
 
-    def __init__(self, sc, precision=FP32, cd_state=1.0):
+    def __init__(self, sc, precision=FP32, cd_state=1.0, tile=None):
         self.sc = sc
         self.ctx = Context(sc.nx, sc.ny, sc.nz, sc.nc, sc.dx, sc.dy, sc.zlayer, precision)
+        if tile is not None:
+            self.ctx.set_tile(*tile)
         self.ctx.set_classes(sc.diam, sc.rhop, sc.sphe, sc.modv)
         self.ctx.set_models(sc.modkv, sc.modkh, sc.rkv0, sc.rkh0, sc.csh)
         self.ctx.set_source(sc.isrc, sc.jsrc, sc.ksrc, sc.tmrat, sc.flowrate)

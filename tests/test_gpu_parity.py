@@ -156,10 +156,13 @@ def test_fp32_context_vs_golden(path):
         r.close()
 
 
-def _compare_fp32_with_oracle(sc, nsteps):
+TILES = [(32, 8), (16, 16), (64, 4), (58, 10), (58, 18), (58, 26)]
+
+
+def _compare_fp32_with_oracle(sc, nsteps, tile=None):
     o = OracleRun(sc)
     o.meteo_update()
-    r = F.Run(sc, precision=F.FP32)
+    r = F.Run(sc, precision=F.FP32, tile=tile)
     try:
         r.meteo_update()
         assert abs(r.dt - o.dt) <= 1e-12 * o.dt
@@ -186,10 +189,20 @@ def test_fp32_vs_oracle_moderate_grid(parity_steps):
     assert m["rho_m"] <= 1e-5, m
 
 
-def test_fp32_tile_edges_partial_tiles():
+@pytest.mark.parametrize("tile", TILES, ids=lambda t: f"{t[0]}x{t[1]}")
+def test_fp32_every_tile_shape(tile):
+    # a grid larger than every tile in both directions (interior and edge
+    # tiles), both sweep orders (11 steps), winds of both signs
+    sc = S.config_small(nx=150, ny=61, nz=10, nc=3, seed=27, mean_u=-3.0, mean_v=2.0)
+    m = _compare_fp32_with_oracle(sc, 11, tile)
+    assert m["c_l1"] <= 1e-4 and m["load_l1"] <= 1e-4, m
+
+
+@pytest.mark.parametrize("tile", TILES, ids=lambda t: f"{t[0]}x{t[1]}")
+def test_fp32_tile_edges_partial_tiles(tile):
     # grid sizes that are not multiples of any tile edge, nz at the small end
     sc = S.config_small(nx=37, ny=11, nz=2, nc=2, seed=5, mean_u=-4.0, mean_v=6.0)
-    m = _compare_fp32_with_oracle(sc, 12)
+    m = _compare_fp32_with_oracle(sc, 12, tile)
     assert m["c_l1"] <= 1e-4 and m["load_l1"] <= 1e-4, m
 
 
