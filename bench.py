@@ -51,7 +51,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--tile", default="32x8")
+    ap.add_argument("--tile", default="58x10")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -98,6 +98,10 @@ def main():
     budget = r.budget()
     fp32_state = None
     r.close()
+
+    # plain 1 GiB copy: achievable HBM bandwidth, and the known-size kernel
+    # that calibrates the rocprofv3 FETCH_SIZE / WRITE_SIZE counters
+    copy_gbs = copy_calibration()
 
     # ---------------- FP64 plain drop-in ----------------
     fp64 = None
@@ -172,8 +176,21 @@ def main():
         "meteo_stage_s": meteo_s,
         "cpu_baseline": cpu,
         "hbm_peak_frac_wall": value * b_alg / 1e9 / HBM_PEAK_GBS,
+        "hbm_copy_gbs": copy_gbs,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """The library's plain copy kernel (f3d_hbm_copy) of nbytes in and nbytes
+    out; under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch give
+    the scale from counter KiB to HBM bytes.  Returns the copy's GB/s."""
+    from fall3d_amd import lib as F
+    ms = F.hbm_copy(nbytes, reps)
+    gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    log(f"copy calibration: {reps} launches, {nbytes} B read + {nbytes} B written each, "
+        f"{ms:.3f} ms, {gbs:.0f} GB/s")
+    return gbs
 
 
 def r_dt(sc, coef):

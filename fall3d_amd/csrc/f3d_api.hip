@@ -422,3 +422,47 @@ void cmass3d_(const double *c, const double *cload, const double *cdry, const do
 const char *f3d_last_error(void) { return f3d::g_err; }
 
 }  // extern "C"
+
+namespace {
+// plain copy, one float4 per thread: the known-size kernel for HBM calibration
+__global__ void k_hbm_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
+}
+}  // namespace
+
+extern "C" int f3d_hbm_copy(long long nbytes, int reps, double *avg_ms)
+{
+    if (nbytes <= 0 || nbytes % 4096 != 0 || reps < 1 || !avg_ms) {
+        set_error("f3d_hbm_copy: nbytes must be a positive multiple of 4096, reps >= 1");
+        return F3D_EINVAL;
+    }
+    DeviceBuffer a, b;
+    if (a.alloc((size_t)nbytes, nullptr) || b.alloc((size_t)nbytes, nullptr)) return F3D_ENOMEM;
+    const long n = (long)(nbytes / 16);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        set_error("f3d_hbm_copy: hipEventCreate failed");
+        return F3D_EHIP;
+    }
+    int rc = F3D_OK;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_hbm_copy, dim3(nb), dim3(256), 0, 0, (const float4 *)a.p, (float4 *)b.p, n);
+    hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_hbm_copy, dim3(nb), dim3(256), 0, 0, (const float4 *)a.p,
+                           (float4 *)b.p, n);
+    hipEventRecord(e1, 0);
+    if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess) {
+        set_error("f3d_hbm_copy: copy kernel failed");
+        rc = F3D_EHIP;
+    } else {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        *avg_ms = (double)ms / reps;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}

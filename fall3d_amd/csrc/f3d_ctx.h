@@ -130,7 +130,7 @@ struct f3d_ctx {
     f3d::DeviceBuffer d_scratch, d_outmass, d_budget;
     // FP32 step fields
     f3d::DeviceBuffer d_f32, d_vsum1;
-    f3d::f32::TileShape tile{32, 8};
+    f3d::f32::TileShape tile{58, 10};
 
     // source points per member (host copies) and their device concatenation:
     // member m owns points [src_off[m], src_off[m+1])

@@ -31,7 +31,7 @@ CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_get_clock",
                "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
                "f3d_set_tile", "f3d_timing", "f3d_kernel_time", "f3d_device_bytes",
-               "f3d_last_error"]
+               "f3d_last_error", "f3d_hbm_copy"]
 
 
 class F3DError(RuntimeError):
@@ -82,6 +82,8 @@ def lib():
         L.f3d_kernel_time.argtypes = [vp, i, dp, ctypes.POINTER(ctypes.c_long)]
         L.f3d_device_bytes.argtypes = [vp]
         L.f3d_device_bytes.restype = ctypes.c_longlong
+        L.f3d_hbm_copy.argtypes = [ctypes.c_longlong, i, dp]
+        L.f3d_hbm_copy.restype = c
         for name in ("f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                      "f3d_ensemble_budget", "f3d_get_member_state", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
                      "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
@@ -91,6 +93,14 @@ def lib():
             getattr(L, name).restype = c
         _L = L
     return _L
+
+
+def hbm_copy(nbytes=1 << 30, reps=5):
+    """Average time (ms) of a plain device copy of nbytes in + nbytes out
+    (f3d_hbm_copy): the calibration kernel for HBM bandwidth and counters."""
+    ms = ctypes.c_double()
+    _check(lib().f3d_hbm_copy(int(nbytes), int(reps), ctypes.byref(ms)), "f3d_hbm_copy")
+    return ms.value
 
 
 def last_error():

@@ -160,8 +160,8 @@ int f3d_set_cd_state(f3d_ctx *ctx, double cd);
 int f3d_sync(f3d_ctx *ctx);
 /* Tile shape (columns x rows) of the fused FP32 step kernel.  No
  * counterpart in the reference: a tuning knob set by the autotuner.
- * Supported: 32x8 (default), 32x16, 64x8, 64x4, 16x16 (column-tile kernel)
- * and 58x10, 58x18, 58x26 (row-per-wavefront kernel). */
+ * Supported: 32x8, 32x16, 64x8, 64x4, 16x16 (column-tile kernel) and
+ * 58x10 (default), 58x18, 58x26 (row-per-wavefront kernel). */
 int f3d_set_tile(f3d_ctx *ctx, int tx, int ty);
 /* HIP-event timing of the step kernels (enabled by f3d_timing(ctx,1)):
  * average duration in ms and launch count of the dominant step kernel
@@ -172,6 +172,12 @@ int f3d_kernel_time(f3d_ctx *ctx, int reset, double *avg_ms, long *count);
 long long f3d_device_bytes(const f3d_ctx *ctx);
 
 const char *f3d_last_error(void);
+
+/* No counterpart in the reference: a plain device copy of nbytes (read) +
+ * nbytes (write), repeated reps times; *avg_ms is the average launch time.
+ * It is the known-size kernel against which HBM counters (rocprofv3
+ * FETCH_SIZE / WRITE_SIZE) and the achievable bandwidth are calibrated. */
+int f3d_hbm_copy(long long nbytes, int reps, double *avg_ms);
 
 #ifdef __cplusplus
 }
