@@ -169,7 +169,7 @@ def main():
                                f"fused FP32 step", "nx": sc.nx, "ny": sc.ny, "nz": sc.nz,
                    "nc": sc.nc, "tile": args.tile, "dt_s": None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.tile),
                      "bytes_per_update": b_alg, "kernel_avg_ms": kms, "launches": kcount},
         "fp64_plain_updates_per_s": fp64,
         "speedup_vs_fp64_plain": (value / fp64) if fp64 else None,
@@ -179,6 +179,20 @@ def main():
         "hbm_copy_gbs": copy_gbs,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(tile):
+    """HBM bytes per launch of the step kernel at this tile shape, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summaries under profiles/ (scaled by the
+    copy-kernel calibration recorded there); None when not measured."""
+    import glob
+    val = None
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_traffic.md"))):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == tile:
+                val = float(f[2])
+    return val
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):

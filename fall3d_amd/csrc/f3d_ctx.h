@@ -41,7 +41,7 @@ struct Fields {
     const float *pkx;           // rho*rkh1  (diffx face products)
     const float *pky;           // rho*rkhor (diffy face products)
     const float *pkz;           // rho*rkver (diffz face products)
-    const float *vs;            // vset (nx,ny,nz,nc)
+    const float *wz;            // vz+vset per class (nx,ny,nz,nc), summed in FP64
     const float *bfac;          // bottom ghost factor (nx,ny,nc): 0, or 1+dz0*vdep/rkver(1)
     const float *tfac;          // top ghost factor (nx,ny,nc): 0 or 1 (setbcc.f90:126-130)
     const double *vsum1;        // min(0, vset(:,:,1)+vdep) (nx,ny,nc), FP64 deposit factor

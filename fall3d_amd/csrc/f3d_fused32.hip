@@ -140,7 +140,7 @@ __device__ __forceinline__ void fused_tile(const Params &p, const int q, const i
     float *cout = p.cout + (size_t)q * p.g.G;
     const float *fvx = p.f.vx, *fvy = p.f.vy, *fvz = p.f.vz, *flr = p.f.lr;
     const float *fpx = p.f.pkx, *fpy = p.f.pky, *fpz = p.f.pkz;
-    const float *vs = p.f.vs + (size_t)ic * N;
+    const float *wz = p.f.wz + (size_t)ic * N;
 
     // per-slot column data
     int kind[NSLOT];       // EDGE only: 0 other, 1 interior, 2 x-ghost, 3 y-ghost
@@ -171,7 +171,7 @@ __device__ __forceinline__ void fused_tile(const Params &p, const int q, const i
     float bm2[NSLOT], bm1[NSLOT], lrm2[NSLOT], lrm1[NSLOT], pzm2[NSLOT], pzm1[NSLOT];
     // next level's global data, loaded one level ahead so that memory latency
     // overlaps the current level's compute
-    float nc2[NSLOT], nvz[NSLOT], nvs[NSLOT], nvx[NSLOT], nvy[NSLOT];
+    float nc2[NSLOT], nvz[NSLOT], nvx[NSLOT], nvy[NSLOT];
     float nlr[NSLOT], npz[NSLOT], npx[NSLOT], npy[NSLOT];
 
     const LevelCoef z1 = zt[1];
@@ -180,7 +180,7 @@ __device__ __forceinline__ void fused_tile(const Params &p, const int q, const i
         cm1[sl] = c0[sl] = cp1[sl] = vk[sl] = W1k[sl] = W2km1[sl] = W2k[sl] = 0.f;
         gtopm[sl] = 0.f;
         bm2[sl] = bm1[sl] = lrm2[sl] = lrm1[sl] = pzm2[sl] = pzm1[sl] = 0.f;
-        nc2[sl] = nvz[sl] = nvs[sl] = nvx[sl] = nvy[sl] = 0.f;
+        nc2[sl] = nvz[sl] = nvx[sl] = nvy[sl] = 0.f;
         nlr[sl] = npz[sl] = npx[sl] = npy[sl] = 0.f;
         if (kind[sl] == 1) {
             const unsigned o = coff[sl], m = m2[sl];
@@ -192,16 +192,15 @@ __device__ __forceinline__ void fused_tile(const Params &p, const int q, const i
             cm1[sl] = gb;
             c0[sl] = c1;
             cp1[sl] = c2;
-            const float v1 = fvz[m] + vs[m];
+            const float v1 = wz[m];
             vk[sl] = v1;
             W1k[sl] = z1.dtdzz * (v1 * c1 - v1 * gb);
             W2k[sl] = w2z(gb, c1, c2, v1, v1, z1);
             W2km1[sl] = W2k[sl];
             bm1[sl] = gb;  // b(0): becomes bm2 after the level-1 shift
             nc2[sl] = (3 <= nz) ? cin[o + 3 * SZ] : 0.f;
-            nvz[sl] = fvz[NXY + m];
-            nvs[sl] = vs[NXY + m];
-            if (ctr[sl]) {
+            nvz[sl] = wz[NXY + m];
+                        if (ctr[sl]) {
                 nlr[sl] = flr[m];
                 npz[sl] = fpz[m];
             }
@@ -236,7 +235,7 @@ __device__ __forceinline__ void fused_tile(const Params &p, const int q, const i
                     float W1n, W2n, vkn = 0.f, cp2 = 0.f;
                     if (s < nz) {
                         cp2 = (s + 2 <= nz) ? nc2[sl] : gtopm[sl] * cp1[sl];  // top ghost (setbcc.f90:126-130)
-                        vkn = nvz[sl] + nvs[sl];
+                        vkn = nvz[sl];
                         W1n = zn.dtdzz * (vkn * cp1[sl] - vk[sl] * c0[sl]);
                         W2n = w2z(c0[sl], cp1[sl], cp2, vkn, vk[sl], zn);
                     } else {
@@ -256,9 +255,8 @@ __device__ __forceinline__ void fused_tile(const Params &p, const int q, const i
                     W2k[sl] = W2n;
                     if (s + 3 <= nz) nc2[sl] = cin[coff[sl] + (unsigned)(s + 3) * SZ];
                     if (s + 2 <= nz) {
-                        nvz[sl] = fvz[lv + 2 * NXY + m2[sl]];
-                        nvs[sl] = vs[lv + 2 * NXY + m2[sl]];
-                    }
+                        nvz[sl] = wz[lv + 2 * NXY + m2[sl]];
+                                            }
                 } else if (EDGE && kind[sl] >= 2) {
                     // face ghost (setbcc.f90:139-169) from the adjacent cell
                     const int li = r % RX, lj = r / RX;
@@ -632,12 +630,12 @@ __global__ void k_fields_face(Grid g, const double *vx, const double *vy, const 
 }
 
 __global__ void k_fields_class(Grid g, const double *vz, const double *vset, const double *vdep,
-                               const double *rkver, const double *dz, float *vs32, float *bfac,
+                               const double *rkver, const double *dz, float *wz, float *bfac,
                                float *tfac, double *vsum1)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int q = blockIdx.y;
-    if (t < g.N) vs32[(long)q * g.N + t] = (float)vset[(long)q * g.N + t];
+    if (t < g.N) wz[(long)q * g.N + t] = (float)(vz[t] + vset[(long)q * g.N + t]);
     if (t < g.NXY) {
         const long top = (long)(g.nz - 1) * g.NXY + t;
         const double vs1 = vset[(long)q * g.N + t];
@@ -720,7 +718,7 @@ Fields fields_view(const Grid &g, const float *b, const double *vsum1)
     f.pkx = b + 4 * g.N;
     f.pky = b + 5 * g.N;
     f.pkz = b + 6 * g.N;
-    f.vs = b + 7 * g.N;
+    f.wz = b + 7 * g.N;
     f.bfac = b + 7 * g.N + (long)g.nc * g.N;
     f.tfac = f.bfac + (long)g.nc * g.NXY;
     const float *face = f.tfac + (long)g.nc * g.NXY;
@@ -744,7 +742,7 @@ int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *v
     F3D_CHECK_LAUNCH();
     Fields f = fields_view(g, f32buf, vsum1);
     hipLaunchKernelGGL(k_fields_class, dim3((unsigned)((g.N + 255) / 256), g.nc), dim3(256), 0, s, g,
-                       vz, vset, vdep, rkver, dz, const_cast<float *>(f.vs),
+                       vz, vset, vdep, rkver, dz, const_cast<float *>(f.wz),
                        const_cast<float *>(f.bfac), const_cast<float *>(f.tfac), vsum1);
     F3D_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_fields_face, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g, vx,

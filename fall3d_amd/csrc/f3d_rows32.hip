@@ -147,7 +147,7 @@ __device__ __forceinline__ void rows_tile(const RowParams &p, const int q, const
     // diffusion planes: face products inside, raw products where lines end
     const float *fdx = EDGE ? p.f.pkx : p.f.dfx;
     const float *fdy = EDGE ? p.f.pky : p.f.dfy;
-    const float *vs = p.f.vs + (size_t)ic * N;
+    const float *wz = p.f.wz + (size_t)ic * N;
 
     const bool lane_c = li >= 2 && li < RX - 2;  // diffz region along the row
 
@@ -173,7 +173,7 @@ __device__ __forceinline__ void rows_tile(const RowParams &p, const int q, const
     float cm1[NSLOT], c0[NSLOT], cp1[NSLOT], vk[NSLOT], W1k[NSLOT], W2km1[NSLOT], W2k[NSLOT];
     float gtopm[NSLOT];
     float bm2[NSLOT], bm1[NSLOT], lrm2[NSLOT], lrm1[NSLOT], pzm2[NSLOT], pzm1[NSLOT];
-    float nc2[NSLOT], nvz[NSLOT], nvs[NSLOT], nax[NSLOT], nfx[NSLOT], nay[NSLOT], nfy[NSLOT];
+    float nc2[NSLOT], nvz[NSLOT], nax[NSLOT], nfx[NSLOT], nay[NSLOT], nfy[NSLOT];
     float nlr[NSLOT], npz[NSLOT], ndx[NSLOT], ndy[NSLOT];
 
     const LevelCoef z1 = zt[1];
@@ -184,7 +184,7 @@ __device__ __forceinline__ void rows_tile(const RowParams &p, const int q, const
         cm1[sl] = c0[sl] = cp1[sl] = vk[sl] = W1k[sl] = W2km1[sl] = W2k[sl] = 0.f;
         gtopm[sl] = 0.f;
         bm2[sl] = bm1[sl] = lrm2[sl] = lrm1[sl] = pzm2[sl] = pzm1[sl] = 0.f;
-        nc2[sl] = nvz[sl] = nvs[sl] = 0.f;
+        nc2[sl] = nvz[sl] = 0.f;
         nlr[sl] = npz[sl] = ndx[sl] = ndy[sl] = 0.f;
         const unsigned o = coff[sl], m = m2[sl];
         if (kind[sl] == 1) {
@@ -196,16 +196,15 @@ __device__ __forceinline__ void rows_tile(const RowParams &p, const int q, const
             cm1[sl] = gb;
             c0[sl] = c1;
             cp1[sl] = c2;
-            const float v1 = fvz[m] + vs[m];
+            const float v1 = wz[m];
             vk[sl] = v1;
             W1k[sl] = z1.dtdzz * (v1 * c1 - v1 * gb);
             W2k[sl] = w2z(gb, c1, c2, v1, v1, z1);
             W2km1[sl] = W2k[sl];
             bm1[sl] = gb;  // b(0): becomes bm2 after the level-1 shift
             nc2[sl] = (3 <= nz) ? cin[o + 3 * SZ] : 0.f;
-            nvz[sl] = fvz[NXY + m];
-            nvs[sl] = vs[NXY + m];
-            if (crow && lane_c) {
+            nvz[sl] = wz[NXY + m];
+                        if (crow && lane_c) {
                 nlr[sl] = flr[m];
                 npz[sl] = fpz[m];
             }
@@ -237,7 +236,7 @@ __device__ __forceinline__ void rows_tile(const RowParams &p, const int q, const
                     float W1n, W2n, vkn = 0.f, cp2 = 0.f;
                     if (s < nz) {
                         cp2 = (s + 2 <= nz) ? nc2[sl] : gtopm[sl] * cp1[sl];  // top ghost (setbcc.f90:126-130)
-                        vkn = nvz[sl] + nvs[sl];
+                        vkn = nvz[sl];
                         W1n = zn.dtdzz * (vkn * cp1[sl] - vk[sl] * c0[sl]);
                         W2n = w2z(c0[sl], cp1[sl], cp2, vkn, vk[sl], zn);
                     } else {
@@ -257,9 +256,8 @@ __device__ __forceinline__ void rows_tile(const RowParams &p, const int q, const
                     W2k[sl] = W2n;
                     if (s + 3 <= nz) nc2[sl] = cin[coff[sl] + (unsigned)(s + 3) * SZ];
                     if (s + 2 <= nz) {
-                        nvz[sl] = fvz[lv + 2 * NXY + m2[sl]];
-                        nvs[sl] = vs[lv + 2 * NXY + m2[sl]];
-                    }
+                        nvz[sl] = wz[lv + 2 * NXY + m2[sl]];
+                                            }
                 } else if (EDGE && kind[sl] >= 2) {
                     // face ghost (setbcc.f90:139-169) from the adjacent cell;
                     // dt/h*v carries the sign of the wind
