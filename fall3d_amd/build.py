@@ -33,7 +33,7 @@ UNITS = {
     "f3d_api.hip": EXACT,
     "f3d_context.hip": EXACT,
     "f3d_fused32.hip": FAST,
-    "f3d_rows32.hip": FAST,
+    "f3d_flux32.hip": FAST,
 }
 
 

@@ -33,6 +33,17 @@ void vertical_coefs(int nz, const double *zlayer, std::vector<double> &dz,
 // ---------------- FP32 fused path (f3d_fused32.hip) ----------------
 namespace f32 {
 
+// Per-level FP32 constants of the vertical operators (dt included)
+struct ZLev {
+    float dtdzz;  // dt*2/(dz(k)+dz(k-1))
+    float a2;     // dt*0.5*dz(k-1)*dzz(k)        (advctzc W2)
+    float a2bc;   // a2*dt/dz(k-1)                 (advctzc W2, Courant part)
+    float dzk;    // 1/dz(k)
+    float hldz;   // 0.5/dz(k)                      (averaged diffz face)
+    float ldz;    // 1/dz(k); level 0: 1/dz(0)
+    float pad0, pad1;
+};
+
 // FP32 step fields, built from the FP64 coefficients after a meteo read.
 // 3-D fields share the (nx,ny,nz) layout; per-class fields are stacked.
 struct Fields {
@@ -47,13 +58,19 @@ struct Fields {
     const double *vsum1;        // min(0, vset(:,:,1)+vdep) (nx,ny,nc), FP64 deposit factor
     // class-independent face coefficients (dt-dependent), see k_fields_face
     const float *ax, *fx, *ay, *fy, *dfx, *dfy;
+    // the same, interleaved per cell for the flux kernel: one 16-byte load
+    // per cell for each group
+    const float4 *cA;  // (ax, fx, ay, fy)
+    const float4 *cD;  // (dfx, dfy, 1/rho, rho*rkver)
+    const float2 *cE;  // (rho*rkh1, rho*rkhor)
+    const ZLev *zl;    // per level 0..nz+1
 };
 
 // Build Fields from the FP64 coefficient arrays (meteo.f90:108-112 applied).
 int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *vy,
                  const double *vz, const double *rho, const double *rkh1, const double *rkhor,
                  const double *rkver, const double *vset, const double *vdep, const double *dz,
-                 double dt, float *f32buf, double *vsum1);
+                 const ZCoef *zc, double dt, float *f32buf, double *vsum1);
 size_t fields_floats(const Grid &g);
 Fields fields_view(const Grid &g, const float *f32buf, const double *vsum1);
 
@@ -68,7 +85,7 @@ struct StepArgs {
 };
 
 // Tile geometry of the fused kernel (chosen by the autotuner).  tx = 58
-// selects the row-per-wave kernel (64-lane region rows, ty interior rows).
+// selects the flux kernel (64-lane region rows, ty interior rows).
 struct TileShape {
     int tx, ty;
 };
@@ -76,10 +93,10 @@ struct TileShape {
 // One fused launch: setbcc + advctzc + advctx/advcty (parity order) + diffz
 // + diffx/diffy (parity order) + accum + clip for every class, c_in -> c_out.
 int fused_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts);
-// Row-per-wavefront form of the same step (f3d_rows32.hip): tiles 58 x ty
-// with ty in {10, 18, 26}.
-bool rows_supported(TileShape ts);
-int rows_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts);
+// Flux form of the same step (f3d_flux32.hip): tiles 58 x ty with ty in
+// {10, 18, 26}.
+bool flux_supported(TileShape ts);
+int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts);
 // Ghost frame of c_out (setbcc values of c_in, clipped) and cmass2d
 // outflux; outmass[m] += member m's outflux (deterministic per member).
 int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,

@@ -629,6 +629,37 @@ __global__ void k_fields_face(Grid g, const double *vx, const double *vy, const 
     dfy[t] = (float)(dt / (g.dy * g.dy) * 0.5 * (rho[t] * rkhor[t] + rho[ty] * rkhor[ty]));
 }
 
+// interleave the per-cell groups read together by the flux kernel
+__global__ void k_fields_pack(long n, Fields f, float4 *cA, float4 *cD, float2 *cE)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    cA[t] = make_float4(f.ax[t], f.fx[t], f.ay[t], f.fy[t]);
+    cD[t] = make_float4(f.dfx[t], f.dfy[t], f.lr[t], f.pkz[t]);
+    cE[t] = make_float2(f.pkx[t], f.pky[t]);
+}
+
+// per-level constants of advctzc (advctzc.f90:153-181) and diffz
+// (diffz.f90:19-62), products formed in FP64 and rounded once
+__global__ void k_zlev(int nz, const ZCoef *zc, double dt, ZLev *zl)
+{
+    for (int k = threadIdx.x; k <= nz + 1; k += blockDim.x) {
+        ZLev z{};
+        if (k >= 1 && k <= nz) {
+            const ZCoef c = zc[k];
+            const double a2 = dt * 0.5 * c.dzm1 * c.dzz;
+            z.dtdzz = (float)(dt * c.dzz);
+            z.a2 = (float)a2;
+            z.a2bc = (float)(a2 * dt * c.dzk_1);
+            z.dzk = (float)c.dzk;
+            z.hldz = (float)(0.5 * c.dzk);
+            z.ldz = (float)c.dzk;
+        }
+        if (k == 0) z.ldz = (float)zc[1].dzk_1;
+        zl[k] = z;
+    }
+}
+
 __global__ void k_fields_class(Grid g, const double *vz, const double *vset, const double *vdep,
                                const double *rkver, const double *dz, float *wz, float *bfac,
                                float *tfac, double *vsum1)
@@ -703,14 +734,28 @@ int launch_fused(hipStream_t s, const Params &p0)
 
 }  // namespace
 
-size_t fields_floats(const Grid &g)
+// the interleaved flux-kernel groups lead the buffer (16-byte aligned),
+// followed by the z-level table and the separate fields
+static size_t separate_offset(const Grid &g)
 {
-    return (size_t)13 * g.N + (size_t)g.nc * g.N + (size_t)2 * g.nc * g.NXY;
+    const size_t zl = (size_t)(g.nz + 2) * (sizeof(ZLev) / sizeof(float));
+    return ((size_t)10 * g.N + zl + 3) & ~(size_t)3;
 }
 
-Fields fields_view(const Grid &g, const float *b, const double *vsum1)
+size_t fields_floats(const Grid &g)
+{
+    return separate_offset(g) + (size_t)13 * g.N + (size_t)g.nc * g.N +
+           (size_t)2 * g.nc * g.NXY;
+}
+
+Fields fields_view(const Grid &g, const float *b0, const double *vsum1)
 {
     Fields f;
+    f.cA = reinterpret_cast<const float4 *>(b0);
+    f.cD = reinterpret_cast<const float4 *>(b0 + 4 * g.N);
+    f.cE = reinterpret_cast<const float2 *>(b0 + 8 * g.N);
+    f.zl = reinterpret_cast<const ZLev *>(b0 + 10 * g.N);
+    const float *b = b0 + separate_offset(g);
     f.vx = b;
     f.vy = b + g.N;
     f.vz = b + 2 * g.N;
@@ -735,12 +780,13 @@ Fields fields_view(const Grid &g, const float *b, const double *vsum1)
 int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *vy,
                  const double *vz, const double *rho, const double *rkh1, const double *rkhor,
                  const double *rkver, const double *vset, const double *vdep, const double *dz,
-                 double dt, float *f32buf, double *vsum1)
+                 const ZCoef *zc, double dt, float *f32buf, double *vsum1)
 {
-    hipLaunchKernelGGL(k_fields3d, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g.N, vx,
-                       vy, vz, rho, rkh1, rkhor, rkver, f32buf);
-    F3D_CHECK_LAUNCH();
     Fields f = fields_view(g, f32buf, vsum1);
+    // vx .. pkz are contiguous from f.vx
+    hipLaunchKernelGGL(k_fields3d, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g.N, vx,
+                       vy, vz, rho, rkh1, rkhor, rkver, const_cast<float *>(f.vx));
+    F3D_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_fields_class, dim3((unsigned)((g.N + 255) / 256), g.nc), dim3(256), 0, s, g,
                        vz, vset, vdep, rkver, dz, const_cast<float *>(f.wz),
                        const_cast<float *>(f.bfac), const_cast<float *>(f.tfac), vsum1);
@@ -750,6 +796,12 @@ int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *v
                        const_cast<float *>(f.fx), const_cast<float *>(f.ay),
                        const_cast<float *>(f.fy), const_cast<float *>(f.dfx),
                        const_cast<float *>(f.dfy));
+    F3D_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_fields_pack, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g.N, f,
+                       const_cast<float4 *>(f.cA), const_cast<float4 *>(f.cD),
+                       const_cast<float2 *>(f.cE));
+    F3D_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_zlev, dim3(1), dim3(64), 0, s, g.nz, zc, dt, const_cast<ZLev *>(f.zl));
     F3D_CHECK_LAUNCH();
     return F3D_OK;
 }
@@ -770,7 +822,7 @@ int fused_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
     p.dt64 = a.dt;
     p.dt = (float)a.dt;
     p.odd = (a.iiter % 2) != 0;
-    if (rows_supported(ts)) return rows_step(s, a, f, ts);
+    if (flux_supported(ts)) return flux_step(s, a, f, ts);
     if (ts.tx == 32 && ts.ty == 8) return launch_fused<32, 8>(s, p);
     if (ts.tx == 32 && ts.ty == 16) return launch_fused<32, 16>(s, p);
     if (ts.tx == 64 && ts.ty == 8) return launch_fused<64, 8>(s, p);
