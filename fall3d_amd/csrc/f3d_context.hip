@@ -697,17 +697,18 @@ int f3d_sync(f3d_ctx *c)
     return F3D_OK;
 }
 
-int f3d_set_tile(f3d_ctx *c, int tx, int ty)
+int f3d_set_tile(f3d_ctx *c, int tx, int ty) { return f3d_set_tile_waves(c, tx, ty, 0); }
+
+int f3d_set_tile_waves(f3d_ctx *c, int tx, int ty, int waves)
 {
     if (!check_ctx(c)) return F3D_EINVAL;
-    const int ok[][2] = {{32, 8}, {32, 16}, {64, 8}, {64, 4}, {16, 16}, {58, 10}, {58, 18}, {58, 26}};
-    for (auto &t : ok)
-        if (t[0] == tx && t[1] == ty) {
-            c->tile = f32::TileShape{tx, ty};
-            return F3D_OK;
-        }
-    set_error("f3d_set_tile: unsupported tile %dx%d", tx, ty);
-    return F3D_EINVAL;
+    const f32::TileShape ts{tx, ty, waves};
+    if (!f32::tile_supported(ts)) {
+        set_error("f3d_set_tile: unsupported tile %dx%d (waves %d)", tx, ty, waves);
+        return F3D_EINVAL;
+    }
+    c->tile = ts;
+    return F3D_OK;
 }
 
 int f3d_timing(f3d_ctx *c, int enable)

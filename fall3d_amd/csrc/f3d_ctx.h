@@ -64,6 +64,8 @@ struct Fields {
     const float4 *cD;  // (dfx, dfy, 1/rho, rho*rkver)
     const float2 *cE;  // (rho*rkh1, rho*rkhor)
     const ZLev *zl;    // per level 0..nz+1
+    const float *base;  // the buffer all FP32 fields live in, and its size
+    size_t bytes;
 };
 
 // Build Fields from the FP64 coefficient arrays (meteo.f90:108-112 applied).
@@ -88,7 +90,9 @@ struct StepArgs {
 // selects the flux kernel (64-lane region rows, ty interior rows).
 struct TileShape {
     int tx, ty;
+    int nw = 0;  // waves per workgroup of the flux kernel (0: its default)
 };
+bool tile_supported(TileShape ts);
 
 // One fused launch: setbcc + advctzc + advctx/advcty (parity order) + diffz
 // + diffx/diffy (parity order) + accum + clip for every class, c_in -> c_out.

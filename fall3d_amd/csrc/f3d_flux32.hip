@@ -41,19 +41,18 @@ struct FluxParams {
     const float *cin;
     float *cout;
     double *cdry;
-    const float4 *cA;  // (ax, fx, ay, fy)
-    const float4 *cD;  // (dfx, dfy, 1/rho, rho*rkver)
-    const float2 *cE;  // (rho*rkh1, rho*rkhor), raw products for line ends
-    const float *wz;   // vz+vset per class
-    const float *bfac, *tfac;
+    // all FP32 fields live in one buffer; byte offsets of the groups in it:
+    // cA (ax, fx, ay, fy), cD (dfx, dfy, 1/rho, rho*rkver), cE (rho*rkh1,
+    // rho*rkhor: raw products for line ends), wz (vz+vset, class 0), bfac and
+    // tfac (class 0)
+    const float *fbase;
+    unsigned fbytes;
+    unsigned oA, oD, oE, oW, oBF, oTF;
     const double *vsum1;
     const ZLev *zl;
     double dt64;
     float dt, dtdx2, dtdy2;
     int ntx, nty;
-    // interior tiles form the rectangle [bx0, bx1] x [by0, by1]; the others
-    // (regions meeting the domain frame) go to the edge kernel
-    int bx0, bx1, by0, by1;
     int nlogical;
     int per_xcd;
 };
@@ -110,82 +109,146 @@ __device__ __forceinline__ float diff_x(float d, float lr, float D, float pk, fl
     return inner ? d + (gr - gl) : d;
 }
 
-template <int RY, int NW, bool ODD, bool EX, bool EY>
-__device__ __forceinline__ void flux_tile(const FluxParams &p, const int q, const int ic,
-                                          const int mem, const int gi0, const int gj0, float *sm)
+// Buffer resources (gfx9 raw buffers: stride 0, range in bytes): loads and
+// stores take the lane's 32-bit byte offset in a VGPR and the level offset
+// in an SGPR, so no per-lane 64-bit address is ever formed.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+
+__device__ __forceinline__ rsrc_t mkbuf(const void *base, size_t bytes)
 {
-    constexpr int NS = RY / NW;
-    constexpr bool EDGE = EX || EY;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0,
+                                             (int)(unsigned)bytes, 0x00020000);
+}
+__device__ __forceinline__ float bld(rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ float2 bld2(rsrc_t r, unsigned voff, unsigned soff)
+{
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+__device__ __forceinline__ float4 bld4(rsrc_t r, unsigned voff, unsigned soff)
+{
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
+                       __uint_as_float(v.w));
+}
+__device__ __forceinline__ void bst(float x, rsrc_t r, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, (int)voff, (int)soff, 0);
+}
+
+// per-row flags of edge tiles
+enum : unsigned {
+    F_INNER = 1u,  // domain cell
+    F_XLO = 2u, F_XHI = 4u, F_YLO = 8u, F_YHI = 16u,  // first/last domain column/row
+    F_GHOST = 32u,  // setbcc face ghost
+    F_GX = 64u,     // ... on an x face (else a y face)
+    F_GLO = 128u,   // ... on the low face (i = 0 or j = 0)
+};
+
+template <int RY, int NW, bool ODD, bool EX, bool EY>
+struct FluxTile {
+    static constexpr int NS = RY / NW;
+    static constexpr bool EDGE = EX || EY;
     static_assert(RY % NW == 0, "region rows must divide evenly over the waves");
-    const int nx = p.g.nx, ny = p.g.ny, nz = p.g.nz;
-    const unsigned NXY = (unsigned)p.g.NXY;
-    const unsigned SZ = (unsigned)((nx + 2) * (ny + 2));
-    const int li = threadIdx.x & (RX - 1);
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int gi = gi0 + li;
-    float *SP = sm, *SF = sm + RX * RY, *SQ = sm + 2 * RX * RY;
 
-    const float *cin = p.cin + (size_t)q * p.g.G;
-    float *cout = p.cout + (size_t)q * p.g.G;
-    const float *wz = p.wz + (size_t)ic * p.g.N;
-    const float *cDf = reinterpret_cast<const float *>(p.cD);
-    const float dt = p.dt;
+    const FluxParams &p;
+    int nz, li, w, ic, mem;
+    unsigned NXY, SZ;
+    float *SP, *SF, *SQ;
+    // class slabs of c_in and c_out, the field buffer; class offsets of the
+    // per-class fields in it
+    rsrc_t rin, rout, rf;
+    unsigned oW, oBF, oTF;
 
-    // per-row geometry and flags
-    unsigned coff[NS], moff[NS], mup[NS], mdn[NS];
-    bool inner[NS], xlo[NS], xhi[NS], ylo[NS], yhi[NS], gst[NS], gzx[NS];
+    // per-row geometry: byte offsets of the (clamped) column in c, in the
+    // meteo arrays and of the rows above/below; edge flags
+    unsigned co4[NS], m4[NS], m16[NS], mup16[NS], mdn8[NS], mup8[NS], fl[NS];
     int lj[NS];
+    // advctzc window: c(s), c(s+1), w(s), U(s), W2(s), delta(s); after level nz
+    // c0 holds the top ghost c(nz+1).  Prefetched c(s+2), w(s+1).
+    float c0[NS], cp1[NS], vk[NS], Uc[NS], W2k[NS], dprev[NS], nc2[NS], nwz[NS];
+    // (ax, fx, ay, fy) for two consecutive levels and (dfx, dfy, lr, pz) for
+    // two; the level parity picks the current one
+    float4 A[2][NS], D[2][NS];
+    // diffz window: b(k), q(k) = b/rho and the face flux it leaves for the
+    // level above (before level 1: b(0))
+    float b0[NS], q0[NS], gprev[NS];
+    // carried between phase 3 and the next phase 2
+    float dc[NS], qc[NS], dyl[NS], dyu[NS], pyo[NS], pym[NS], pyp[NS], lrc[NS], dxc[NS], pxc[NS];
+    float P[NS], phi[NS];
+
+    __device__ __forceinline__ FluxTile(const FluxParams &pp, int q, int ic_, int mem_, int gi0,
+                                        int gj0, float *sm)
+        : p(pp)
+    {
+        const int nx = p.g.nx, ny = p.g.ny;
+        nz = p.g.nz;
+        ic = ic_;
+        mem = mem_;
+        NXY = (unsigned)p.g.NXY;
+        SZ = (unsigned)((nx + 2) * (ny + 2));
+        li = threadIdx.x & (RX - 1);
+        w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        SP = sm;
+        SF = sm + RX * RY;
+        SQ = sm + 2 * RX * RY;
+        const size_t G = (size_t)p.g.G, N = (size_t)p.g.N;
+        rin = mkbuf(p.cin + (size_t)q * G, 4 * G);
+        rout = mkbuf(p.cout + (size_t)q * G, 4 * G);
+        rf = mkbuf(p.fbase, p.fbytes);
+        oW = p.oW + 4u * (unsigned)(ic * N);
+        oBF = p.oBF + 4u * (unsigned)ic * NXY;
+        oTF = p.oTF + 4u * (unsigned)ic * NXY;
+        const int gi = gi0 + li;
 #pragma unroll
-    for (int sl = 0; sl < NS; ++sl) {
-        lj[sl] = w * NS + sl;
-        const int gj = gj0 + lj[sl];
-        const int ci = EX ? min(max(gi, 1), nx) : gi;
-        const int cj = EY ? min(max(gj, 1), ny) : gj;
-        coff[sl] = (unsigned)cidx(nx, ny, ci, cj, 0);
-        moff[sl] = (unsigned)((ci - 1) + nx * (cj - 1));
-        // row above / below (clamped), for the face j+1/2 and line-end products
-        mup[sl] = (unsigned)((ci - 1) + nx * (min(cj + 1, ny) - 1));
-        mdn[sl] = (unsigned)((ci - 1) + nx * (max(cj - 1, 1) - 1));
-        const bool inx = !EX || (gi >= 1 && gi <= nx);
-        const bool iny = !EY || (gj >= 1 && gj <= ny);
-        inner[sl] = inx && iny;
-        xlo[sl] = EX && gi == 1;
-        xhi[sl] = EX && gi == nx;
-        ylo[sl] = EY && gj == 1;
-        yhi[sl] = EY && gj == ny;
-        // setbcc face ghosts (setbcc.f90:139-169): x faces inside the y range,
-        // y faces inside the x range; gzx selects which wind decides
-        const bool xg = EX && iny && (gi == 0 || gi == nx + 1);
-        const bool yg = EY && inx && (gj == 0 || gj == ny + 1);
-        gst[sl] = xg || yg;
-        gzx[sl] = xg;
+        for (int sl = 0; sl < NS; ++sl) {
+            lj[sl] = w * NS + sl;
+            const int gj = gj0 + lj[sl];
+            const int ci = EX ? min(max(gi, 1), nx) : gi;
+            const int cj = EY ? min(max(gj, 1), ny) : gj;
+            const unsigned mo = (unsigned)((ci - 1) + nx * (cj - 1));
+            co4[sl] = 4u * (unsigned)cidx(nx, ny, ci, cj, 0);
+            m4[sl] = 4u * mo;
+            m16[sl] = 16u * mo;
+            const unsigned mu = (unsigned)((ci - 1) + nx * (min(cj + 1, ny) - 1));
+            const unsigned md = (unsigned)((ci - 1) + nx * (max(cj - 1, 1) - 1));
+            mup16[sl] = 16u * mu;
+            mup8[sl] = 8u * mu;
+            mdn8[sl] = 8u * md;
+            unsigned f = 0;
+            const bool inx = !EX || (gi >= 1 && gi <= nx);
+            const bool iny = !EY || (gj >= 1 && gj <= ny);
+            if (inx && iny) f |= F_INNER;
+            if (EX && gi == 1) f |= F_XLO;
+            if (EX && gi == nx) f |= F_XHI;
+            if (EY && gj == 1) f |= F_YLO;
+            if (EY && gj == ny) f |= F_YHI;
+            // setbcc face ghosts (setbcc.f90:139-169): x faces inside the y
+            // range, y faces inside the x range
+            if (EX && iny && (gi == 0 || gi == nx + 1)) f |= F_GHOST | F_GX | (gi == 0 ? F_GLO : 0u);
+            if (EY && inx && (gj == 0 || gj == ny + 1)) f |= F_GHOST | (gj == 0 ? F_GLO : 0u);
+            fl[sl] = f;
+        }
     }
 
-    // advctzc column window (per row): c(s), c(s+1), w(s), U(s), W2(s), delta(s);
-    // after level nz, c0 holds the top ghost c(nz+1)
-    float c0[NS], cp1[NS], vk[NS], Uc[NS], W2k[NS], dprev[NS];
-    float nc2[NS], nwz[NS];
-    float4 Ac[NS], An[NS];  // (ax, fx, ay, fy) at level s, prefetched level s+1
-    // diffz window: b(k), q(k) = b/rho and the face flux gb(k+1) it leaves for
-    // the level above (before level 1: b(0))
-    float b0[NS], q0[NS], gprev[NS];
-    float4 D0[NS], Dn[NS];  // (dfx, dfy, lr, pz) at levels k and s
-    // carried between phase 3 and the next phase 2: level k state and q,
-    // face products of diffy at j-1/2 and j+1/2, raw products (EY)
-    float dc[NS], qc[NS], dyl[NS], dyu[NS], pyo[NS], pym[NS], pyp[NS], lrc[NS], dxc[NS], pxc[NS];
-    float P[NS], phi[NS], apl[NS], apr[NS];
+    __device__ __forceinline__ bool has(int sl, unsigned b) const { return (fl[sl] & b) != 0u; }
+    __device__ __forceinline__ bool inner(int sl) const { return !EDGE || has(sl, F_INNER); }
 
+    __device__ __forceinline__ void init()
     {
         const ZLev z1 = p.zl[1];
         const float ldz0 = p.zl[0].ldz;
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
-            const unsigned o = coff[sl], m = moff[sl];
-            const float c1 = cin[o + SZ];
-            const float c2 = cin[o + 2 * SZ];
-            const float bf = p.bfac[(size_t)ic * NXY + m];
-            const float gb = c1 * bf;  // c(0), setbcc.f90:108-113
-            const float v1 = wz[m];
+            const float c1 = bld(rin, co4[sl], 4 * SZ);
+            const float c2 = bld(rin, co4[sl], 8 * SZ);
+            const float gb = c1 * bld(rf, m4[sl], oBF);  // c(0), setbcc.f90:108-113
+            const float v1 = bld(rf, m4[sl], oW);
             c0[sl] = c1;
             cp1[sl] = c2;
             vk[sl] = v1;
@@ -196,30 +259,40 @@ __device__ __forceinline__ void flux_tile(const FluxParams &p, const int q, cons
             Uc[sl] = z1.dtdzz * (v1 * c1 - v1 * gb);  // U(1) = W1(1)
             dprev[sl] = d1;
             gprev[sl] = gb;
-            nc2[sl] = (3 <= nz) ? cin[o + 3 * SZ] : 0.f;
-            nwz[sl] = wz[NXY + m];
-            An[sl] = p.cA[m];
+            nc2[sl] = (3 <= nz) ? bld(rin, co4[sl], 12 * SZ) : 0.f;
+            nwz[sl] = bld(rf, m4[sl], oW + 4 * NXY);
+            A[0][sl] = bld4(rf, m16[sl], p.oA);  // level 1
             b0[sl] = q0[sl] = 0.f;
-            D0[sl] = Dn[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
             dc[sl] = qc[sl] = dyl[sl] = dyu[sl] = pyo[sl] = pym[sl] = pyp[sl] = 0.f;
             lrc[sl] = dxc[sl] = pxc[sl] = 0.f;
-            P[sl] = phi[sl] = apl[sl] = apr[sl] = 0.f;
+            P[sl] = phi[sl] = 0.f;
         }
     }
-    const float dtop = dt * p.zl[nz].ldz;
 
-    for (int s = 1; s <= nz + 2; ++s) {
+    // One level of the pipeline.  STEADY promises 4 <= s <= nz-3 (no level
+    // is at a boundary of any stage).
+    template <bool STEADY, int PAR>
+    __device__ __forceinline__ void level(const int s)
+    {
+        const int nzl = nz;
+        float4 *Ac = A[PAR], *An = A[1 - PAR];  // levels s, s+1
+        float4 *Dn = D[PAR], *D0 = D[1 - PAR];   // levels s, s-1
+        const float dt = p.dt;
+        const float dtop = dt * p.zl[nzl].ldz;
+
         // ======== phase 1: advctzc at level s (and advctx on even steps) ========
-        if (s <= nz) {
+        if (STEADY || s <= nzl) {
             const ZLev zn = p.zl[s + 1];
+            const unsigned lz = 4u * NXY * (unsigned)s;  // byte offset of level s+1 in a 3-D field
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
-                Ac[sl] = An[sl];
-                Dn[sl] = p.cD[(unsigned)(s - 1) * NXY + moff[sl]];  // used after two barriers
+                Dn[sl] = bld4(rf, m16[sl], p.oD + 4u * (lz - 4u * NXY));  // level s, used two barriers later
                 float cp2 = 0.f, vkn = 0.f, W1n, W2n, dn = 0.f;
-                if (s < nz) {
-                    // top ghost c(nz+1) (setbcc.f90:126-130)
-                    cp2 = (s + 2 <= nz) ? nc2[sl] : p.tfac[(size_t)ic * NXY + moff[sl]] * cp1[sl];
+                if (STEADY || s < nzl) {
+                    // top ghost c(nz+1) = tfac*c(nz) (setbcc.f90:126-130)
+                    cp2 = (STEADY || s + 2 <= nzl)
+                              ? nc2[sl]
+                              : bld(rf, m4[sl], oTF) * cp1[sl];
                     vkn = nwz[sl];
                     W1n = zn.dtdzz * (vkn * cp1[sl] - vk[sl] * c0[sl]);
                     dn = (cp2 - cp1[sl]) * zn.dzk;
@@ -232,11 +305,11 @@ __device__ __forceinline__ void flux_tile(const FluxParams &p, const int q, cons
                 const float Un = W1n + (W2n - W2k[sl]);
                 float a = c0[sl] - ((vk[sl] > 0.f) ? Uc[sl] : Un);
                 if (EDGE) {
-                    // ghost faces: zero when the wind at the face cell blows inward
-                    const bool gi0s = (gi == 0), gj0s = (gj0 + lj[sl] == 0);
-                    const float v = gzx[sl] ? Ac[sl].x : Ac[sl].z;
-                    const bool zero = gzx[sl] ? (gi0s ? v >= 0.f : v <= 0.f) : (gj0s ? v >= 0.f : v <= 0.f);
-                    a = inner[sl] ? a : (gst[sl] && !zero ? c0[sl] : 0.f);
+                    // ghost faces: zero when the wind at the face cell blows
+                    // out of the domain through it, else the adjacent cell
+                    const float v = has(sl, F_GX) ? Ac[sl].x : Ac[sl].z;
+                    const bool zero = has(sl, F_GLO) ? v >= 0.f : v <= 0.f;
+                    a = inner(sl) ? a : (has(sl, F_GHOST) && !zero ? c0[sl] : 0.f);
                 }
                 c0[sl] = cp1[sl];
                 cp1[sl] = cp2;
@@ -244,11 +317,11 @@ __device__ __forceinline__ void flux_tile(const FluxParams &p, const int q, cons
                 Uc[sl] = Un;
                 W2k[sl] = W2n;
                 dprev[sl] = dn;
-                // prefetch level s+1 coefficients, c(s+3), w(s+2)
-                if (s + 3 <= nz) nc2[sl] = cin[coff[sl] + (unsigned)(s + 3) * SZ];
-                if (s + 2 <= nz) nwz[sl] = wz[(unsigned)(s + 1) * NXY + moff[sl]];
-                if (s + 1 <= nz) An[sl] = p.cA[(unsigned)s * NXY + moff[sl]];
-                if (!ODD) a = adv_x<EX>(a, Ac[sl].x, Ac[sl].y, inner[sl], xlo[sl], xhi[sl]);
+                if (STEADY || s + 3 <= nzl) nc2[sl] = bld(rin, co4[sl], 4u * SZ * (unsigned)(s + 3));
+                if (STEADY || s + 2 <= nzl) nwz[sl] = bld(rf, m4[sl], oW + lz + 4u * NXY);
+                if (STEADY || s + 1 <= nzl) An[sl] = bld4(rf, m16[sl], p.oA + 4u * lz);
+                if (!ODD)
+                    a = adv_x<EX>(a, Ac[sl].x, Ac[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
                 P[sl] = a;
             }
 #pragma unroll
@@ -258,7 +331,7 @@ __device__ __forceinline__ void flux_tile(const FluxParams &p, const int q, cons
         __syncthreads();
 
         // ======== phase 2: advcty fluxes at level s; last sweep of level s-2 ========
-        if (s <= nz) {
+        if (STEADY || s <= nzl) {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 if (lj[sl] < 1 || lj[sl] >= RY - 1) continue;  // wave-uniform
@@ -266,16 +339,12 @@ __device__ __forceinline__ void flux_tile(const FluxParams &p, const int q, cons
                 const float Pp = (sl < NS - 1) ? P[sl + 1] : SP[(lj[sl] + 1) * RX + li];
                 const float w2 = Ac[sl].w * minmod(Pp - P[sl], P[sl] - Pm);
                 phi[sl] = fmaf(Ac[sl].z, P[sl], w2);
-                if (EY) {
-                    apl[sl] = fmaf(Ac[sl].z, Pm, w2);
-                    apr[sl] = fmaf(Ac[sl].z, Pp, w2);
-                }
                 if (sl == 0 || sl == NS - 1) SF[lj[sl] * RX + li] = phi[sl];
             }
         }
-        if (s >= 3) {
+        if (STEADY || s >= 3) {
             const int k = s - 2;
-            float *co = cout + (unsigned)k * SZ;
+            const unsigned ko = 4u * SZ * (unsigned)k;
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 if (lj[sl] < 3 || lj[sl] >= RY - 3) continue;  // wave-uniform
@@ -287,22 +356,21 @@ __device__ __forceinline__ void flux_tile(const FluxParams &p, const int q, cons
                 float ghi = Du * (qp - qc[sl]);
                 if (EY) {
                     const float d2 = p.dtdy2;
-                    glo = ylo[sl] ? fmaxf(0.f, d2 * pyo[sl] * (qc[sl] - qm))
-                                  : (yhi[sl] ? d2 * pym[sl] * (qc[sl] - qm) : glo);
-                    ghi = ylo[sl] ? d2 * pyp[sl] * (qp - qc[sl])
-                                  : (yhi[sl] ? fminf(0.f, d2 * pyo[sl] * (qp - qc[sl])) : ghi);
+                    const bool lo = has(sl, F_YLO), hi = has(sl, F_YHI);
+                    glo = lo ? fmaxf(0.f, d2 * pyo[sl] * (qc[sl] - qm)) : (hi ? d2 * pym[sl] * (qc[sl] - qm) : glo);
+                    ghi = lo ? d2 * pyp[sl] * (qp - qc[sl]) : (hi ? fminf(0.f, d2 * pyo[sl] * (qp - qc[sl])) : ghi);
                 }
                 float out = dc[sl] + (ghi - glo);
-                if (EDGE) out = inner[sl] ? out : dc[sl];
+                if (EDGE) out = inner(sl) ? out : dc[sl];
                 if (ODD)
-                    out = diff_x<EX>(out, lrc[sl], dxc[sl], pxc[sl], p.dtdx2, inner[sl], xlo[sl],
-                                     xhi[sl]);
-                const bool store = li >= 3 && li < RX - 3 && (!EDGE || inner[sl]);
+                    out = diff_x<EX>(out, lrc[sl], dxc[sl], pxc[sl], p.dtdx2, inner(sl), has(sl, F_XLO),
+                                     has(sl, F_XHI));
+                const bool store = li >= 3 && li < RX - 3 && inner(sl);
                 if (store) {
-                    co[coff[sl]] = fmaxf(0.f, out);  // sizloop.f90:77-78
-                    if (k == 1) {
+                    bst(fmaxf(0.f, out), rout, co4[sl], ko);  // sizloop.f90:77-78
+                    if (!STEADY && k == 1) {
                         // accum.f90:22: deposit from c(:,:,1) before the clip, FP64
-                        const size_t t2 = moff[sl];
+                        const size_t t2 = m4[sl] / 4u;
                         double *cd = p.cdry + ((size_t)mem * (p.g.nc + 1) + ic + 1) * NXY + t2;
                         *cd = *cd - p.dt64 * (double)out * p.vsum1[(size_t)ic * NXY + t2];
                     }
@@ -313,87 +381,138 @@ __device__ __forceinline__ void flux_tile(const FluxParams &p, const int q, cons
 
         // ======== phase 3: advcty (+ advctx on odd steps) at level s; diffz and
         // the first horizontal diffusion at level s-1 ========
-        if (s >= nz + 2) continue;
+        if (!STEADY && s >= nzl + 2) return;
         const int k = s - 1;
         const ZLev zk = p.zl[k > 0 ? k : 0];
+        const unsigned lk = 4u * NXY * (unsigned)(k > 0 ? k - 1 : 0);  // level k in a 3-D field
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
             if (lj[sl] < 2 || lj[sl] >= RY - 2) continue;  // wave-uniform
             float bnew;
-            if (s <= nz) {
+            if (STEADY || s <= nzl) {
+                const float Pm = (sl > 0) ? P[sl - 1] : SP[(lj[sl] - 1) * RX + li];
+                const float Pp = (sl < NS - 1) ? P[sl + 1] : SP[(lj[sl] + 1) * RX + li];
                 float pl = (sl > 0) ? phi[sl - 1] : SF[(lj[sl] - 1) * RX + li];
                 float pr = (sl < NS - 1) ? phi[sl + 1] : SF[(lj[sl] + 1) * RX + li];
+                const float ay = Ac[sl].z;
                 if (EY) {
-                    pl = ylo[sl] ? apl[sl] : pl;
-                    pr = yhi[sl] ? apr[sl] : pr;
+                    // advcty.f90:49-51: W2 clamps to the line at j = 1 / ny,
+                    // A at the ghost row is v(1) / v(ny)
+                    const float w2 = phi[sl] - ay * P[sl];
+                    pl = has(sl, F_YLO) ? fmaf(ay, Pm, w2) : pl;
+                    pr = has(sl, F_YHI) ? fmaf(ay, Pp, w2) : pr;
                 }
-                bnew = (Ac[sl].z > 0.f) ? P[sl] + (pl - phi[sl]) : P[sl] + (phi[sl] - pr);
-                if (EDGE) bnew = inner[sl] ? bnew : P[sl];
-                if (ODD) bnew = adv_x<EX>(bnew, Ac[sl].x, Ac[sl].y, inner[sl], xlo[sl], xhi[sl]);
+                bnew = (ay > 0.f) ? P[sl] + (pl - phi[sl]) : P[sl] + (phi[sl] - pr);
+                if (EDGE) bnew = inner(sl) ? bnew : P[sl];
+                if (ODD)
+                    bnew = adv_x<EX>(bnew, Ac[sl].x, Ac[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
             } else {
                 bnew = c0[sl];  // b(nz+1): the top ghost of c_in
             }
             // diffz (diffz.f90:19-62) at level k = s-1, input b(k+1) = bnew
             const float lrn = Dn[sl].z, pzn = Dn[sl].w;  // level s (unused at s = nz+1)
-            if (s == 1) {
+            if (!STEADY && s == 1) {
                 // bottom face: c(0)/rho(1), un-averaged rho*K(1), inflow only
                 const float q1 = bnew * lrn;
                 gprev[sl] = fmaxf(0.f, pzn * (q1 - gprev[sl] * lrn) * p.zl[0].ldz);
                 b0[sl] = bnew;
                 q0[sl] = q1;
-                D0[sl] = Dn[sl];
                 continue;
             }
-            const float lr0 = D0[sl].z, pz0 = D0[sl].w;
+            const float4 Dk = D0[sl];
+            const float lr0 = Dk.z, pz0 = Dk.w;
             float d, qn = 0.f;
-            if (k < nz) {
+            if (STEADY || k < nzl) {
                 qn = bnew * lrn;
                 const float dq = qn - q0[sl];
                 const float gav = (pzn + pz0) * zk.hldz * dq;
                 // level 1 uses the un-averaged rho*K(2) on its top face
-                const float gt = (k == 1) ? pzn * dq * zk.ldz : gav;
+                const float gt = (!STEADY && k == 1) ? pzn * dq * zk.ldz : gav;
                 d = b0[sl] + zk.dtdzz * (gt - gprev[sl]);
                 // level nz takes the un-averaged rho*K(nz-1) on its bottom face
-                gprev[sl] = (k == nz - 1) ? pz0 * dq * zk.ldz : gav;
+                gprev[sl] = (!STEADY && k == nzl - 1) ? pz0 * dq * zk.ldz : gav;
             } else {
                 const float gt = fminf(0.f, pz0 * (bnew * lr0 - q0[sl]) * zk.ldz);
                 d = b0[sl] + dtop * (gt - gprev[sl]);
             }
-            if (EDGE) d = inner[sl] ? d : b0[sl];
-            const float4 Dk = D0[sl];
-            // shift the diffz window
+            if (EDGE) d = inner(sl) ? d : b0[sl];
             q0[sl] = qn;
             b0[sl] = bnew;
-            D0[sl] = Dn[sl];
             // first horizontal diffusion at level k
             if (!ODD) {
-                const float pk = EX ? p.cE[(unsigned)(k - 1) * NXY + moff[sl]].x : 0.f;
-                d = diff_x<EX>(d, Dk.z, Dk.x, pk, p.dtdx2, inner[sl], xlo[sl], xhi[sl]);
+                const float pk = EX ? bld2(rf, m4[sl] * 2u, p.oE + 2u * lk).x : 0.f;
+                d = diff_x<EX>(d, lr0, Dk.x, pk, p.dtdx2, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
             } else {
-                lrc[sl] = Dk.z;
+                lrc[sl] = lr0;
                 dxc[sl] = Dk.x;
-                if (EX) pxc[sl] = p.cE[(unsigned)(k - 1) * NXY + moff[sl]].x;
+                if (EX) pxc[sl] = bld2(rf, m4[sl] * 2u, p.oE + 2u * lk).x;
             }
             dc[sl] = d;
-            qc[sl] = d * Dk.z;
+            qc[sl] = d * lr0;
             dyl[sl] = Dk.y;
-            if (sl == NS - 1) dyu[sl] = cDf[4 * ((unsigned)(k - 1) * NXY + mup[sl]) + 1];
+            if (sl == NS - 1) dyu[sl] = bld(rf, mup16[sl] + 4u, p.oD + 4u * lk);
             if (EY) {
-                pyo[sl] = p.cE[(unsigned)(k - 1) * NXY + moff[sl]].y;
-                pym[sl] = p.cE[(unsigned)(k - 1) * NXY + mdn[sl]].y;
-                pyp[sl] = p.cE[(unsigned)(k - 1) * NXY + mup[sl]].y;
+                pyo[sl] = bld(rf, m4[sl] * 2u + 4u, p.oE + 2u * lk);
+                pym[sl] = bld(rf, mdn8[sl] + 4u, p.oE + 2u * lk);
+                pyp[sl] = bld(rf, mup8[sl] + 4u, p.oE + 2u * lk);
             }
         }
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl)
-            if ((sl == 0 || sl == NS - 1) && lj[sl] >= 2 && lj[sl] < RY - 2 && s >= 2)
+            if ((sl == 0 || sl == NS - 1) && lj[sl] >= 2 && lj[sl] < RY - 2 && (STEADY || s >= 2))
                 SQ[lj[sl] * RX + li] = qc[sl];
     }
+
+    // after a level: its next-level coefficients become current, its
+    // diffusion coefficients become the lagging level's
+    __device__ __forceinline__ void rotate()
+    {
+#pragma unroll
+        for (int sl = 0; sl < NS; ++sl) {
+            A[0][sl] = A[1][sl];
+            D[1][sl] = D[0][sl];
+        }
+    }
+
+    // Interior tiles: prologue, steady and epilogue levels run as three loops
+    // so each keeps its own register allocation.
+    __device__ __forceinline__ void run()
+    {
+        init();
+        const int nzl = nz;
+        int s = 1;
+        if (EDGE) {
+            // edge tiles: one generic loop keeps their code and registers small
+            for (; s <= nzl + 2; ++s) {
+                level<false, 0>(s);
+                rotate();
+            }
+            return;
+        }
+        for (; s <= 3 && s <= nzl + 2; ++s) {
+            level<false, 0>(s);
+            rotate();
+        }
+        for (; s <= nzl - 3; ++s) {
+            level<true, 0>(s);
+            rotate();
+        }
+        for (; s <= nzl + 2; ++s) {
+            level<false, 0>(s);
+            rotate();
+        }
+    }
+};
+
+template <int RY, int NW, bool ODD, bool EX, bool EY>
+__device__ __forceinline__ void flux_tile(const FluxParams &p, const int q, const int ic,
+                                          const int mem, const int gi0, const int gj0, float *sm)
+{
+    FluxTile<RY, NW, ODD, EX, EY> t(p, q, ic, mem, gi0, gj0, sm);
+    t.run();
 }
 
-// Interior tiles (no frame in the region) and edge tiles run as two kernels
-// so the interior one keeps the register budget of the plain path.
-template <int RY, int NW, bool ODD, bool EDGEK>
+template <int RY, int NW, bool ODD>
 __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
 {
     const int b = blockIdx.x;
@@ -403,95 +522,50 @@ __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
     const int nq = nc * p.g.nm;
     const int q = L % nq;
     const int ic = q % nc, mem = q / nc;
+    // consecutive tiles run down y so the 6-row halos are shared in L2
     const int t = L / nq;
-    int bx, by;
-    if (!EDGEK) {
-        // consecutive tiles run down y so the 6-row halos are shared in L2
-        const int nyi = p.by1 - p.by0 + 1;
-        by = p.by0 + t % nyi;
-        bx = p.bx0 + t / nyi;
-    } else {
-        // edge tiles: full rows below by0, the side columns of rows
-        // by0..by1, full rows above by1
-        const int nside = p.bx0 + (p.ntx - 1 - p.bx1);
-        const int nmid = (p.by1 >= p.by0) ? (p.by1 - p.by0 + 1) : 0;
-        int e = t;
-        if (e < p.by0 * p.ntx) {
-            by = e / p.ntx;
-            bx = e % p.ntx;
-        } else if ((e -= p.by0 * p.ntx) < nmid * nside) {
-            by = p.by0 + e / nside;
-            const int c = e % nside;
-            bx = (c < p.bx0) ? c : p.bx1 + 1 + (c - p.bx0);
-        } else {
-            e -= nmid * nside;
-            by = p.by0 + nmid + e / p.ntx;
-            bx = e % p.ntx;
-        }
-    }
+    const int by = t % p.nty, bx = t / p.nty;
     const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * (RY - 6) - 3;
     __shared__ float sm[3 * RX * RY];
-    if (!EDGEK) {
+    const bool ex = gi0 < 1 || gi0 + RX - 1 > p.g.nx;
+    const bool ey = gj0 < 1 || gj0 + RY - 1 > p.g.ny;
+    if (!ex && !ey)
         flux_tile<RY, NW, ODD, false, false>(p, q, ic, mem, gi0, gj0, sm);
-    } else {
-        const bool ex = gi0 < 1 || gi0 + RX - 1 > p.g.nx;
-        const bool ey = gj0 < 1 || gj0 + RY - 1 > p.g.ny;
-        if (ex && ey)
-            flux_tile<RY, NW, ODD, true, true>(p, q, ic, mem, gi0, gj0, sm);
-        else if (ex)
-            flux_tile<RY, NW, ODD, true, false>(p, q, ic, mem, gi0, gj0, sm);
-        else
-            flux_tile<RY, NW, ODD, false, true>(p, q, ic, mem, gi0, gj0, sm);
-    }
-}
-
-template <int RY, int NW, bool EDGEK>
-void launch_one(hipStream_t s, FluxParams p, int ntiles, bool odd)
-{
-    if (ntiles <= 0) return;
-    p.nlogical = ntiles * p.g.nc * p.g.nm;
-    p.per_xcd = (p.nlogical + kXCD - 1) / kXCD;
-    const unsigned nb = (unsigned)(p.per_xcd * kXCD);
-    if (odd)
-        hipLaunchKernelGGL((k_flux<RY, NW, true, EDGEK>), dim3(nb), dim3(NW * 64), 0, s, p);
+    else if (ex && ey)
+        flux_tile<RY, NW, ODD, true, true>(p, q, ic, mem, gi0, gj0, sm);
+    else if (ex)
+        flux_tile<RY, NW, ODD, true, false>(p, q, ic, mem, gi0, gj0, sm);
     else
-        hipLaunchKernelGGL((k_flux<RY, NW, false, EDGEK>), dim3(nb), dim3(NW * 64), 0, s, p);
+        flux_tile<RY, NW, ODD, false, true>(p, q, ic, mem, gi0, gj0, sm);
 }
 
 template <int RY, int NW>
 int launch_flux(hipStream_t s, const FluxParams &p0, bool odd)
 {
     FluxParams p = p0;
-    const int nx = p.g.nx, ny = p.g.ny, ty = RY - 6;
-    p.ntx = (nx + TXR - 1) / TXR;
-    p.nty = (ny + ty - 1) / ty;
-    // tile bx is interior when 1 <= gi0 and gi0 + 63 <= nx (gi0 = bx*58 - 2),
-    // likewise in y with RY rows
-    p.bx0 = 1;
-    p.bx1 = min(p.ntx - 1, (nx - (RX - 3)) >= 0 ? (nx - (RX - 3)) / TXR : -1);
-    p.by0 = 1;
-    p.by1 = min(p.nty - 1, (ny - (RY - 3)) >= 0 ? (ny - (RY - 3)) / ty : -1);
-    if (p.bx1 < p.bx0 || p.by1 < p.by0) {
-        // no interior tile: everything is edge (rows 0 .. nty-1 below "by0")
-        p.bx0 = 0;
-        p.bx1 = p.ntx - 1;
-        p.by0 = p.nty;
-        p.by1 = p.nty - 1;
-    }
-    const int nint = (p.bx1 - p.bx0 + 1) * (p.by1 >= p.by0 ? p.by1 - p.by0 + 1 : 0);
-    const int nedge = p.ntx * p.nty - nint;
-    launch_one<RY, NW, false>(s, p, nint, odd);
-    F3D_CHECK_LAUNCH();
-    launch_one<RY, NW, true>(s, p, nedge, odd);
+    p.ntx = (p.g.nx + TXR - 1) / TXR;
+    p.nty = (p.g.ny + (RY - 6) - 1) / (RY - 6);
+    p.nlogical = p.ntx * p.nty * p.g.nc * p.g.nm;
+    p.per_xcd = (p.nlogical + kXCD - 1) / kXCD;
+    const unsigned nb = (unsigned)(p.per_xcd * kXCD);
+    if (odd)
+        hipLaunchKernelGGL((k_flux<RY, NW, true>), dim3(nb), dim3(NW * 64), 0, s, p);
+    else
+        hipLaunchKernelGGL((k_flux<RY, NW, false>), dim3(nb), dim3(NW * 64), 0, s, p);
     F3D_CHECK_LAUNCH();
     return F3D_OK;
 }
 
 }  // namespace
 
+// (ty, waves) pairs of the flux kernel: region rows RY = ty + 6 split in
+// contiguous bands of RY / waves rows
 bool flux_supported(TileShape ts)
 {
-    return ts.tx == TXR && (ts.ty == 10 || ts.ty == 18 || ts.ty == 26);
+    if (ts.tx != TXR) return false;
+    const int nw = ts.nw ? ts.nw : 8;
+    return (ts.ty == 10 && (nw == 8 || nw == 4)) || (ts.ty == 18 && (nw == 8 || nw == 6)) ||
+           (ts.ty == 26 && (nw == 8 || nw == 16));
 }
 
 int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
@@ -501,12 +575,24 @@ int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
     p.cin = a.cin;
     p.cout = a.cout;
     p.cdry = a.cdry;
-    p.cA = f.cA;
-    p.cD = f.cD;
-    p.cE = f.cE;
-    p.wz = f.wz;
-    p.bfac = f.bfac;
-    p.tfac = f.tfac;
+    const char *fb = reinterpret_cast<const char *>(f.base);
+    auto off = [&](const void *q) {
+        return (unsigned)(reinterpret_cast<const char *>(q) - fb);
+    };
+    // every byte offset the kernel forms (field buffer, class slab of c) is
+    // 32-bit
+    if (f.bytes >= (size_t)1 << 32 || (size_t)a.g.G * 4 >= (size_t)1 << 32) {
+        set_error("flux kernel: fields (%zu B) or a class slab exceed 4 GiB", f.bytes);
+        return F3D_EINVAL;
+    }
+    p.fbase = f.base;
+    p.fbytes = (unsigned)f.bytes;
+    p.oA = off(f.cA);
+    p.oD = off(f.cD);
+    p.oE = off(f.cE);
+    p.oW = off(f.wz);
+    p.oBF = off(f.bfac);
+    p.oTF = off(f.tfac);
     p.vsum1 = f.vsum1;
     p.zl = f.zl;
     p.dt64 = a.dt;
@@ -514,10 +600,14 @@ int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
     p.dtdx2 = (float)(a.dt / (a.g.dx * a.g.dx));
     p.dtdy2 = (float)(a.dt / (a.g.dy * a.g.dy));
     const bool odd = (a.iiter % 2) != 0;
-    if (ts.ty == 10) return launch_flux<16, 8>(s, p, odd);
-    if (ts.ty == 18) return launch_flux<24, 8>(s, p, odd);
-    if (ts.ty == 26) return launch_flux<32, 8>(s, p, odd);
-    set_error("flux kernel: unsupported tile %dx%d", ts.tx, ts.ty);
+    const int nw = ts.nw ? ts.nw : 8;
+    if (ts.ty == 10 && nw == 8) return launch_flux<16, 8>(s, p, odd);
+    if (ts.ty == 10 && nw == 4) return launch_flux<16, 4>(s, p, odd);
+    if (ts.ty == 18 && nw == 8) return launch_flux<24, 8>(s, p, odd);
+    if (ts.ty == 18 && nw == 6) return launch_flux<24, 6>(s, p, odd);
+    if (ts.ty == 26 && nw == 8) return launch_flux<32, 8>(s, p, odd);
+    if (ts.ty == 26 && nw == 16) return launch_flux<32, 16>(s, p, odd);
+    set_error("flux kernel: unsupported tile %dx%d with %d waves", ts.tx, ts.ty, nw);
     return F3D_EINVAL;
 }
 

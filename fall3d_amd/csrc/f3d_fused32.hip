@@ -755,6 +755,8 @@ Fields fields_view(const Grid &g, const float *b0, const double *vsum1)
     f.cD = reinterpret_cast<const float4 *>(b0 + 4 * g.N);
     f.cE = reinterpret_cast<const float2 *>(b0 + 8 * g.N);
     f.zl = reinterpret_cast<const ZLev *>(b0 + 10 * g.N);
+    f.base = b0;
+    f.bytes = sizeof(float) * fields_floats(g);
     const float *b = b0 + separate_offset(g);
     f.vx = b;
     f.vy = b + g.N;
@@ -804,6 +806,16 @@ int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *v
     hipLaunchKernelGGL(k_zlev, dim3(1), dim3(64), 0, s, g.nz, zc, dt, const_cast<ZLev *>(f.zl));
     F3D_CHECK_LAUNCH();
     return F3D_OK;
+}
+
+bool tile_supported(TileShape ts)
+{
+    if (flux_supported(ts)) return true;
+    if (ts.nw != 0) return false;
+    const int ok[][2] = {{32, 8}, {32, 16}, {64, 8}, {64, 4}, {16, 16}};
+    for (auto &t : ok)
+        if (t[0] == ts.tx && t[1] == ts.ty) return true;
+    return false;
 }
 
 int fused_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)

@@ -161,8 +161,12 @@ int f3d_sync(f3d_ctx *ctx);
 /* Tile shape (columns x rows) of the fused FP32 step kernel.  No
  * counterpart in the reference: a tuning knob set by the autotuner.
  * Supported: 32x8, 32x16, 64x8, 64x4, 16x16 (column-tile kernel) and
- * 58x10 (default), 58x18, 58x26 (row-per-wavefront kernel). */
+ * 58x10 (default), 58x18, 58x26 (flux kernel, 64-lane rows). */
 int f3d_set_tile(f3d_ctx *ctx, int tx, int ty);
+/* The same with the number of wavefronts per workgroup of the 58-column
+ * kernel (0: its default of 8): 58x10 with 8 or 4, 58x18 with 8 or 6,
+ * 58x26 with 8 or 16.  No counterpart in the reference. */
+int f3d_set_tile_waves(f3d_ctx *ctx, int tx, int ty, int waves);
 /* HIP-event timing of the step kernels (enabled by f3d_timing(ctx,1)):
  * average duration in ms and launch count of the dominant step kernel
  * since the last reset. */

@@ -156,7 +156,12 @@ def test_fp32_context_vs_golden(path):
         r.close()
 
 
-TILES = [(32, 8), (16, 16), (64, 4), (58, 10), (58, 18), (58, 26)]
+TILES = [(32, 8), (16, 16), (64, 4), (58, 10), (58, 18), (58, 26), (58, 10, 4), (58, 18, 6),
+         (58, 26, 16)]
+
+
+def _tile_id(t):
+    return f"{t[0]}x{t[1]}" + (f"w{t[2]}" if len(t) > 2 else "")
 
 
 def _compare_fp32_with_oracle(sc, nsteps, tile=None):
@@ -189,7 +194,7 @@ def test_fp32_vs_oracle_moderate_grid(parity_steps):
     assert m["rho_m"] <= 1e-5, m
 
 
-@pytest.mark.parametrize("tile", TILES, ids=lambda t: f"{t[0]}x{t[1]}")
+@pytest.mark.parametrize("tile", TILES, ids=_tile_id)
 def test_fp32_every_tile_shape(tile):
     # a grid larger than every tile in both directions (interior and edge
     # tiles), both sweep orders (11 steps), winds of both signs
@@ -198,7 +203,7 @@ def test_fp32_every_tile_shape(tile):
     assert m["c_l1"] <= 1e-4 and m["load_l1"] <= 1e-4, m
 
 
-@pytest.mark.parametrize("tile", TILES, ids=lambda t: f"{t[0]}x{t[1]}")
+@pytest.mark.parametrize("tile", TILES, ids=_tile_id)
 def test_fp32_tile_edges_partial_tiles(tile):
     # grid sizes that are not multiples of any tile edge, nz at the small end
     sc = S.config_small(nx=37, ny=11, nz=2, nc=2, seed=5, mean_u=-4.0, mean_v=6.0)
