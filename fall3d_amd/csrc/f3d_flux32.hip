@@ -35,6 +35,9 @@ namespace {
 constexpr int kXCD = 8;
 constexpr int RX = 64;       // region row = one wavefront
 constexpr int TXR = RX - 6;  // output columns per tile
+constexpr int kMaxLevels = 64;
+// s_waitcnt immediate of gfx9: vmcnt(0), expcnt and lgkmcnt left open
+constexpr int kWaitVm0 = 0x0F70;
 
 struct FluxParams {
     Grid g;
@@ -164,6 +167,8 @@ struct FluxTile {
     // per-class fields in it
     rsrc_t rin, rout, rf;
     unsigned oW, oBF, oTF;
+    const ZLev *zt;  // the z-level table, staged in LDS
+    float dtop, ldz0;
 
     // per-row geometry: byte offsets of the (clamped) column in c, in the
     // meteo arrays and of the rows above/below; edge flags
@@ -172,9 +177,14 @@ struct FluxTile {
     // advctzc window: c(s), c(s+1), w(s), U(s), W2(s), delta(s); after level nz
     // c0 holds the top ghost c(nz+1).  Prefetched c(s+2), w(s+1).
     float c0[NS], cp1[NS], vk[NS], Uc[NS], W2k[NS], dprev[NS], nc2[NS], nwz[NS];
-    // (ax, fx, ay, fy) for two consecutive levels and (dfx, dfy, lr, pz) for
-    // two; the level parity picks the current one
-    float4 A[2][NS], D[2][NS];
+    // Coefficients of the current level: (ax, fx), (ay, fy), (dfx, dfy, lr,
+    // pz) and the latter of the level below.  Each register is reloaded with
+    // the next level's value right after its last use in this level, so a
+    // load is never copied while in flight (a copy would wait for it).
+    // gv: wind at the face cell of a ghost (edge tiles), one level ahead.
+    float2 AX[NS], AY[NS];
+    float4 Dn[NS], D0[NS];
+    float gv[NS];
     // diffz window: b(k), q(k) = b/rho and the face flux it leaves for the
     // level above (before level 1: b(0))
     float b0[NS], q0[NS], gprev[NS];
@@ -183,9 +193,10 @@ struct FluxTile {
     float P[NS], phi[NS];
 
     __device__ __forceinline__ FluxTile(const FluxParams &pp, int q, int ic_, int mem_, int gi0,
-                                        int gj0, float *sm)
+                                        int gj0, float *sm, const ZLev *zt_)
         : p(pp)
     {
+        zt = zt_;
         const int nx = p.g.nx, ny = p.g.ny;
         nz = p.g.nz;
         ic = ic_;
@@ -241,8 +252,9 @@ struct FluxTile {
 
     __device__ __forceinline__ void init()
     {
-        const ZLev z1 = p.zl[1];
-        const float ldz0 = p.zl[0].ldz;
+        const ZLev z1 = zt[1];
+        ldz0 = zt[0].ldz;
+        dtop = p.dt * zt[nz].ldz;
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
             const float c1 = bld(rin, co4[sl], 4 * SZ);
@@ -261,7 +273,11 @@ struct FluxTile {
             gprev[sl] = gb;
             nc2[sl] = (3 <= nz) ? bld(rin, co4[sl], 12 * SZ) : 0.f;
             nwz[sl] = bld(rf, m4[sl], oW + 4 * NXY);
-            A[0][sl] = bld4(rf, m16[sl], p.oA);  // level 1
+            AX[sl] = bld2(rf, m16[sl], p.oA);  // level 1
+            AY[sl] = bld2(rf, m16[sl] + 8u, p.oA);
+            Dn[sl] = bld4(rf, m16[sl], p.oD);
+            D0[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
+            gv[sl] = EDGE ? bld(rf, m16[sl] + (has(sl, F_GX) ? 0u : 8u), p.oA) : 0.f;
             b0[sl] = q0[sl] = 0.f;
             dc[sl] = qc[sl] = dyl[sl] = dyu[sl] = pyo[sl] = pym[sl] = pyp[sl] = 0.f;
             lrc[sl] = dxc[sl] = pxc[sl] = 0.f;
@@ -271,22 +287,17 @@ struct FluxTile {
 
     // One level of the pipeline.  STEADY promises 4 <= s <= nz-3 (no level
     // is at a boundary of any stage).
-    template <bool STEADY, int PAR>
+    template <bool STEADY>
     __device__ __forceinline__ void level(const int s)
     {
         const int nzl = nz;
-        float4 *Ac = A[PAR], *An = A[1 - PAR];  // levels s, s+1
-        float4 *Dn = D[PAR], *D0 = D[1 - PAR];   // levels s, s-1
-        const float dt = p.dt;
-        const float dtop = dt * p.zl[nzl].ldz;
 
         // ======== phase 1: advctzc at level s (and advctx on even steps) ========
         if (STEADY || s <= nzl) {
-            const ZLev zn = p.zl[s + 1];
+            const ZLev zn = zt[s + 1];
             const unsigned lz = 4u * NXY * (unsigned)s;  // byte offset of level s+1 in a 3-D field
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
-                Dn[sl] = bld4(rf, m16[sl], p.oD + 4u * (lz - 4u * NXY));  // level s, used two barriers later
                 float cp2 = 0.f, vkn = 0.f, W1n, W2n, dn = 0.f;
                 if (STEADY || s < nzl) {
                     // top ghost c(nz+1) = tfac*c(nz) (setbcc.f90:126-130)
@@ -307,8 +318,10 @@ struct FluxTile {
                 if (EDGE) {
                     // ghost faces: zero when the wind at the face cell blows
                     // out of the domain through it, else the adjacent cell
-                    const float v = has(sl, F_GX) ? Ac[sl].x : Ac[sl].z;
+                    const float v = gv[sl];
                     const bool zero = has(sl, F_GLO) ? v >= 0.f : v <= 0.f;
+                    if (STEADY || s + 1 <= nzl)
+                        gv[sl] = bld(rf, m16[sl] + (has(sl, F_GX) ? 0u : 8u), p.oA + 4u * lz);
                     a = inner(sl) ? a : (has(sl, F_GHOST) && !zero ? c0[sl] : 0.f);
                 }
                 c0[sl] = cp1[sl];
@@ -319,9 +332,10 @@ struct FluxTile {
                 dprev[sl] = dn;
                 if (STEADY || s + 3 <= nzl) nc2[sl] = bld(rin, co4[sl], 4u * SZ * (unsigned)(s + 3));
                 if (STEADY || s + 2 <= nzl) nwz[sl] = bld(rf, m4[sl], oW + lz + 4u * NXY);
-                if (STEADY || s + 1 <= nzl) An[sl] = bld4(rf, m16[sl], p.oA + 4u * lz);
-                if (!ODD)
-                    a = adv_x<EX>(a, Ac[sl].x, Ac[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
+                if (!ODD) {
+                    a = adv_x<EX>(a, AX[sl].x, AX[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
+                    if (STEADY || s + 1 <= nzl) AX[sl] = bld2(rf, m16[sl], p.oA + 4u * lz);
+                }
                 P[sl] = a;
             }
 #pragma unroll
@@ -337,8 +351,8 @@ struct FluxTile {
                 if (lj[sl] < 1 || lj[sl] >= RY - 1) continue;  // wave-uniform
                 const float Pm = (sl > 0) ? P[sl - 1] : SP[(lj[sl] - 1) * RX + li];
                 const float Pp = (sl < NS - 1) ? P[sl + 1] : SP[(lj[sl] + 1) * RX + li];
-                const float w2 = Ac[sl].w * minmod(Pp - P[sl], P[sl] - Pm);
-                phi[sl] = fmaf(Ac[sl].z, P[sl], w2);
+                const float w2 = AY[sl].y * minmod(Pp - P[sl], P[sl] - Pm);
+                phi[sl] = fmaf(AY[sl].x, P[sl], w2);
                 if (sl == 0 || sl == NS - 1) SF[lj[sl] * RX + li] = phi[sl];
             }
         }
@@ -383,7 +397,7 @@ struct FluxTile {
         // the first horizontal diffusion at level s-1 ========
         if (!STEADY && s >= nzl + 2) return;
         const int k = s - 1;
-        const ZLev zk = p.zl[k > 0 ? k : 0];
+        const ZLev zk = zt[k > 0 ? k : 0];
         const unsigned lk = 4u * NXY * (unsigned)(k > 0 ? k - 1 : 0);  // level k in a 3-D field
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
@@ -394,7 +408,7 @@ struct FluxTile {
                 const float Pp = (sl < NS - 1) ? P[sl + 1] : SP[(lj[sl] + 1) * RX + li];
                 float pl = (sl > 0) ? phi[sl - 1] : SF[(lj[sl] - 1) * RX + li];
                 float pr = (sl < NS - 1) ? phi[sl + 1] : SF[(lj[sl] + 1) * RX + li];
-                const float ay = Ac[sl].z;
+                const float ay = AY[sl].x;
                 if (EY) {
                     // advcty.f90:49-51: W2 clamps to the line at j = 1 / ny,
                     // A at the ghost row is v(1) / v(ny)
@@ -405,7 +419,7 @@ struct FluxTile {
                 bnew = (ay > 0.f) ? P[sl] + (pl - phi[sl]) : P[sl] + (phi[sl] - pr);
                 if (EDGE) bnew = inner(sl) ? bnew : P[sl];
                 if (ODD)
-                    bnew = adv_x<EX>(bnew, Ac[sl].x, Ac[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
+                    bnew = adv_x<EX>(bnew, AX[sl].x, AX[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
             } else {
                 bnew = c0[sl];  // b(nz+1): the top ghost of c_in
             }
@@ -414,12 +428,16 @@ struct FluxTile {
             if (!STEADY && s == 1) {
                 // bottom face: c(0)/rho(1), un-averaged rho*K(1), inflow only
                 const float q1 = bnew * lrn;
-                gprev[sl] = fmaxf(0.f, pzn * (q1 - gprev[sl] * lrn) * p.zl[0].ldz);
+                gprev[sl] = fmaxf(0.f, pzn * (q1 - gprev[sl] * lrn) * ldz0);
                 b0[sl] = bnew;
                 q0[sl] = q1;
+                D0[sl] = Dn[sl];
+                Dn[sl] = bld4(rf, m16[sl], p.oD + 16u * NXY);  // level 2 (nz >= 2)
                 continue;
             }
             const float4 Dk = D0[sl];
+            D0[sl] = Dn[sl];
+            if (STEADY || s + 1 <= nzl) Dn[sl] = bld4(rf, m16[sl], p.oD + 16u * NXY * (unsigned)s);
             const float lr0 = Dk.z, pz0 = Dk.w;
             float d, qn = 0.f;
             if (STEADY || k < nzl) {
@@ -457,58 +475,47 @@ struct FluxTile {
                 pyp[sl] = bld(rf, mup8[sl] + 4u, p.oE + 2u * lk);
             }
         }
+        // (ay, fy) of every row, and (ax, fx) on odd steps, for level s+1
+        if (STEADY || s + 1 <= nzl) {
+            const unsigned la = p.oA + 16u * NXY * (unsigned)s;
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl) {
+                AY[sl] = bld2(rf, m16[sl] + 8u, la);
+                if (ODD) AX[sl] = bld2(rf, m16[sl], la);
+            }
+        }
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl)
             if ((sl == 0 || sl == NS - 1) && lj[sl] >= 2 && lj[sl] < RY - 2 && (STEADY || s >= 2))
                 SQ[lj[sl] * RX + li] = qc[sl];
     }
 
-    // after a level: its next-level coefficients become current, its
-    // diffusion coefficients become the lagging level's
-    __device__ __forceinline__ void rotate()
-    {
-#pragma unroll
-        for (int sl = 0; sl < NS; ++sl) {
-            A[0][sl] = A[1][sl];
-            D[1][sl] = D[0][sl];
-        }
-    }
-
     // Interior tiles: prologue, steady and epilogue levels run as three loops
-    // so each keeps its own register allocation.
+    // so each keeps its own register allocation; edge tiles run one generic
+    // loop.
     __device__ __forceinline__ void run()
     {
         init();
         const int nzl = nz;
         int s = 1;
-        if (EDGE) {
-            // edge tiles: one generic loop keeps their code and registers small
-            for (; s <= nzl + 2; ++s) {
-                level<false, 0>(s);
-                rotate();
-            }
-            return;
+        if (!EDGE) {
+            for (; s <= 3; ++s) level<false>(s);
+            // drain the loads of the prologue: the steady loop then enters
+            // with the same pending loads as its own back edge, and its
+            // waits count only the loads issued one level back
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            for (; s <= nzl - 3; ++s) level<true>(s);
         }
-        for (; s <= 3 && s <= nzl + 2; ++s) {
-            level<false, 0>(s);
-            rotate();
-        }
-        for (; s <= nzl - 3; ++s) {
-            level<true, 0>(s);
-            rotate();
-        }
-        for (; s <= nzl + 2; ++s) {
-            level<false, 0>(s);
-            rotate();
-        }
+        for (; s <= nzl + 2; ++s) level<false>(s);
     }
 };
 
 template <int RY, int NW, bool ODD, bool EX, bool EY>
 __device__ __forceinline__ void flux_tile(const FluxParams &p, const int q, const int ic,
-                                          const int mem, const int gi0, const int gj0, float *sm)
+                                          const int mem, const int gi0, const int gj0, float *sm,
+                                          const ZLev *zt)
 {
-    FluxTile<RY, NW, ODD, EX, EY> t(p, q, ic, mem, gi0, gj0, sm);
+    FluxTile<RY, NW, ODD, EX, EY> t(p, q, ic, mem, gi0, gj0, sm, zt);
     t.run();
 }
 
@@ -527,16 +534,19 @@ __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
     const int by = t % p.nty, bx = t / p.nty;
     const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * (RY - 6) - 3;
     __shared__ float sm[3 * RX * RY];
+    __shared__ ZLev zt[kMaxLevels + 2];
+    for (int k = threadIdx.x; k <= p.g.nz + 1; k += NW * 64) zt[k] = p.zl[k];
+    __syncthreads();
     const bool ex = gi0 < 1 || gi0 + RX - 1 > p.g.nx;
     const bool ey = gj0 < 1 || gj0 + RY - 1 > p.g.ny;
     if (!ex && !ey)
-        flux_tile<RY, NW, ODD, false, false>(p, q, ic, mem, gi0, gj0, sm);
+        flux_tile<RY, NW, ODD, false, false>(p, q, ic, mem, gi0, gj0, sm, zt);
     else if (ex && ey)
-        flux_tile<RY, NW, ODD, true, true>(p, q, ic, mem, gi0, gj0, sm);
+        flux_tile<RY, NW, ODD, true, true>(p, q, ic, mem, gi0, gj0, sm, zt);
     else if (ex)
-        flux_tile<RY, NW, ODD, true, false>(p, q, ic, mem, gi0, gj0, sm);
+        flux_tile<RY, NW, ODD, true, false>(p, q, ic, mem, gi0, gj0, sm, zt);
     else
-        flux_tile<RY, NW, ODD, false, true>(p, q, ic, mem, gi0, gj0, sm);
+        flux_tile<RY, NW, ODD, false, true>(p, q, ic, mem, gi0, gj0, sm, zt);
 }
 
 template <int RY, int NW>
