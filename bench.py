@@ -52,8 +52,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--tile", default="58x10")
-    ap.add_argument("--waves", type=int, default=0,
-                    help="waves per workgroup of the 58-column kernel (0: its default)")
+    ap.add_argument("--slabs", type=int, default=0,
+                    help="member*class slabs per workgroup of the 58-column kernel (0: one)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -74,7 +74,7 @@ def main():
     # ---------------- FP32 fused path ----------------
     r = F.Run(sc, precision=F.FP32)
     tx, ty = (int(v) for v in args.tile.split("x"))
-    r.ctx.set_tile(tx, ty, args.waves)
+    r.ctx.set_tile(tx, ty, 0, args.slabs)
     t0 = time.time()
     r.meteo_update()
     r.ctx.sync()
@@ -169,7 +169,7 @@ def main():
         "data": "synthetic (seeded Fourier-mode meteorology, Suzuki source)",
         "config": {"workload": f"precision form {sc.nx}x{sc.ny}x{sc.nz} grid, {sc.nc} classes, "
                                f"fused FP32 step", "nx": sc.nx, "ny": sc.ny, "nz": sc.nz,
-                   "nc": sc.nc, "tile": args.tile, "waves": args.waves, "dt_s": None},
+                   "nc": sc.nc, "tile": args.tile, "slabs": args.slabs, "dt_s": None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.tile),
                      "bytes_per_update": b_alg, "kernel_avg_ms": kms, "launches": kcount},

@@ -697,14 +697,14 @@ int f3d_sync(f3d_ctx *c)
     return F3D_OK;
 }
 
-int f3d_set_tile(f3d_ctx *c, int tx, int ty) { return f3d_set_tile_waves(c, tx, ty, 0); }
+int f3d_set_tile(f3d_ctx *c, int tx, int ty) { return f3d_set_tile_config(c, tx, ty, 0, 0); }
 
-int f3d_set_tile_waves(f3d_ctx *c, int tx, int ty, int waves)
+int f3d_set_tile_config(f3d_ctx *c, int tx, int ty, int waves, int slabs)
 {
     if (!check_ctx(c)) return F3D_EINVAL;
-    const f32::TileShape ts{tx, ty, waves};
+    const f32::TileShape ts{tx, ty, waves, slabs};
     if (!f32::tile_supported(ts)) {
-        set_error("f3d_set_tile: unsupported tile %dx%d (waves %d)", tx, ty, waves);
+        set_error("f3d_set_tile: unsupported tile %dx%d (waves %d, slabs %d)", tx, ty, waves, slabs);
         return F3D_EINVAL;
     }
     c->tile = ts;

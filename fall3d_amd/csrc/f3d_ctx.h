@@ -91,6 +91,7 @@ struct StepArgs {
 struct TileShape {
     int tx, ty;
     int nw = 0;  // waves per workgroup of the flux kernel (0: its default)
+    int mc = 0;  // member*class slabs per workgroup of the flux kernel (0: 1)
 };
 bool tile_supported(TileShape ts);
 

@@ -56,6 +56,7 @@ struct FluxParams {
     double dt64;
     float dt, dtdx2, dtdy2;
     int ntx, nty;
+    int bx0, bx1, by0, by1;  // the rectangle of interior tiles
     int nlogical;
     int per_xcd;
 };
@@ -153,68 +154,82 @@ enum : unsigned {
     F_GLO = 128u,   // ... on the low face (i = 0 or j = 0)
 };
 
-template <int RY, int NW, bool ODD, bool EX, bool EY>
+template <int RY, int NW, int MC, bool ODD, bool EX, bool EY>
 struct FluxTile {
     static constexpr int NS = RY / NW;
     static constexpr bool EDGE = EX || EY;
+    static constexpr int PL = RX * RY;  // one LDS plane
     static_assert(RY % NW == 0, "region rows must divide evenly over the waves");
 
     const FluxParams &p;
-    int nz, li, w, ic, mem;
+    int nz, li, w;
     unsigned NXY, SZ;
-    float *SP, *SF, *SQ;
-    // class slabs of c_in and c_out, the field buffer; class offsets of the
-    // per-class fields in it
-    rsrc_t rin, rout, rf;
-    unsigned oW, oBF, oTF;
-    const ZLev *zt;  // the z-level table, staged in LDS
+    float *SP, *SF, *SQ;  // MC planes each
+    rsrc_t rf;            // the field buffer
+    const ZLev *zt;       // the z-level table, staged in LDS
     float dtop, ldz0;
+    // per class: slabs of c_in / c_out, offsets of its per-class fields,
+    // class and member index; a pair's second class may be a duplicate of
+    // the first (odd class count), computed but not stored
+    rsrc_t rin[MC], rout[MC];
+    unsigned oW[MC], oBF[MC], oTF[MC];
+    int icv[MC], memv[MC];
+    bool live[MC];
 
     // per-row geometry: byte offsets of the (clamped) column in c, in the
     // meteo arrays and of the rows above/below; edge flags
     unsigned co4[NS], m4[NS], m16[NS], mup16[NS], mdn8[NS], mup8[NS], fl[NS];
     int lj[NS];
-    // advctzc window: c(s), c(s+1), w(s), U(s), W2(s), delta(s); after level nz
-    // c0 holds the top ghost c(nz+1).  Prefetched c(s+2), w(s+1).
-    float c0[NS], cp1[NS], vk[NS], Uc[NS], W2k[NS], dprev[NS], nc2[NS], nwz[NS];
-    // Coefficients of the current level: (ax, fx), (ay, fy), (dfx, dfy, lr,
-    // pz) and the latter of the level below.  Each register is reloaded with
-    // the next level's value right after its last use in this level, so a
-    // load is never copied while in flight (a copy would wait for it).
-    // gv: wind at the face cell of a ghost (edge tiles), one level ahead.
+    // Class-independent coefficients of the current level: (ax, fx), (ay,
+    // fy), (dfx, dfy, lr, pz) and the latter of the level below.  Each
+    // register is reloaded with the next level's value right after its last
+    // use in this level, so a load is never copied while in flight (a copy
+    // would wait for it).  gv: wind at the face cell of a ghost (edge tiles),
+    // one level ahead.
     float2 AX[NS], AY[NS];
     float4 Dn[NS], D0[NS];
     float gv[NS];
-    // diffz window: b(k), q(k) = b/rho and the face flux it leaves for the
-    // level above (before level 1: b(0))
-    float b0[NS], q0[NS], gprev[NS];
-    // carried between phase 3 and the next phase 2
-    float dc[NS], qc[NS], dyl[NS], dyu[NS], pyo[NS], pym[NS], pyp[NS], lrc[NS], dxc[NS], pxc[NS];
-    float P[NS], phi[NS];
+    // carried between phase 3 and the next phase 2 (class-independent)
+    float dyl[NS], dyu[NS], pyo[NS], pym[NS], pyp[NS], lrc[NS], dxc[NS], pxc[NS];
+    // per class: advctzc window c(s), c(s+1), w(s), U(s), W2(s), delta(s) (after
+    // level nz, c0 holds the top ghost c(nz+1)), prefetched c(s+2), w(s+1);
+    // diffz window b(k), q(k) = b/rho and the face flux it leaves for the
+    // level above (before level 1: b(0)); carried state and q of the level
+    // awaiting its last sweep; the advected value and its y flux
+    float c0[MC][NS], cp1[MC][NS], vk[MC][NS], Uc[MC][NS], W2k[MC][NS], dprev[MC][NS];
+    float nc2[MC][NS], nwz[MC][NS];
+    float b0[MC][NS], q0[MC][NS], gprev[MC][NS];
+    float dc[MC][NS], qc[MC][NS], P[MC][NS], phi[MC][NS];
 
-    __device__ __forceinline__ FluxTile(const FluxParams &pp, int q, int ic_, int mem_, int gi0,
-                                        int gj0, float *sm, const ZLev *zt_)
+    __device__ __forceinline__ FluxTile(const FluxParams &pp, const int *qv, int gi0, int gj0,
+                                        float *sm, const ZLev *zt_)
         : p(pp)
     {
         zt = zt_;
-        const int nx = p.g.nx, ny = p.g.ny;
+        const int nx = p.g.nx, ny = p.g.ny, nc = p.g.nc;
         nz = p.g.nz;
-        ic = ic_;
-        mem = mem_;
         NXY = (unsigned)p.g.NXY;
         SZ = (unsigned)((nx + 2) * (ny + 2));
         li = threadIdx.x & (RX - 1);
         w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         SP = sm;
-        SF = sm + RX * RY;
-        SQ = sm + 2 * RX * RY;
+        SF = sm + MC * PL;
+        SQ = sm + 2 * MC * PL;
         const size_t G = (size_t)p.g.G, N = (size_t)p.g.N;
-        rin = mkbuf(p.cin + (size_t)q * G, 4 * G);
-        rout = mkbuf(p.cout + (size_t)q * G, 4 * G);
         rf = mkbuf(p.fbase, p.fbytes);
-        oW = p.oW + 4u * (unsigned)(ic * N);
-        oBF = p.oBF + 4u * (unsigned)ic * NXY;
-        oTF = p.oTF + 4u * (unsigned)ic * NXY;
+        const int nq = nc * p.g.nm;
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            live[c] = qv[c] < nq;
+            const int q = live[c] ? qv[c] : qv[0];
+            icv[c] = q % nc;
+            memv[c] = q / nc;
+            rin[c] = mkbuf(p.cin + (size_t)q * G, 4 * G);
+            rout[c] = mkbuf(p.cout + (size_t)q * G, 4 * G);
+            oW[c] = p.oW + 4u * (unsigned)(icv[c] * N);
+            oBF[c] = p.oBF + 4u * (unsigned)icv[c] * NXY;
+            oTF[c] = p.oTF + 4u * (unsigned)icv[c] * NXY;
+        }
         const int gi = gi0 + li;
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
@@ -249,6 +264,10 @@ struct FluxTile {
 
     __device__ __forceinline__ bool has(int sl, unsigned b) const { return (fl[sl] & b) != 0u; }
     __device__ __forceinline__ bool inner(int sl) const { return !EDGE || has(sl, F_INNER); }
+    __device__ __forceinline__ float *row(float *plane, int c, int r) const
+    {
+        return plane + c * PL + r * RX + li;
+    }
 
     __device__ __forceinline__ void init()
     {
@@ -257,31 +276,35 @@ struct FluxTile {
         dtop = p.dt * zt[nz].ldz;
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
-            const float c1 = bld(rin, co4[sl], 4 * SZ);
-            const float c2 = bld(rin, co4[sl], 8 * SZ);
-            const float gb = c1 * bld(rf, m4[sl], oBF);  // c(0), setbcc.f90:108-113
-            const float v1 = bld(rf, m4[sl], oW);
-            c0[sl] = c1;
-            cp1[sl] = c2;
-            vk[sl] = v1;
-            const float d1 = (c2 - c1) * z1.dzk;
-            const float d0 = (c1 - gb) * ldz0;
-            const float av = fabsf(v1);
-            W2k[sl] = minmod(d1, d0) * av * (z1.a2 - z1.a2bc * av);
-            Uc[sl] = z1.dtdzz * (v1 * c1 - v1 * gb);  // U(1) = W1(1)
-            dprev[sl] = d1;
-            gprev[sl] = gb;
-            nc2[sl] = (3 <= nz) ? bld(rin, co4[sl], 12 * SZ) : 0.f;
-            nwz[sl] = bld(rf, m4[sl], oW + 4 * NXY);
             AX[sl] = bld2(rf, m16[sl], p.oA);  // level 1
             AY[sl] = bld2(rf, m16[sl] + 8u, p.oA);
             Dn[sl] = bld4(rf, m16[sl], p.oD);
             D0[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
             gv[sl] = EDGE ? bld(rf, m16[sl] + (has(sl, F_GX) ? 0u : 8u), p.oA) : 0.f;
-            b0[sl] = q0[sl] = 0.f;
-            dc[sl] = qc[sl] = dyl[sl] = dyu[sl] = pyo[sl] = pym[sl] = pyp[sl] = 0.f;
+            dyl[sl] = dyu[sl] = pyo[sl] = pym[sl] = pyp[sl] = 0.f;
             lrc[sl] = dxc[sl] = pxc[sl] = 0.f;
-            P[sl] = phi[sl] = 0.f;
+#pragma unroll
+            for (int c = 0; c < MC; ++c) {
+                const float c1 = bld(rin[c], co4[sl], 4 * SZ);
+                const float c2 = bld(rin[c], co4[sl], 8 * SZ);
+                const float gb = c1 * bld(rf, m4[sl], oBF[c]);  // c(0), setbcc.f90:108-113
+                const float v1 = bld(rf, m4[sl], oW[c]);
+                c0[c][sl] = c1;
+                cp1[c][sl] = c2;
+                vk[c][sl] = v1;
+                const float d1 = (c2 - c1) * z1.dzk;
+                const float d0 = (c1 - gb) * ldz0;
+                const float av = fabsf(v1);
+                W2k[c][sl] = minmod(d1, d0) * av * (z1.a2 - z1.a2bc * av);
+                Uc[c][sl] = z1.dtdzz * (v1 * c1 - v1 * gb);  // U(1) = W1(1)
+                dprev[c][sl] = d1;
+                gprev[c][sl] = gb;
+                nc2[c][sl] = (3 <= nz) ? bld(rin[c], co4[sl], 12 * SZ) : 0.f;
+                nwz[c][sl] = bld(rf, m4[sl], oW[c] + 4 * NXY);
+                b0[c][sl] = q0[c][sl] = 0.f;
+                dc[c][sl] = qc[c][sl] = 0.f;
+                P[c][sl] = phi[c][sl] = 0.f;
+            }
         }
     }
 
@@ -298,49 +321,53 @@ struct FluxTile {
             const unsigned lz = 4u * NXY * (unsigned)s;  // byte offset of level s+1 in a 3-D field
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
-                float cp2 = 0.f, vkn = 0.f, W1n, W2n, dn = 0.f;
-                if (STEADY || s < nzl) {
-                    // top ghost c(nz+1) = tfac*c(nz) (setbcc.f90:126-130)
-                    cp2 = (STEADY || s + 2 <= nzl)
-                              ? nc2[sl]
-                              : bld(rf, m4[sl], oTF) * cp1[sl];
-                    vkn = nwz[sl];
-                    W1n = zn.dtdzz * (vkn * cp1[sl] - vk[sl] * c0[sl]);
-                    dn = (cp2 - cp1[sl]) * zn.dzk;
-                    const float av = 0.5f * fabsf(vkn + vk[sl]);
-                    W2n = minmod(dn, dprev[sl]) * av * (zn.a2 - zn.a2bc * av);
-                } else {
-                    W1n = dtop * vk[sl] * (cp1[sl] - c0[sl]);  // advctzc.f90:175-181
-                    W2n = W2k[sl];
-                }
-                const float Un = W1n + (W2n - W2k[sl]);
-                float a = c0[sl] - ((vk[sl] > 0.f) ? Uc[sl] : Un);
+                bool zero = false;
                 if (EDGE) {
                     // ghost faces: zero when the wind at the face cell blows
                     // out of the domain through it, else the adjacent cell
                     const float v = gv[sl];
-                    const bool zero = has(sl, F_GLO) ? v >= 0.f : v <= 0.f;
+                    zero = has(sl, F_GLO) ? v >= 0.f : v <= 0.f;
                     if (STEADY || s + 1 <= nzl)
                         gv[sl] = bld(rf, m16[sl] + (has(sl, F_GX) ? 0u : 8u), p.oA + 4u * lz);
-                    a = inner(sl) ? a : (has(sl, F_GHOST) && !zero ? c0[sl] : 0.f);
                 }
-                c0[sl] = cp1[sl];
-                cp1[sl] = cp2;
-                vk[sl] = vkn;
-                Uc[sl] = Un;
-                W2k[sl] = W2n;
-                dprev[sl] = dn;
-                if (STEADY || s + 3 <= nzl) nc2[sl] = bld(rin, co4[sl], 4u * SZ * (unsigned)(s + 3));
-                if (STEADY || s + 2 <= nzl) nwz[sl] = bld(rf, m4[sl], oW + lz + 4u * NXY);
-                if (!ODD) {
-                    a = adv_x<EX>(a, AX[sl].x, AX[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
-                    if (STEADY || s + 1 <= nzl) AX[sl] = bld2(rf, m16[sl], p.oA + 4u * lz);
+#pragma unroll
+                for (int c = 0; c < MC; ++c) {
+                    float cp2 = 0.f, vkn = 0.f, W1n, W2n, dn = 0.f;
+                    if (STEADY || s < nzl) {
+                        // top ghost c(nz+1) = tfac*c(nz) (setbcc.f90:126-130)
+                        cp2 = (STEADY || s + 2 <= nzl) ? nc2[c][sl]
+                                                       : bld(rf, m4[sl], oTF[c]) * cp1[c][sl];
+                        vkn = nwz[c][sl];
+                        W1n = zn.dtdzz * (vkn * cp1[c][sl] - vk[c][sl] * c0[c][sl]);
+                        dn = (cp2 - cp1[c][sl]) * zn.dzk;
+                        const float av = 0.5f * fabsf(vkn + vk[c][sl]);
+                        W2n = minmod(dn, dprev[c][sl]) * av * (zn.a2 - zn.a2bc * av);
+                    } else {
+                        W1n = dtop * vk[c][sl] * (cp1[c][sl] - c0[c][sl]);  // advctzc.f90:175-181
+                        W2n = W2k[c][sl];
+                    }
+                    const float Un = W1n + (W2n - W2k[c][sl]);
+                    float a = c0[c][sl] - ((vk[c][sl] > 0.f) ? Uc[c][sl] : Un);
+                    if (EDGE) a = inner(sl) ? a : (has(sl, F_GHOST) && !zero ? c0[c][sl] : 0.f);
+                    c0[c][sl] = cp1[c][sl];
+                    cp1[c][sl] = cp2;
+                    vk[c][sl] = vkn;
+                    Uc[c][sl] = Un;
+                    W2k[c][sl] = W2n;
+                    dprev[c][sl] = dn;
+                    if (STEADY || s + 3 <= nzl)
+                        nc2[c][sl] = bld(rin[c], co4[sl], 4u * SZ * (unsigned)(s + 3));
+                    if (STEADY || s + 2 <= nzl) nwz[c][sl] = bld(rf, m4[sl], oW[c] + lz + 4u * NXY);
+                    if (!ODD) a = adv_x<EX>(a, AX[sl].x, AX[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
+                    P[c][sl] = a;
                 }
-                P[sl] = a;
+                if (!ODD && (STEADY || s + 1 <= nzl)) AX[sl] = bld2(rf, m16[sl], p.oA + 4u * lz);
             }
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl)
-                if (sl == 0 || sl == NS - 1) SP[lj[sl] * RX + li] = P[sl];
+                if (sl == 0 || sl == NS - 1)
+#pragma unroll
+                    for (int c = 0; c < MC; ++c) *row(SP, c, lj[sl]) = P[c][sl];
         }
         __syncthreads();
 
@@ -349,11 +376,14 @@ struct FluxTile {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 if (lj[sl] < 1 || lj[sl] >= RY - 1) continue;  // wave-uniform
-                const float Pm = (sl > 0) ? P[sl - 1] : SP[(lj[sl] - 1) * RX + li];
-                const float Pp = (sl < NS - 1) ? P[sl + 1] : SP[(lj[sl] + 1) * RX + li];
-                const float w2 = AY[sl].y * minmod(Pp - P[sl], P[sl] - Pm);
-                phi[sl] = fmaf(AY[sl].x, P[sl], w2);
-                if (sl == 0 || sl == NS - 1) SF[lj[sl] * RX + li] = phi[sl];
+#pragma unroll
+                for (int c = 0; c < MC; ++c) {
+                    const float Pm = (sl > 0) ? P[c][sl - 1] : *row(SP, c, lj[sl] - 1);
+                    const float Pp = (sl < NS - 1) ? P[c][sl + 1] : *row(SP, c, lj[sl] + 1);
+                    const float w2 = AY[sl].y * minmod(Pp - P[c][sl], P[c][sl] - Pm);
+                    phi[c][sl] = fmaf(AY[sl].x, P[c][sl], w2);
+                    if (sl == 0 || sl == NS - 1) *row(SF, c, lj[sl]) = phi[c][sl];
+                }
             }
         }
         if (STEADY || s >= 3) {
@@ -362,31 +392,35 @@ struct FluxTile {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 if (lj[sl] < 3 || lj[sl] >= RY - 3) continue;  // wave-uniform
-                // diffy (diffy.f90:30-44; line ends 20-29, 45-53)
-                const float qm = (sl > 0) ? qc[sl - 1] : SQ[(lj[sl] - 1) * RX + li];
-                const float qp = (sl < NS - 1) ? qc[sl + 1] : SQ[(lj[sl] + 1) * RX + li];
                 const float Du = (sl < NS - 1) ? dyl[sl + 1] : dyu[sl];
-                float glo = dyl[sl] * (qc[sl] - qm);
-                float ghi = Du * (qp - qc[sl]);
-                if (EY) {
-                    const float d2 = p.dtdy2;
-                    const bool lo = has(sl, F_YLO), hi = has(sl, F_YHI);
-                    glo = lo ? fmaxf(0.f, d2 * pyo[sl] * (qc[sl] - qm)) : (hi ? d2 * pym[sl] * (qc[sl] - qm) : glo);
-                    ghi = lo ? d2 * pyp[sl] * (qp - qc[sl]) : (hi ? fminf(0.f, d2 * pyo[sl] * (qp - qc[sl])) : ghi);
-                }
-                float out = dc[sl] + (ghi - glo);
-                if (EDGE) out = inner(sl) ? out : dc[sl];
-                if (ODD)
-                    out = diff_x<EX>(out, lrc[sl], dxc[sl], pxc[sl], p.dtdx2, inner(sl), has(sl, F_XLO),
-                                     has(sl, F_XHI));
-                const bool store = li >= 3 && li < RX - 3 && inner(sl);
-                if (store) {
-                    bst(fmaxf(0.f, out), rout, co4[sl], ko);  // sizloop.f90:77-78
-                    if (!STEADY && k == 1) {
-                        // accum.f90:22: deposit from c(:,:,1) before the clip, FP64
-                        const size_t t2 = m4[sl] / 4u;
-                        double *cd = p.cdry + ((size_t)mem * (p.g.nc + 1) + ic + 1) * NXY + t2;
-                        *cd = *cd - p.dt64 * (double)out * p.vsum1[(size_t)ic * NXY + t2];
+#pragma unroll
+                for (int c = 0; c < MC; ++c) {
+                    // diffy (diffy.f90:30-44; line ends 20-29, 45-53)
+                    const float qm = (sl > 0) ? qc[c][sl - 1] : *row(SQ, c, lj[sl] - 1);
+                    const float qp = (sl < NS - 1) ? qc[c][sl + 1] : *row(SQ, c, lj[sl] + 1);
+                    const float q = qc[c][sl];
+                    float glo = dyl[sl] * (q - qm);
+                    float ghi = Du * (qp - q);
+                    if (EY) {
+                        const float d2 = p.dtdy2;
+                        const bool lo = has(sl, F_YLO), hi = has(sl, F_YHI);
+                        glo = lo ? fmaxf(0.f, d2 * pyo[sl] * (q - qm)) : (hi ? d2 * pym[sl] * (q - qm) : glo);
+                        ghi = lo ? d2 * pyp[sl] * (qp - q) : (hi ? fminf(0.f, d2 * pyo[sl] * (qp - q)) : ghi);
+                    }
+                    float out = dc[c][sl] + (ghi - glo);
+                    if (EDGE) out = inner(sl) ? out : dc[c][sl];
+                    if (ODD)
+                        out = diff_x<EX>(out, lrc[sl], dxc[sl], pxc[sl], p.dtdx2, inner(sl),
+                                         has(sl, F_XLO), has(sl, F_XHI));
+                    const bool store = li >= 3 && li < RX - 3 && inner(sl) && (c == 0 || live[c]);
+                    if (store) {
+                        bst(fmaxf(0.f, out), rout[c], co4[sl], ko);  // sizloop.f90:77-78
+                        if (!STEADY && k == 1) {
+                            // accum.f90:22: deposit from c(:,:,1) before the clip, FP64
+                            const size_t t2 = m4[sl] / 4u;
+                            double *cd = p.cdry + ((size_t)memv[c] * (p.g.nc + 1) + icv[c] + 1) * NXY + t2;
+                            *cd = *cd - p.dt64 * (double)out * p.vsum1[(size_t)icv[c] * NXY + t2];
+                        }
                     }
                 }
             }
@@ -402,71 +436,82 @@ struct FluxTile {
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
             if (lj[sl] < 2 || lj[sl] >= RY - 2) continue;  // wave-uniform
-            float bnew;
-            if (STEADY || s <= nzl) {
-                const float Pm = (sl > 0) ? P[sl - 1] : SP[(lj[sl] - 1) * RX + li];
-                const float Pp = (sl < NS - 1) ? P[sl + 1] : SP[(lj[sl] + 1) * RX + li];
-                float pl = (sl > 0) ? phi[sl - 1] : SF[(lj[sl] - 1) * RX + li];
-                float pr = (sl < NS - 1) ? phi[sl + 1] : SF[(lj[sl] + 1) * RX + li];
-                const float ay = AY[sl].x;
-                if (EY) {
-                    // advcty.f90:49-51: W2 clamps to the line at j = 1 / ny,
-                    // A at the ghost row is v(1) / v(ny)
-                    const float w2 = phi[sl] - ay * P[sl];
-                    pl = has(sl, F_YLO) ? fmaf(ay, Pm, w2) : pl;
-                    pr = has(sl, F_YHI) ? fmaf(ay, Pp, w2) : pr;
-                }
-                bnew = (ay > 0.f) ? P[sl] + (pl - phi[sl]) : P[sl] + (phi[sl] - pr);
-                if (EDGE) bnew = inner(sl) ? bnew : P[sl];
-                if (ODD)
-                    bnew = adv_x<EX>(bnew, AX[sl].x, AX[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
-            } else {
-                bnew = c0[sl];  // b(nz+1): the top ghost of c_in
-            }
-            // diffz (diffz.f90:19-62) at level k = s-1, input b(k+1) = bnew
             const float lrn = Dn[sl].z, pzn = Dn[sl].w;  // level s (unused at s = nz+1)
+            const float4 Dk = D0[sl];                    // level k (unused at s = 1)
+            const float lr0 = Dk.z, pz0 = Dk.w;
+            const float ay = AY[sl].x;
+#pragma unroll
+            for (int c = 0; c < MC; ++c) {
+                float bnew;
+                if (STEADY || s <= nzl) {
+                    const float Pc = P[c][sl], ph = phi[c][sl];
+                    float pl = (sl > 0) ? phi[c][sl - 1] : *row(SF, c, lj[sl] - 1);
+                    float pr = (sl < NS - 1) ? phi[c][sl + 1] : *row(SF, c, lj[sl] + 1);
+                    if (EY) {
+                        // advcty.f90:49-51: W2 clamps to the line at j = 1 / ny,
+                        // A at the ghost row is v(1) / v(ny)
+                        const float Pm = (sl > 0) ? P[c][sl - 1] : *row(SP, c, lj[sl] - 1);
+                        const float Pp = (sl < NS - 1) ? P[c][sl + 1] : *row(SP, c, lj[sl] + 1);
+                        const float w2 = ph - ay * Pc;
+                        pl = has(sl, F_YLO) ? fmaf(ay, Pm, w2) : pl;
+                        pr = has(sl, F_YHI) ? fmaf(ay, Pp, w2) : pr;
+                    }
+                    bnew = (ay > 0.f) ? Pc + (pl - ph) : Pc + (ph - pr);
+                    if (EDGE) bnew = inner(sl) ? bnew : Pc;
+                    if (ODD)
+                        bnew = adv_x<EX>(bnew, AX[sl].x, AX[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
+                } else {
+                    bnew = c0[c][sl];  // b(nz+1): the top ghost of c_in
+                }
+                // diffz (diffz.f90:19-62) at level k = s-1, input b(k+1) = bnew
+                if (!STEADY && s == 1) {
+                    // bottom face: c(0)/rho(1), un-averaged rho*K(1), inflow only
+                    const float q1 = bnew * lrn;
+                    gprev[c][sl] = fmaxf(0.f, pzn * (q1 - gprev[c][sl] * lrn) * ldz0);
+                    b0[c][sl] = bnew;
+                    q0[c][sl] = q1;
+                    continue;
+                }
+                float d, qn = 0.f;
+                if (STEADY || k < nzl) {
+                    qn = bnew * lrn;
+                    const float dq = qn - q0[c][sl];
+                    const float gav = (pzn + pz0) * zk.hldz * dq;
+                    // level 1 uses the un-averaged rho*K(2) on its top face
+                    const float gt = (!STEADY && k == 1) ? pzn * dq * zk.ldz : gav;
+                    d = b0[c][sl] + zk.dtdzz * (gt - gprev[c][sl]);
+                    // level nz takes the un-averaged rho*K(nz-1) on its bottom face
+                    gprev[c][sl] = (!STEADY && k == nzl - 1) ? pz0 * dq * zk.ldz : gav;
+                } else {
+                    const float gt = fminf(0.f, pz0 * (bnew * lr0 - q0[c][sl]) * zk.ldz);
+                    d = b0[c][sl] + dtop * (gt - gprev[c][sl]);
+                }
+                if (EDGE) d = inner(sl) ? d : b0[c][sl];
+                q0[c][sl] = qn;
+                b0[c][sl] = bnew;
+                dc[c][sl] = d;  // first horizontal diffusion below (even steps)
+            }
             if (!STEADY && s == 1) {
-                // bottom face: c(0)/rho(1), un-averaged rho*K(1), inflow only
-                const float q1 = bnew * lrn;
-                gprev[sl] = fmaxf(0.f, pzn * (q1 - gprev[sl] * lrn) * ldz0);
-                b0[sl] = bnew;
-                q0[sl] = q1;
                 D0[sl] = Dn[sl];
                 Dn[sl] = bld4(rf, m16[sl], p.oD + 16u * NXY);  // level 2 (nz >= 2)
                 continue;
             }
-            const float4 Dk = D0[sl];
             D0[sl] = Dn[sl];
             if (STEADY || s + 1 <= nzl) Dn[sl] = bld4(rf, m16[sl], p.oD + 16u * NXY * (unsigned)s);
-            const float lr0 = Dk.z, pz0 = Dk.w;
-            float d, qn = 0.f;
-            if (STEADY || k < nzl) {
-                qn = bnew * lrn;
-                const float dq = qn - q0[sl];
-                const float gav = (pzn + pz0) * zk.hldz * dq;
-                // level 1 uses the un-averaged rho*K(2) on its top face
-                const float gt = (!STEADY && k == 1) ? pzn * dq * zk.ldz : gav;
-                d = b0[sl] + zk.dtdzz * (gt - gprev[sl]);
-                // level nz takes the un-averaged rho*K(nz-1) on its bottom face
-                gprev[sl] = (!STEADY && k == nzl - 1) ? pz0 * dq * zk.ldz : gav;
-            } else {
-                const float gt = fminf(0.f, pz0 * (bnew * lr0 - q0[sl]) * zk.ldz);
-                d = b0[sl] + dtop * (gt - gprev[sl]);
-            }
-            if (EDGE) d = inner(sl) ? d : b0[sl];
-            q0[sl] = qn;
-            b0[sl] = bnew;
             // first horizontal diffusion at level k
             if (!ODD) {
                 const float pk = EX ? bld2(rf, m4[sl] * 2u, p.oE + 2u * lk).x : 0.f;
-                d = diff_x<EX>(d, lr0, Dk.x, pk, p.dtdx2, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
+#pragma unroll
+                for (int c = 0; c < MC; ++c)
+                    dc[c][sl] = diff_x<EX>(dc[c][sl], lr0, Dk.x, pk, p.dtdx2, inner(sl), has(sl, F_XLO),
+                                           has(sl, F_XHI));
             } else {
                 lrc[sl] = lr0;
                 dxc[sl] = Dk.x;
                 if (EX) pxc[sl] = bld2(rf, m4[sl] * 2u, p.oE + 2u * lk).x;
             }
-            dc[sl] = d;
-            qc[sl] = d * lr0;
+#pragma unroll
+            for (int c = 0; c < MC; ++c) qc[c][sl] = dc[c][sl] * lr0;
             dyl[sl] = Dk.y;
             if (sl == NS - 1) dyu[sl] = bld(rf, mup16[sl] + 4u, p.oD + 4u * lk);
             if (EY) {
@@ -487,7 +532,8 @@ struct FluxTile {
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl)
             if ((sl == 0 || sl == NS - 1) && lj[sl] >= 2 && lj[sl] < RY - 2 && (STEADY || s >= 2))
-                SQ[lj[sl] * RX + li] = qc[sl];
+#pragma unroll
+                for (int c = 0; c < MC; ++c) *row(SQ, c, lj[sl]) = qc[c][sl];
     }
 
     // Interior tiles: prologue, steady and epilogue levels run as three loops
@@ -510,72 +556,128 @@ struct FluxTile {
     }
 };
 
-template <int RY, int NW, bool ODD, bool EX, bool EY>
-__device__ __forceinline__ void flux_tile(const FluxParams &p, const int q, const int ic,
-                                          const int mem, const int gi0, const int gj0, float *sm,
-                                          const ZLev *zt)
+template <int RY, int NW, int MC, bool ODD, bool EX, bool EY>
+__device__ __forceinline__ void flux_tile(const FluxParams &p, const int *qv, const int gi0,
+                                          const int gj0, float *sm, const ZLev *zt)
 {
-    FluxTile<RY, NW, ODD, EX, EY> t(p, q, ic, mem, gi0, gj0, sm, zt);
+    FluxTile<RY, NW, MC, ODD, EX, EY> t(p, qv, gi0, gj0, sm, zt);
     t.run();
 }
 
-template <int RY, int NW, bool ODD>
+// One workgroup per (tile, group of MC consecutive member*class slabs).
+// Interior tiles (the rectangle [bx0, bx1] x [by0, by1] of tiles whose region
+// lies inside the domain) and edge tiles run as two kernels, so the interior
+// one keeps the register budget of the plain path.
+template <int RY, int NW, int MC, bool ODD, bool EDGEK>
 __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
 {
     const int b = blockIdx.x;
     const int L = (b % kXCD) * p.per_xcd + b / kXCD;  // XCD-aware logical block
     if (L >= p.nlogical) return;
-    const int nc = p.g.nc;
-    const int nq = nc * p.g.nm;
-    const int q = L % nq;
-    const int ic = q % nc, mem = q / nc;
-    // consecutive tiles run down y so the 6-row halos are shared in L2
-    const int t = L / nq;
-    const int by = t % p.nty, bx = t / p.nty;
+    const int nq = p.g.nc * p.g.nm;
+    const int ngrp = (nq + MC - 1) / MC;
+    const int grp = L % ngrp;
+    int qv[MC];
+#pragma unroll
+    for (int c = 0; c < MC; ++c) qv[c] = grp * MC + c;
+    const int t = L / ngrp;
+    int bx, by;
+    if (!EDGEK) {
+        // consecutive tiles run down y so the 6-row halos are shared in L2
+        const int nyi = p.by1 - p.by0 + 1;
+        by = p.by0 + t % nyi;
+        bx = p.bx0 + t / nyi;
+    } else {
+        // edge tiles: full rows below by0, the side columns of rows by0..by1,
+        // full rows above by1
+        const int nside = p.bx0 + (p.ntx - 1 - p.bx1);
+        const int nmid = (p.by1 >= p.by0) ? (p.by1 - p.by0 + 1) : 0;
+        int e = t;
+        if (e < p.by0 * p.ntx) {
+            by = e / p.ntx;
+            bx = e % p.ntx;
+        } else if ((e -= p.by0 * p.ntx) < nmid * nside) {
+            by = p.by0 + e / nside;
+            const int c = e % nside;
+            bx = (c < p.bx0) ? c : p.bx1 + 1 + (c - p.bx0);
+        } else {
+            e -= nmid * nside;
+            by = p.by0 + nmid + e / p.ntx;
+            bx = e % p.ntx;
+        }
+    }
     const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * (RY - 6) - 3;
-    __shared__ float sm[3 * RX * RY];
+    __shared__ float sm[3 * MC * RX * RY];
     __shared__ ZLev zt[kMaxLevels + 2];
     for (int k = threadIdx.x; k <= p.g.nz + 1; k += NW * 64) zt[k] = p.zl[k];
     __syncthreads();
-    const bool ex = gi0 < 1 || gi0 + RX - 1 > p.g.nx;
-    const bool ey = gj0 < 1 || gj0 + RY - 1 > p.g.ny;
-    if (!ex && !ey)
-        flux_tile<RY, NW, ODD, false, false>(p, q, ic, mem, gi0, gj0, sm, zt);
-    else if (ex && ey)
-        flux_tile<RY, NW, ODD, true, true>(p, q, ic, mem, gi0, gj0, sm, zt);
-    else if (ex)
-        flux_tile<RY, NW, ODD, true, false>(p, q, ic, mem, gi0, gj0, sm, zt);
-    else
-        flux_tile<RY, NW, ODD, false, true>(p, q, ic, mem, gi0, gj0, sm, zt);
+    if (!EDGEK) {
+        flux_tile<RY, NW, MC, ODD, false, false>(p, qv, gi0, gj0, sm, zt);
+    } else {
+        const bool ex = gi0 < 1 || gi0 + RX - 1 > p.g.nx;
+        const bool ey = gj0 < 1 || gj0 + RY - 1 > p.g.ny;
+        if (ex && ey)
+            flux_tile<RY, NW, MC, ODD, true, true>(p, qv, gi0, gj0, sm, zt);
+        else if (ex)
+            flux_tile<RY, NW, MC, ODD, true, false>(p, qv, gi0, gj0, sm, zt);
+        else
+            flux_tile<RY, NW, MC, ODD, false, true>(p, qv, gi0, gj0, sm, zt);
+    }
 }
 
-template <int RY, int NW>
-int launch_flux(hipStream_t s, const FluxParams &p0, bool odd)
+template <int RY, int NW, int MC, bool EDGEK>
+void launch_part(hipStream_t s, FluxParams p, int ntiles, bool odd)
 {
-    FluxParams p = p0;
-    p.ntx = (p.g.nx + TXR - 1) / TXR;
-    p.nty = (p.g.ny + (RY - 6) - 1) / (RY - 6);
-    p.nlogical = p.ntx * p.nty * p.g.nc * p.g.nm;
+    if (ntiles <= 0) return;
+    const int ngrp = (p.g.nc * p.g.nm + MC - 1) / MC;
+    p.nlogical = ntiles * ngrp;
     p.per_xcd = (p.nlogical + kXCD - 1) / kXCD;
     const unsigned nb = (unsigned)(p.per_xcd * kXCD);
     if (odd)
-        hipLaunchKernelGGL((k_flux<RY, NW, true>), dim3(nb), dim3(NW * 64), 0, s, p);
+        hipLaunchKernelGGL((k_flux<RY, NW, MC, true, EDGEK>), dim3(nb), dim3(NW * 64), 0, s, p);
     else
-        hipLaunchKernelGGL((k_flux<RY, NW, false>), dim3(nb), dim3(NW * 64), 0, s, p);
+        hipLaunchKernelGGL((k_flux<RY, NW, MC, false, EDGEK>), dim3(nb), dim3(NW * 64), 0, s, p);
+}
+
+template <int RY, int NW, int MC>
+int launch_flux(hipStream_t s, const FluxParams &p0, bool odd)
+{
+    FluxParams p = p0;
+    const int nx = p.g.nx, ny = p.g.ny, ty = RY - 6;
+    p.ntx = (nx + TXR - 1) / TXR;
+    p.nty = (ny + ty - 1) / ty;
+    // tile bx is interior when 1 <= gi0 and gi0 + 63 <= nx (gi0 = bx*58 - 2),
+    // likewise in y with RY rows
+    p.bx0 = 1;
+    p.bx1 = min(p.ntx - 1, (nx - (RX - 3)) >= 0 ? (nx - (RX - 3)) / TXR : -1);
+    p.by0 = 1;
+    p.by1 = min(p.nty - 1, (ny - (RY - 3)) >= 0 ? (ny - (RY - 3)) / ty : -1);
+    if (p.bx1 < p.bx0 || p.by1 < p.by0) {
+        // no interior tile: every tile is an edge tile (all rows "below by0")
+        p.bx0 = 0;
+        p.bx1 = p.ntx - 1;
+        p.by0 = p.nty;
+        p.by1 = p.nty - 1;
+    }
+    const int nint = (p.bx1 - p.bx0 + 1) * (p.by1 >= p.by0 ? p.by1 - p.by0 + 1 : 0);
+    const int nedge = p.ntx * p.nty - nint;
+    launch_part<RY, NW, MC, false>(s, p, nint, odd);
+    F3D_CHECK_LAUNCH();
+    launch_part<RY, NW, 1, true>(s, p, nedge, odd);  // edge tiles: one slab per group
     F3D_CHECK_LAUNCH();
     return F3D_OK;
 }
 
 }  // namespace
 
-// (ty, waves) pairs of the flux kernel: region rows RY = ty + 6 split in
-// contiguous bands of RY / waves rows
+// Configurations of the flux kernel: ty output rows (region rows RY = ty + 6,
+// two per wavefront: 8, 12 or 16 waves) and mc = 1 or 2 member*class slabs
+// per workgroup.
 bool flux_supported(TileShape ts)
 {
     if (ts.tx != TXR) return false;
-    const int nw = ts.nw ? ts.nw : 8;
-    return (ts.ty == 10 && (nw == 8 || nw == 4)) || (ts.ty == 18 && (nw == 8 || nw == 6)) ||
-           (ts.ty == 26 && (nw == 8 || nw == 16));
+    if (ts.nw != 0 && ts.nw != (ts.ty + 6) / 2) return false;
+    return (ts.ty == 10 || ts.ty == 18 || ts.ty == 26) && (ts.mc == 0 || ts.mc == 1 || ts.mc == 2);
 }
 
 int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
@@ -610,14 +712,14 @@ int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
     p.dtdx2 = (float)(a.dt / (a.g.dx * a.g.dx));
     p.dtdy2 = (float)(a.dt / (a.g.dy * a.g.dy));
     const bool odd = (a.iiter % 2) != 0;
-    const int nw = ts.nw ? ts.nw : 8;
-    if (ts.ty == 10 && nw == 8) return launch_flux<16, 8>(s, p, odd);
-    if (ts.ty == 10 && nw == 4) return launch_flux<16, 4>(s, p, odd);
-    if (ts.ty == 18 && nw == 8) return launch_flux<24, 8>(s, p, odd);
-    if (ts.ty == 18 && nw == 6) return launch_flux<24, 6>(s, p, odd);
-    if (ts.ty == 26 && nw == 8) return launch_flux<32, 8>(s, p, odd);
-    if (ts.ty == 26 && nw == 16) return launch_flux<32, 16>(s, p, odd);
-    set_error("flux kernel: unsupported tile %dx%d with %d waves", ts.tx, ts.ty, nw);
+    const int mc = ts.mc ? ts.mc : 1;
+    if (ts.ty == 10 && mc == 1) return launch_flux<16, 8, 1>(s, p, odd);
+    if (ts.ty == 10 && mc == 2) return launch_flux<16, 8, 2>(s, p, odd);
+    if (ts.ty == 18 && mc == 1) return launch_flux<24, 12, 1>(s, p, odd);
+    if (ts.ty == 18 && mc == 2) return launch_flux<24, 12, 2>(s, p, odd);
+    if (ts.ty == 26 && mc == 1) return launch_flux<32, 16, 1>(s, p, odd);
+    if (ts.ty == 26 && mc == 2) return launch_flux<32, 16, 2>(s, p, odd);
+    set_error("flux kernel: unsupported tile %dx%d (%d slabs per group)", ts.tx, ts.ty, mc);
     return F3D_EINVAL;
 }
 

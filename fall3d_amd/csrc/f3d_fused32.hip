@@ -811,7 +811,7 @@ int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *v
 bool tile_supported(TileShape ts)
 {
     if (flux_supported(ts)) return true;
-    if (ts.nw != 0) return false;
+    if (ts.nw != 0 || ts.mc != 0) return false;
     const int ok[][2] = {{32, 8}, {32, 16}, {64, 8}, {64, 4}, {16, 16}};
     for (auto &t : ok)
         if (t[0] == ts.tx && t[1] == ts.ty) return true;

@@ -30,7 +30,7 @@ CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_get_clock",
                "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
-               "f3d_set_tile", "f3d_set_tile_waves", "f3d_timing", "f3d_kernel_time", "f3d_device_bytes",
+               "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time", "f3d_device_bytes",
                "f3d_last_error", "f3d_hbm_copy"]
 
 
@@ -78,7 +78,7 @@ def lib():
         L.f3d_set_cd_state.argtypes = [vp, d]
         L.f3d_sync.argtypes = [vp]
         L.f3d_set_tile.argtypes = [vp, i, i]
-        L.f3d_set_tile_waves.argtypes = [vp, i, i, i]
+        L.f3d_set_tile_config.argtypes = [vp, i, i, i, i]
         L.f3d_timing.argtypes = [vp, i]
         L.f3d_kernel_time.argtypes = [vp, i, dp, ctypes.POINTER(ctypes.c_long)]
         L.f3d_device_bytes.argtypes = [vp]
@@ -90,7 +90,7 @@ def lib():
                      "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                      "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_get_clock",
                      "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
-                     "f3d_set_tile", "f3d_set_tile_waves", "f3d_timing", "f3d_kernel_time"):
+                     "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time"):
             getattr(L, name).restype = c
         _L = L
     return _L
@@ -257,8 +257,9 @@ class Context:
     def cd_state(self, v):
         _check(lib().f3d_set_cd_state(self.h, float(v)), "f3d_set_cd_state")
 
-    def set_tile(self, tx, ty, waves=0):
-        _check(lib().f3d_set_tile_waves(self.h, int(tx), int(ty), int(waves)), "f3d_set_tile")
+    def set_tile(self, tx, ty, waves=0, slabs=0):
+        _check(lib().f3d_set_tile_config(self.h, int(tx), int(ty), int(waves), int(slabs)),
+               "f3d_set_tile")
 
     def timing(self, enable=True):
         _check(lib().f3d_timing(self.h, 1 if enable else 0), "f3d_timing")

@@ -163,10 +163,11 @@ int f3d_sync(f3d_ctx *ctx);
  * Supported: 32x8, 32x16, 64x8, 64x4, 16x16 (column-tile kernel) and
  * 58x10 (default), 58x18, 58x26 (flux kernel, 64-lane rows). */
 int f3d_set_tile(f3d_ctx *ctx, int tx, int ty);
-/* The same with the number of wavefronts per workgroup of the 58-column
- * kernel (0: its default of 8): 58x10 with 8 or 4, 58x18 with 8 or 6,
- * 58x26 with 8 or 16.  No counterpart in the reference. */
-int f3d_set_tile_waves(f3d_ctx *ctx, int tx, int ty, int waves);
+/* The same with the 58-column kernel's options: waves per workgroup (0 or
+ * (ty+6)/2) and member*class slabs per workgroup (0 or 1: one; 2: two slabs
+ * share the coefficient loads and barriers).  No counterpart in the
+ * reference. */
+int f3d_set_tile_config(f3d_ctx *ctx, int tx, int ty, int waves, int slabs);
 /* HIP-event timing of the step kernels (enabled by f3d_timing(ctx,1)):
  * average duration in ms and launch count of the dominant step kernel
  * since the last reset. */

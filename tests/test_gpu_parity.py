@@ -156,12 +156,12 @@ def test_fp32_context_vs_golden(path):
         r.close()
 
 
-TILES = [(32, 8), (16, 16), (64, 4), (58, 10), (58, 18), (58, 26), (58, 10, 4), (58, 18, 6),
-         (58, 26, 16)]
+TILES = [(32, 8), (16, 16), (64, 4), (58, 10), (58, 18), (58, 26), (58, 10, 0, 2), (58, 18, 0, 2),
+         (58, 26, 0, 2)]
 
 
 def _tile_id(t):
-    return f"{t[0]}x{t[1]}" + (f"w{t[2]}" if len(t) > 2 else "")
+    return f"{t[0]}x{t[1]}" + (f"m{t[3]}" if len(t) > 3 else "")
 
 
 def _compare_fp32_with_oracle(sc, nsteps, tile=None):
