@@ -51,9 +51,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--tile", default="58x10")
-    ap.add_argument("--slabs", type=int, default=0,
-                    help="member*class slabs per workgroup of the 58-column kernel (0: one)")
+    ap.add_argument("--tile", default="58x18")
+    ap.add_argument("--slabs", type=int, default=2,
+                    help="member*class slabs per workgroup of the 58-column kernel")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -171,7 +171,8 @@ def main():
                                f"fused FP32 step", "nx": sc.nx, "ny": sc.ny, "nz": sc.nz,
                    "nc": sc.nc, "tile": args.tile, "slabs": args.slabs, "dt_s": None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.tile),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(f"{args.tile}m{args.slabs}"),
                      "bytes_per_update": b_alg, "kernel_avg_ms": kms, "launches": kcount},
         "fp64_plain_updates_per_s": fp64,
         "speedup_vs_fp64_plain": (value / fp64) if fp64 else None,

@@ -117,7 +117,8 @@ int step_fp32(f3d_ctx *c)
     TRY(f32::setsrc(s, g, cin, c->d_srcoff.as<int>(), c->d_isrc.as<int>(), c->d_jsrc.as<int>(),
                     c->d_ksrc.as<int>(), c->d_tmrat.as<double>(), c->d_zlayer.as<double>(),
                     c->d_dz.as<double>(), c->d_hm1.as<double>(), c->dt));
-    f32::StepArgs a{g, cin, cout, c->d_cdry.as<double>(), c->dt, c->iiter, c->d_zc.as<ZCoef>()};
+    f32::StepArgs a{g,      cin,      cout,       c->d_cdry.as<double>(), c->dt, c->iiter,
+                    c->d_zc.as<ZCoef>(), c->side, c->ev_fork, c->ev_join};
     record_start(c);
     TRY(f32::fused_step(s, a, f, c->tile));
     record_stop(c);
@@ -196,9 +197,12 @@ static int create_impl(f3d_ctx **out, const f3d_grid *gr, int precision, int nm)
         for (double h : c->hm1)
             if (h != 1.0) c->uniform_hm1 = false;
     }
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        set_error("f3d_create: hipStreamCreate failed");
-        delete c;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        set_error("f3d_create: hipStreamCreate / hipEventCreate failed");
+        f3d_destroy(c);
         return F3D_EHIP;
     }
     long long *B = &c->bytes;
@@ -282,6 +286,11 @@ void f3d_destroy(f3d_ctx *c)
 {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    c->side = nullptr;
     for (auto e : c->ev_start) (void)hipEventDestroy(e);
     for (auto e : c->ev_stop) (void)hipEventDestroy(e);
     c->ev_start.clear();

@@ -84,6 +84,10 @@ struct StepArgs {
     double dt;
     int iiter;
     const ZCoef *zc;    // per level (FP64, converted in kernel)
+    // optional second stream and fork/join events: edge tiles of the flux
+    // kernel run beside the interior tiles
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 // Tile geometry of the fused kernel (chosen by the autotuner).  tx = 58
@@ -124,6 +128,8 @@ struct f3d_ctx {
     f3d::Grid g{};
     int lonlat = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;                     // edge tiles of the flux kernel
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     long long bytes = 0;
 
     // host copies
@@ -152,7 +158,7 @@ struct f3d_ctx {
     f3d::DeviceBuffer d_scratch, d_outmass, d_budget;
     // FP32 step fields
     f3d::DeviceBuffer d_f32, d_vsum1;
-    f3d::f32::TileShape tile{58, 10};
+    f3d::f32::TileShape tile{58, 18, 0, 2};
 
     // source points per member (host copies) and their device concatenation:
     // member m owns points [src_off[m], src_off[m+1])

@@ -536,15 +536,14 @@ struct FluxTile {
                 for (int c = 0; c < MC; ++c) *row(SQ, c, lj[sl]) = qc[c][sl];
     }
 
-    // Interior tiles: prologue, steady and epilogue levels run as three loops
-    // so each keeps its own register allocation; edge tiles run one generic
-    // loop.
+    // Prologue, steady and epilogue levels run as three loops so each keeps
+    // its own register allocation.
     __device__ __forceinline__ void run()
     {
         init();
         const int nzl = nz;
         int s = 1;
-        if (!EDGE) {
+        {
             for (; s <= 3; ++s) level<false>(s);
             // drain the loads of the prologue: the steady loop then enters
             // with the same pending loads as its own back edge, and its
@@ -640,7 +639,7 @@ void launch_part(hipStream_t s, FluxParams p, int ntiles, bool odd)
 }
 
 template <int RY, int NW, int MC>
-int launch_flux(hipStream_t s, const FluxParams &p0, bool odd)
+int launch_flux(hipStream_t s, const StepArgs &a, const FluxParams &p0, bool odd)
 {
     FluxParams p = p0;
     const int nx = p.g.nx, ny = p.g.ny, ty = RY - 6;
@@ -661,10 +660,23 @@ int launch_flux(hipStream_t s, const FluxParams &p0, bool odd)
     }
     const int nint = (p.bx1 - p.bx0 + 1) * (p.by1 >= p.by0 ? p.by1 - p.by0 + 1 : 0);
     const int nedge = p.ntx * p.nty - nint;
+    // edge tiles on the side stream when there is one; 1024-lane groups take
+    // one slab per group there (two would exceed the register budget)
+    constexpr int MCE = (NW * 64 > 768) ? 1 : MC;
+    const bool fork = a.side && a.fork && a.join && nint > 0 && nedge > 0;
+    hipStream_t se = fork ? a.side : s;
+    if (fork) {
+        F3D_HIP(hipEventRecord(a.fork, s));
+        F3D_HIP(hipStreamWaitEvent(se, a.fork, 0));
+    }
+    launch_part<RY, NW, MCE, true>(se, p, nedge, odd);
+    F3D_CHECK_LAUNCH();
     launch_part<RY, NW, MC, false>(s, p, nint, odd);
     F3D_CHECK_LAUNCH();
-    launch_part<RY, NW, 1, true>(s, p, nedge, odd);  // edge tiles: one slab per group
-    F3D_CHECK_LAUNCH();
+    if (fork) {
+        F3D_HIP(hipEventRecord(a.join, se));
+        F3D_HIP(hipStreamWaitEvent(s, a.join, 0));
+    }
     return F3D_OK;
 }
 
@@ -713,12 +725,12 @@ int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
     p.dtdy2 = (float)(a.dt / (a.g.dy * a.g.dy));
     const bool odd = (a.iiter % 2) != 0;
     const int mc = ts.mc ? ts.mc : 1;
-    if (ts.ty == 10 && mc == 1) return launch_flux<16, 8, 1>(s, p, odd);
-    if (ts.ty == 10 && mc == 2) return launch_flux<16, 8, 2>(s, p, odd);
-    if (ts.ty == 18 && mc == 1) return launch_flux<24, 12, 1>(s, p, odd);
-    if (ts.ty == 18 && mc == 2) return launch_flux<24, 12, 2>(s, p, odd);
-    if (ts.ty == 26 && mc == 1) return launch_flux<32, 16, 1>(s, p, odd);
-    if (ts.ty == 26 && mc == 2) return launch_flux<32, 16, 2>(s, p, odd);
+    if (ts.ty == 10 && mc == 1) return launch_flux<16, 8, 1>(s, a, p, odd);
+    if (ts.ty == 10 && mc == 2) return launch_flux<16, 8, 2>(s, a, p, odd);
+    if (ts.ty == 18 && mc == 1) return launch_flux<24, 12, 1>(s, a, p, odd);
+    if (ts.ty == 18 && mc == 2) return launch_flux<24, 12, 2>(s, a, p, odd);
+    if (ts.ty == 26 && mc == 1) return launch_flux<32, 16, 1>(s, a, p, odd);
+    if (ts.ty == 26 && mc == 2) return launch_flux<32, 16, 2>(s, a, p, odd);
     set_error("flux kernel: unsupported tile %dx%d (%d slabs per group)", ts.tx, ts.ty, mc);
     return F3D_EINVAL;
 }

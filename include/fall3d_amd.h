@@ -161,7 +161,8 @@ int f3d_sync(f3d_ctx *ctx);
 /* Tile shape (columns x rows) of the fused FP32 step kernel.  No
  * counterpart in the reference: a tuning knob set by the autotuner.
  * Supported: 32x8, 32x16, 64x8, 64x4, 16x16 (column-tile kernel) and
- * 58x10 (default), 58x18, 58x26 (flux kernel, 64-lane rows). */
+ * 58x10, 58x18, 58x26 (flux kernel, 64-lane rows).  Default: 58x18 with two
+ * member*class slabs per workgroup (f3d_set_tile_config). */
 int f3d_set_tile(f3d_ctx *ctx, int tx, int ty);
 /* The same with the 58-column kernel's options: waves per workgroup (0 or
  * (ty+6)/2) and member*class slabs per workgroup (0 or 1: one; 2: two slabs
