@@ -4,6 +4,7 @@
 // Numeric grid scalars, the Fall3d.f90 clock) in HBM.  f3d_step runs the
 // loop body of Fall3d.f90:112-117 on the device; nothing is copied to the
 // host during a step.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -114,7 +115,8 @@ int step_fp32(f3d_ctx *c)
     float *cin = c->d_c32[c->cur].as<float>();
     float *cout = c->d_c32[1 - c->cur].as<float>();
     const f32::Fields f = f32::fields_view(g, c->d_f32.as<float>(), c->d_vsum1.as<double>());
-    TRY(f32::setsrc(s, g, cin, c->d_srcoff.as<int>(), c->d_isrc.as<int>(), c->d_jsrc.as<int>(),
+    TRY(f32::setsrc(s, g, cin, c->d_srcoff.as<int>(), c->d_srcgrp.as<int4>(), c->nsrcgrp,
+                    c->d_srcperm.as<int>(), c->d_isrc.as<int>(), c->d_jsrc.as<int>(),
                     c->d_ksrc.as<int>(), c->d_tmrat.as<double>(), c->d_zlayer.as<double>(),
                     c->d_dz.as<double>(), c->d_hm1.as<double>(), c->dt));
     f32::StepArgs a{g,      cin,      cout,       c->d_cdry.as<double>(), c->dt, c->iiter,
@@ -447,6 +449,25 @@ static int upload_sources(f3d_ctx *c)
         T.insert(T.end(), c->src_tm[m].begin(), c->src_tm[m].end());
     }
     const int n = off[g.nm];
+    // group each member's points by target cell, keeping the point order
+    // inside a group (the order setsrc.f90 adds them in)
+    std::vector<int> perm;
+    std::vector<int4> grp;
+    for (int m = 0; m < g.nm; ++m) {
+        std::vector<int> idx(off[m + 1] - off[m]);
+        for (size_t a = 0; a < idx.size(); ++a) idx[a] = off[m] + (int)a;
+        auto key = [&](int P) {
+            return ((long long)K[P] * (g.ny + 2) + J[P]) * (g.nx + 2) + I[P];
+        };
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
+        for (size_t a = 0; a < idx.size();) {
+            size_t b = a;
+            while (b < idx.size() && key(idx[b]) == key(idx[a])) ++b;
+            grp.push_back(make_int4(m, (int)perm.size(), (int)(b - a), 0));
+            perm.insert(perm.end(), idx.begin() + a, idx.begin() + b);
+            a = b;
+        }
+    }
     long long *B = &c->bytes;
     const size_t nn = n > 0 ? n : 1;
     int rc = 0;
@@ -454,6 +475,8 @@ static int upload_sources(f3d_ctx *c)
     rc |= c->d_jsrc.alloc(sizeof(int) * nn, B);
     rc |= c->d_ksrc.alloc(sizeof(int) * nn, B);
     rc |= c->d_tmrat.alloc(sizeof(double) * nn * g.nc, B);
+    rc |= c->d_srcperm.alloc(sizeof(int) * nn, B);
+    rc |= c->d_srcgrp.alloc(sizeof(int4) * (grp.empty() ? 1 : grp.size()), B);
     if (rc) return F3D_ENOMEM;
     TRY(upload(c->d_srcoff, off.data(), sizeof(int) * (g.nm + 1), c->stream));
     if (n > 0) {
@@ -461,7 +484,10 @@ static int upload_sources(f3d_ctx *c)
         TRY(upload(c->d_jsrc, J.data(), sizeof(int) * n, c->stream));
         TRY(upload(c->d_ksrc, K.data(), sizeof(int) * n, c->stream));
         TRY(upload(c->d_tmrat, T.data(), sizeof(double) * n * g.nc, c->stream));
+        TRY(upload(c->d_srcperm, perm.data(), sizeof(int) * n, c->stream));
+        TRY(upload(c->d_srcgrp, grp.data(), sizeof(int4) * grp.size(), c->stream));
     }
+    c->nsrcgrp = (int)grp.size();
     F3D_HIP(hipStreamSynchronize(c->stream));
     c->nsou = n;
     return F3D_OK;

@@ -111,10 +111,12 @@ int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts);
 int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,
                       const double *vy64, const double *vz64, const double *vset64,
                       const double *dz, double *scratch, double *outmass);
-// setsrc for every (member, class); off[m]..off[m+1] are member m's points
-int setsrc(hipStream_t s, const Grid &g, float *c, const int *off, const int *isrc,
-           const int *jsrc, const int *ksrc, const double *tmrat, const double *zlayer,
-           const double *dz, const double *hm1, double dt);
+// setsrc for every (member, class); off[m]..off[m+1] are member m's points,
+// grp/perm group them by target cell in point order
+int setsrc(hipStream_t s, const Grid &g, float *c, const int *off, const int4 *grp, int ngrp,
+           const int *perm, const int *isrc, const int *jsrc, const int *ksrc,
+           const double *tmrat, const double *zlayer, const double *dz, const double *hm1,
+           double dt);
 int to_fp64(hipStream_t s, const float *c32, double *c64, long n);
 int from_fp64(hipStream_t s, const double *c64, float *c32, long n);
 
@@ -167,6 +169,8 @@ struct f3d_ctx {
     std::vector<double> flowrate;             // per member
     int nsou = 0;                             // total points
     f3d::DeviceBuffer d_isrc, d_jsrc, d_ksrc, d_tmrat, d_srcoff;
+    f3d::DeviceBuffer d_srcgrp, d_srcperm;  // source points grouped by target cell
+    int nsrcgrp = 0;
 
     // clock (shared by the members of an ensemble: one dt)
     double time = 0.0, dt = 0.0, dtc = 0.0;

@@ -680,27 +680,36 @@ __global__ void k_fields_class(Grid g, const double *vz, const double *vset, con
     }
 }
 
-// setsrc.f90:20-35 per (member, class); member m owns source points
-// [off[m], off[m+1]) and its tmrat block (nsou_m, nc) starts at nc*off[m]
-__global__ void k_setsrc32(Grid g, float *c, const int *off, const int *isrc, const int *jsrc,
-                           const int *ksrc, const double *tmrat, const double *zlayer,
-                           const double *dz, const double *hm1, double dt)
+// setsrc.f90:20-35.  Member m owns source points [off[m], off[m+1]) and
+// its tmrat block (nsou_m, nc) starts at nc*off[m].  Points are grouped by
+// target cell (grp[g] = {member, first, count} into perm, in point order):
+// one thread per (group, class) adds its points in the reference order, so
+// every cell sees the same sequence of roundings as the serial loop.
+__global__ void k_setsrc32(Grid g, float *c, const int *off, const int4 *grp, int ngrp,
+                           const int *perm, const int *isrc, const int *jsrc, const int *ksrc,
+                           const double *tmrat, const double *zlayer, const double *dz,
+                           const double *hm1, double dt)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.nc * g.nm) return;
-    const int q = t % g.nc, mem = t / g.nc;
+    if (t >= ngrp * g.nc) return;
+    const int q = t % g.nc;
+    const int4 gr = grp[t / g.nc];
+    const int mem = gr.x;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int o = off[mem], nsou = off[mem + 1] - o;
     const double *tm = tmrat + (long)g.nc * o;
-    for (int is = 0; is < nsou; ++is) {
-        const int i = isrc[o + is], j = jsrc[o + is], k = ksrc[o + is];
-        if (k >= 1 && k <= nz) {
-            const double dzeta = (k == 1) ? 0.5 * dz[k] + zlayer[k - 1] : 0.5 * (dz[k - 1] + dz[k]);
-            const double vol0 = hm1[(i - 1) + (long)nx * (j - 1)] * g.dx * g.dy * dzeta;
-            float *cc = c + ((long)mem * g.nc + q) * g.G + cidx(nx, ny, i, j, k);
-            *cc = (float)((double)*cc + dt * tm[is + (long)nsou * q] / vol0);
-        }
+    const int P0 = perm[gr.y];
+    const int i = isrc[P0], j = jsrc[P0], k = ksrc[P0];
+    if (k < 1 || k > nz) return;
+    const double dzeta = (k == 1) ? 0.5 * dz[k] + zlayer[k - 1] : 0.5 * (dz[k - 1] + dz[k]);
+    const double vol0 = hm1[(i - 1) + (long)nx * (j - 1)] * g.dx * g.dy * dzeta;
+    float *cc = c + ((long)mem * g.nc + q) * g.G + cidx(nx, ny, i, j, k);
+    float v = *cc;
+    for (int n = 0; n < gr.z; ++n) {
+        const int is = perm[gr.y + n] - o;
+        v = (float)((double)v + dt * tm[is + (long)nsou * q] / vol0);
     }
+    *cc = v;
 }
 
 __global__ void k_to64(const float *a, double *b, long n)
@@ -854,13 +863,15 @@ int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const d
     return red::finish_rows(s, scratch, red::kFramePartials, a.g.nm, outmass, 1);
 }
 
-int setsrc(hipStream_t s, const Grid &g, float *c, const int *off, const int *isrc,
-           const int *jsrc, const int *ksrc, const double *tmrat, const double *zlayer,
-           const double *dz, const double *hm1, double dt)
+int setsrc(hipStream_t s, const Grid &g, float *c, const int *off, const int4 *grp, int ngrp,
+           const int *perm, const int *isrc, const int *jsrc, const int *ksrc,
+           const double *tmrat, const double *zlayer, const double *dz, const double *hm1,
+           double dt)
 {
-    const int n = g.nc * g.nm;
-    hipLaunchKernelGGL(k_setsrc32, dim3((n + 63) / 64), dim3(64), 0, s, g, c, off, isrc, jsrc,
-                       ksrc, tmrat, zlayer, dz, hm1, dt);
+    const int n = ngrp * g.nc;
+    if (n == 0) return F3D_OK;
+    hipLaunchKernelGGL(k_setsrc32, dim3((n + 255) / 256), dim3(256), 0, s, g, c, off, grp, ngrp,
+                       perm, isrc, jsrc, ksrc, tmrat, zlayer, dz, hm1, dt);
     F3D_CHECK_LAUNCH();
     return F3D_OK;
 }
