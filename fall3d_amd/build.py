@@ -34,6 +34,7 @@ UNITS = {
     "f3d_context.hip": EXACT,
     "f3d_fused32.hip": FAST,
     "f3d_flux32.hip": FAST,
+    "f3d_batch.hip": EXACT,
 }
 
 

@@ -1,0 +1,414 @@
+// Batched and autotuned FP32 steps (the batching layer of the C ABI).
+//
+// A batch is a set of FP32 contexts -- scenarios of any grid size, class
+// count and member count, each with its own meteorology, sources and dt --
+// stepped together.  One step of the batch is, on the device:
+//   setsrc of every context,
+//   one persistent launch over the (scenario, tile, slab group) work list of
+//     the interior tiles (and one of the edge tiles on a second stream),
+//   ghost frame + outflux of every context,
+// and the host side advances each context's clock as f3d_step does
+// (Fall3d.f90:112-114).  The device work of a step is captured once per
+// iiter parity in a HIP graph and replayed.  Every work item reads and
+// writes only its own scenario's buffers, so a scenario's result in the
+// batch is the one it gets stepped alone with the same tile shape.
+//
+// The autotuner times the flux step of each candidate tile shape on the
+// actual scenarios (deposit into scratch, output into the buffer the next
+// step overwrites, so no state changes) and keeps the fastest.
+#include <algorithm>
+#include <cfloat>
+#include <vector>
+
+#include "f3d_ctx.h"
+
+using namespace f3d;
+
+struct f3d_batch {
+    std::vector<f3d_ctx *> ctx;
+    hipStream_t stream = nullptr, side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    f32::TileShape ts{58, 18, 0, 2};
+    f32::FluxBatchPlan plan;
+    bool planned = false;
+    std::vector<unsigned> gen;     // context generations the plan was built for
+    std::vector<int> cur_odd;      // state buffer read by an odd step, per context
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // device work of an even / odd step
+    bool use_graph = true;
+};
+
+namespace {
+
+const f32::TileShape kCandidates[] = {{58, 10, 0, 1}, {58, 10, 0, 2}, {58, 18, 0, 1},
+                                      {58, 18, 0, 2}, {58, 26, 0, 1}, {58, 26, 0, 2}};
+
+bool check_batch(const f3d_batch *b)
+{
+    if (!b) {
+        set_error("null batch");
+        return false;
+    }
+    return true;
+}
+
+void drop_graphs(f3d_batch *b)
+{
+    for (auto &g : b->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// every context ready and on the same iiter parity
+int check_members(const f3d_batch *b)
+{
+    const int par = b->ctx[0]->iiter & 1;
+    for (const f3d_ctx *c : b->ctx) {
+        if (!c->meteo_ready) {
+            set_error("batch: a context has no meteo read yet");
+            return F3D_ESTATE;
+        }
+        if ((c->iiter & 1) != par) {
+            set_error("batch: contexts must be on the same iiter parity (%d vs %d)", c->iiter, b->ctx[0]->iiter);
+            return F3D_ESTATE;
+        }
+    }
+    return F3D_OK;
+}
+
+// step arguments of every context for an even and an odd step, reading the
+// buffer each will read at that parity; cdry_alt, when given, replaces the
+// deposit target (autotuning)
+void step_lists(f3d_batch *b, std::vector<f32::StepArgs> &even, std::vector<f32::StepArgs> &odd,
+                std::vector<f32::Fields> &f, const std::vector<double *> *cdry_alt)
+{
+    const size_t n = b->ctx.size();
+    even.resize(n);
+    odd.resize(n);
+    f.resize(n);
+    b->cur_odd.resize(n);
+    for (size_t k = 0; k < n; ++k) {
+        f3d_ctx *c = b->ctx[k];
+        // the next step has parity (iiter+1)&1 and reads buffer cur
+        const int next_odd = (c->iiter + 1) & 1;
+        const int cur_odd = next_odd ? c->cur : 1 - c->cur;
+        b->cur_odd[k] = cur_odd;
+        odd[k] = fp32_args(c, cur_odd, 1);
+        even[k] = fp32_args(c, 1 - cur_odd, 0);
+        if (cdry_alt) odd[k].cdry = even[k].cdry = (*cdry_alt)[k];
+        f[k] = fp32_fields(c);
+    }
+}
+
+int replan(f3d_batch *b)
+{
+    drop_graphs(b);
+    std::vector<f32::StepArgs> even, odd;
+    std::vector<f32::Fields> f;
+    step_lists(b, even, odd, f, nullptr);
+    b->planned = false;
+    TRY(f32::flux_batch_plan(b->plan, even, odd, f, b->ts));
+    b->gen.resize(b->ctx.size());
+    for (size_t k = 0; k < b->ctx.size(); ++k) b->gen[k] = b->ctx[k]->gen;
+    b->planned = true;
+    return F3D_OK;
+}
+
+bool plan_current(const f3d_batch *b)
+{
+    if (!b->planned) return false;
+    for (size_t k = 0; k < b->ctx.size(); ++k) {
+        const f3d_ctx *c = b->ctx[k];
+        if (c->gen != b->gen[k]) return false;
+        const int next_odd = (c->iiter + 1) & 1;
+        if ((next_odd ? c->cur : 1 - c->cur) != b->cur_odd[k]) return false;
+    }
+    return true;
+}
+
+// device work of one step of parity odd
+int record_step(f3d_batch *b, bool odd)
+{
+    hipStream_t s = b->stream;
+    std::vector<f32::StepArgs> args(b->ctx.size());
+    for (size_t k = 0; k < b->ctx.size(); ++k) {
+        const int cur_in = odd ? b->cur_odd[k] : 1 - b->cur_odd[k];
+        args[k] = fp32_args(b->ctx[k], cur_in, odd ? 1 : 0);
+        TRY(fp32_setsrc(b->ctx[k], s, args[k]));
+    }
+    TRY(f32::flux_batch_step(s, b->side, b->fork, b->join, b->plan, odd));
+    for (size_t k = 0; k < b->ctx.size(); ++k) TRY(fp32_frame(b->ctx[k], s, args[k]));
+    return F3D_OK;
+}
+
+int capture(f3d_batch *b, bool odd)
+{
+    hipGraph_t g = nullptr;
+    F3D_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = record_step(b, odd);
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        set_error("batch: graph capture failed: %s", hipGetErrorString(e));
+        return F3D_EHIP;
+    }
+    const hipError_t ei = hipGraphInstantiate(&b->graph[odd ? 1 : 0], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        b->graph[odd ? 1 : 0] = nullptr;
+        set_error("batch: graph instantiation failed: %s", hipGetErrorString(ei));
+        return F3D_EHIP;
+    }
+    return F3D_OK;
+}
+
+// average time (ms) of the flux step of the next parity with tile shape ts,
+// deposits into scratch; nothing of the contexts' state changes
+int time_candidate(f3d_batch *b, f32::TileShape ts, int trials, const std::vector<double *> &scr,
+                   double *ms)
+{
+    std::vector<f32::StepArgs> even, odd;
+    std::vector<f32::Fields> f;
+    step_lists(b, even, odd, f, &scr);
+    f32::FluxBatchPlan plan;
+    TRY(f32::flux_batch_plan(plan, even, odd, f, ts));
+    const bool next_odd = ((b->ctx[0]->iiter + 1) & 1) != 0;
+    hipEvent_t e0, e1;
+    F3D_HIP(hipEventCreate(&e0));
+    F3D_HIP(hipEventCreate(&e1));
+    int rc = f32::flux_batch_step(b->stream, b->side, b->fork, b->join, plan, next_odd);  // warm-up
+    if (!rc) rc = hipEventRecord(e0, b->stream) == hipSuccess ? F3D_OK : F3D_EHIP;
+    for (int t = 0; t < trials && !rc; ++t)
+        rc = f32::flux_batch_step(b->stream, b->side, b->fork, b->join, plan, next_odd);
+    if (!rc) rc = hipEventRecord(e1, b->stream) == hipSuccess ? F3D_OK : F3D_EHIP;
+    if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? F3D_OK : F3D_EHIP;
+    float t_ms = 0.f;
+    if (!rc) (void)hipEventElapsedTime(&t_ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        if (rc == F3D_EHIP) set_error("batch autotune: timing failed");
+        return rc;
+    }
+    *ms = t_ms / std::max(1, trials);
+    return F3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int f3d_batch_create(f3d_batch **out, f3d_ctx *const *ctxs, int n)
+{
+    if (!out || !ctxs || n < 1) {
+        set_error("f3d_batch_create: need at least one context");
+        return F3D_EINVAL;
+    }
+    *out = nullptr;
+    for (int k = 0; k < n; ++k) {
+        if (!ctxs[k] || ctxs[k]->precision != F3D_FP32) {
+            set_error("f3d_batch_create: context %d is not an FP32 context", k);
+            return F3D_EINVAL;
+        }
+        for (int j = 0; j < k; ++j)
+            if (ctxs[j] == ctxs[k]) {
+                set_error("f3d_batch_create: context %d appears twice", k);
+                return F3D_EINVAL;
+            }
+    }
+    f3d_batch *b = new f3d_batch;
+    b->ctx.assign(ctxs, ctxs + n);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+        set_error("f3d_batch_create: hipStreamCreate / hipEventCreate failed");
+        f3d_batch_destroy(b);
+        return F3D_EHIP;
+    }
+    // the contexts' own streams may hold pending work (meteo, sources)
+    for (f3d_ctx *c : b->ctx)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) {
+            set_error("f3d_batch_create: a context's stream failed");
+            f3d_batch_destroy(b);
+            return F3D_EHIP;
+        }
+    *out = b;
+    return F3D_OK;
+}
+
+void f3d_batch_destroy(f3d_batch *b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->side) (void)hipStreamSynchronize(b->side);
+    drop_graphs(b);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->side) (void)hipStreamDestroy(b->side);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int f3d_batch_set_tile(f3d_batch *b, int tx, int ty, int slabs)
+{
+    if (!check_batch(b)) return F3D_EINVAL;
+    const f32::TileShape ts{tx, ty, 0, slabs};
+    if (!f32::flux_supported(ts)) {
+        set_error("f3d_batch_set_tile: unsupported tile %dx%d (slabs %d)", tx, ty, slabs);
+        return F3D_EINVAL;
+    }
+    b->ts = ts;
+    b->planned = false;
+    drop_graphs(b);
+    return F3D_OK;
+}
+
+int f3d_batch_get_tile(const f3d_batch *b, int *tx, int *ty, int *slabs)
+{
+    if (!check_batch(b)) return F3D_EINVAL;
+    if (tx) *tx = b->ts.tx;
+    if (ty) *ty = b->ts.ty;
+    if (slabs) *slabs = b->ts.mc ? b->ts.mc : 1;
+    return F3D_OK;
+}
+
+int f3d_batch_use_graph(f3d_batch *b, int on)
+{
+    if (!check_batch(b)) return F3D_EINVAL;
+    b->use_graph = on != 0;
+    if (!b->use_graph) drop_graphs(b);
+    return F3D_OK;
+}
+
+int f3d_batch_step(f3d_batch *b, int nsteps)
+{
+    if (!check_batch(b)) return F3D_EINVAL;
+    for (int n = 0; n < nsteps; ++n) {
+        TRY(check_members(b));
+        if (!plan_current(b)) TRY(replan(b));
+        // Fall3d.f90:112-114 for every context
+        for (f3d_ctx *c : b->ctx) {
+            c->time = c->time + c->dt;
+            c->iiter = c->iiter + 1;
+            for (int m = 0; m < c->g.nm; ++m) c->erumass[m] = c->erumass[m] + c->flowrate[m] * c->dt;
+        }
+        const bool odd = (b->ctx[0]->iiter & 1) != 0;
+        if (b->use_graph) {
+            if (!b->graph[odd ? 1 : 0]) TRY(capture(b, odd));
+            F3D_HIP(hipGraphLaunch(b->graph[odd ? 1 : 0], b->stream));
+        } else {
+            TRY(record_step(b, odd));
+        }
+        for (f3d_ctx *c : b->ctx) c->cur = 1 - c->cur;
+    }
+    return F3D_OK;
+}
+
+int f3d_batch_sync(f3d_batch *b)
+{
+    if (!check_batch(b)) return F3D_EINVAL;
+    F3D_HIP(hipStreamSynchronize(b->stream));
+    return F3D_OK;
+}
+
+int f3d_batch_autotune(f3d_batch *b, int trials, int *tx, int *ty, int *slabs, double *best_ms)
+{
+    if (!check_batch(b)) return F3D_EINVAL;
+    TRY(check_members(b));
+    std::vector<double *> scr(b->ctx.size(), nullptr);
+    int rc = F3D_OK;
+    for (size_t k = 0; k < b->ctx.size() && !rc; ++k) {
+        const Grid &g = b->ctx[k]->g;
+        const size_t nb = sizeof(double) * (size_t)g.NXY * (g.nc + 1) * g.nm;
+        if (hipMalloc(&scr[k], nb) != hipSuccess) {
+            set_error("f3d_batch_autotune: scratch allocation failed");
+            rc = F3D_ENOMEM;
+        }
+    }
+    double best = DBL_MAX;
+    f32::TileShape pick = b->ts;
+    for (const f32::TileShape &ts : kCandidates) {
+        if (rc) break;
+        double ms = 0.0;
+        rc = time_candidate(b, ts, trials, scr, &ms);
+        if (!rc && ms < best) {
+            best = ms;
+            pick = ts;
+        }
+    }
+    (void)hipStreamSynchronize(b->stream);
+    for (double *p : scr)
+        if (p) (void)hipFree(p);
+    if (rc) return rc;
+    b->ts = pick;
+    b->planned = false;
+    drop_graphs(b);
+    if (tx) *tx = pick.tx;
+    if (ty) *ty = pick.ty;
+    if (slabs) *slabs = pick.mc ? pick.mc : 1;
+    if (best_ms) *best_ms = best;
+    return F3D_OK;
+}
+
+int f3d_autotune(f3d_ctx *c, int trials, int *tx, int *ty, int *slabs, double *best_ms)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (c->precision != F3D_FP32 || !c->meteo_ready) {
+        set_error("f3d_autotune: needs an FP32 context with meteo read");
+        return F3D_ESTATE;
+    }
+    const Grid &g = c->g;
+    double *scr = nullptr;
+    if (hipMalloc(&scr, sizeof(double) * (size_t)g.NXY * (g.nc + 1) * g.nm) != hipSuccess) {
+        set_error("f3d_autotune: scratch allocation failed");
+        return F3D_ENOMEM;
+    }
+    // the next step's arguments, deposit into scratch: cout is the buffer the
+    // next step overwrites
+    f32::StepArgs a = fp32_args(c, c->cur, c->iiter + 1);
+    a.cdry = scr;
+    const f32::Fields f = fp32_fields(c);
+    std::vector<f32::TileShape> cand(std::begin(kCandidates), std::end(kCandidates));
+    cand.push_back({16, 16, 0, 0});
+    cand.push_back({32, 8, 0, 0});
+    double best = DBL_MAX;
+    f32::TileShape pick = c->tile;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) ? F3D_OK : F3D_EHIP;
+    for (const f32::TileShape &ts : cand) {
+        if (rc) break;
+        rc = f32::fused_step(c->stream, a, f, ts);  // warm-up
+        if (!rc) rc = hipEventRecord(e0, c->stream) == hipSuccess ? F3D_OK : F3D_EHIP;
+        for (int t = 0; t < trials && !rc; ++t) rc = f32::fused_step(c->stream, a, f, ts);
+        if (!rc) rc = hipEventRecord(e1, c->stream) == hipSuccess ? F3D_OK : F3D_EHIP;
+        if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? F3D_OK : F3D_EHIP;
+        float ms = 0.f;
+        if (!rc) (void)hipEventElapsedTime(&ms, e0, e1);
+        const double avg = ms / std::max(1, trials);
+        if (!rc && avg < best) {
+            best = avg;
+            pick = ts;
+        }
+    }
+    (void)hipStreamSynchronize(c->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(scr);
+    if (rc) {
+        if (rc == F3D_EHIP) set_error("f3d_autotune: timing failed");
+        return rc;
+    }
+    c->tile = pick;
+    if (tx) *tx = pick.tx;
+    if (ty) *ty = pick.ty;
+    if (slabs) *slabs = pick.mc ? pick.mc : 1;
+    if (best_ms) *best_ms = best;
+    return F3D_OK;
+}
+
+}  // extern "C"

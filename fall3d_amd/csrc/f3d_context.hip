@@ -13,19 +13,39 @@
 
 using namespace f3d;
 
-namespace {
+namespace f3d {
 
-int upload(f3d::DeviceBuffer &b, const void *h, size_t n, hipStream_t s)
+f32::Fields fp32_fields(const f3d_ctx *c)
 {
-    F3D_HIP(hipMemcpyAsync(b.p, h, n, hipMemcpyHostToDevice, s));
-    return F3D_OK;
+    return f32::fields_view(c->g, c->d_f32.as<float>(), c->d_vsum1.as<double>());
 }
 
-#define TRY(x)                   \
-    do {                         \
-        int rc_ = (x);           \
-        if (rc_) return rc_;     \
-    } while (0)
+f32::StepArgs fp32_args(f3d_ctx *c, int cur_in, int iiter)
+{
+    f32::StepArgs a{c->g,      c->d_c32[cur_in].as<float>(), c->d_c32[1 - cur_in].as<float>(),
+                    c->d_cdry.as<double>(), c->dt, iiter, c->d_zc.as<ZCoef>(), c->side,
+                    c->ev_fork, c->ev_join};
+    return a;
+}
+
+int fp32_setsrc(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a)
+{
+    return f32::setsrc(s, c->g, const_cast<float *>(a.cin), c->d_srcoff.as<int>(),
+                       c->d_srcgrp.as<int4>(), c->nsrcgrp, c->d_srcperm.as<int>(),
+                       c->d_isrc.as<int>(), c->d_jsrc.as<int>(), c->d_ksrc.as<int>(),
+                       c->d_tmrat.as<double>(), c->d_zlayer.as<double>(), c->d_dz.as<double>(),
+                       c->d_hm1.as<double>(), c->dt);
+}
+
+int fp32_frame(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a)
+{
+    return f32::frame_and_outflux(s, a, fp32_fields(c), c->d_vx.as<double>(), c->d_vy.as<double>(),
+                                  c->d_vz.as<double>(), c->d_vset.as<double>(),
+                                  c->d_dz.as<double>(), c->d_scratch.as<double>(),
+                                  c->d_outmass.as<double>());
+}
+
+}  // namespace f3d
 
 bool check_ctx(const f3d_ctx *c)
 {
@@ -34,6 +54,14 @@ bool check_ctx(const f3d_ctx *c)
         return false;
     }
     return true;
+}
+
+namespace {
+
+int upload(f3d::DeviceBuffer &b, const void *h, size_t n, hipStream_t s)
+{
+    F3D_HIP(hipMemcpyAsync(b.p, h, n, hipMemcpyHostToDevice, s));
+    return F3D_OK;
 }
 
 void record_start(f3d_ctx *c)
@@ -110,23 +138,14 @@ int step_fp64(f3d_ctx *c)
 
 int step_fp32(f3d_ctx *c)
 {
-    const Grid &g = c->g;
     hipStream_t s = c->stream;
-    float *cin = c->d_c32[c->cur].as<float>();
-    float *cout = c->d_c32[1 - c->cur].as<float>();
-    const f32::Fields f = f32::fields_view(g, c->d_f32.as<float>(), c->d_vsum1.as<double>());
-    TRY(f32::setsrc(s, g, cin, c->d_srcoff.as<int>(), c->d_srcgrp.as<int4>(), c->nsrcgrp,
-                    c->d_srcperm.as<int>(), c->d_isrc.as<int>(), c->d_jsrc.as<int>(),
-                    c->d_ksrc.as<int>(), c->d_tmrat.as<double>(), c->d_zlayer.as<double>(),
-                    c->d_dz.as<double>(), c->d_hm1.as<double>(), c->dt));
-    f32::StepArgs a{g,      cin,      cout,       c->d_cdry.as<double>(), c->dt, c->iiter,
-                    c->d_zc.as<ZCoef>(), c->side, c->ev_fork, c->ev_join};
+    const f32::Fields f = fp32_fields(c);
+    const f32::StepArgs a = fp32_args(c, c->cur, c->iiter);
+    TRY(fp32_setsrc(c, s, a));
     record_start(c);
     TRY(f32::fused_step(s, a, f, c->tile));
     record_stop(c);
-    TRY(f32::frame_and_outflux(s, a, f, c->d_vx.as<double>(), c->d_vy.as<double>(),
-                               c->d_vz.as<double>(), c->d_vset.as<double>(), c->d_dz.as<double>(),
-                               c->d_scratch.as<double>(), c->d_outmass.as<double>()));
+    TRY(fp32_frame(c, s, a));
     c->cur = 1 - c->cur;
     return F3D_OK;
 }
@@ -136,6 +155,7 @@ int step_fp32(f3d_ctx *c)
 int rebuild_fields(f3d_ctx *c)
 {
     if (c->precision != F3D_FP32) return F3D_OK;
+    ++c->gen;
     const Grid &g = c->g;
     TRY(f32::build_fields(c->stream, g, c->d_vx.as<double>(), c->d_vy.as<double>(),
                           c->d_vz.as<double>(), c->d_rho.as<double>(), c->d_rkh1.as<double>(),
@@ -488,6 +508,7 @@ static int upload_sources(f3d_ctx *c)
         TRY(upload(c->d_srcgrp, grp.data(), sizeof(int4) * grp.size(), c->stream));
     }
     c->nsrcgrp = (int)grp.size();
+    ++c->gen;
     F3D_HIP(hipStreamSynchronize(c->stream));
     c->nsou = n;
     return F3D_OK;

@@ -106,6 +106,21 @@ int fused_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts);
 // {10, 18, 26}.
 bool flux_supported(TileShape ts);
 int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts);
+// Batched form of the flux step: every (scenario, tile, slab group) of a set
+// of FP32 contexts as one work list, drawn by persistent workgroups.
+struct FluxBatchPlan {
+    TileShape ts{58, 18, 0, 2};
+    DeviceBuffer par[2];   // kernel parameters per scenario, even / odd iiter
+    DeviceBuffer items;    // interior items, then edge items (int4 each)
+    DeviceBuffer counter;  // work counters of the two lists
+    int nint = 0, nedge = 0;
+    int grid_int = 0, grid_edge = 0;  // persistent grid sizes (resident capacity)
+};
+int flux_batch_plan(FluxBatchPlan &b, const std::vector<StepArgs> &even,
+                    const std::vector<StepArgs> &odd, const std::vector<Fields> &f, TileShape ts);
+// zero the counters, interior list on s, edge list on side (fork/join)
+int flux_batch_step(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join,
+                    const FluxBatchPlan &b, bool odd);
 // Ghost frame of c_out (setbcc values of c_in, clipped) and cmass2d
 // outflux; outmass[m] += member m's outflux (deterministic per member).
 int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,
@@ -131,6 +146,7 @@ struct f3d_ctx {
     int lonlat = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;                     // edge tiles of the flux kernel
+    unsigned gen = 0;  // bumped when fields or sources change (batched steps re-plan)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     long long bytes = 0;
 
@@ -186,3 +202,15 @@ struct f3d_ctx {
     double t_ms = 0.0;
     long t_count = 0;
 };
+
+// Pieces of the FP32 step, shared by f3d_step and the batched step.
+namespace f3d {
+f32::Fields fp32_fields(const f3d_ctx *c);
+// step arguments reading state buffer cur_in and writing the other
+f32::StepArgs fp32_args(f3d_ctx *c, int cur_in, int iiter);
+int fp32_setsrc(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a);
+int fp32_frame(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a);
+}  // namespace f3d
+
+// non-null and alive; sets the error message otherwise
+bool check_ctx(const f3d_ctx *c);

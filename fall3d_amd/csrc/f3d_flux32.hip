@@ -26,6 +26,9 @@
 // the last sweep by two, so a level costs two workgroup barriers.  Tiles whose
 // region meets the domain frame take the EX / EY paths (ghost columns / rows
 // from setbcc, line-end clamps of the limiter and of the diffusion faces).
+#include <algorithm>
+#include <vector>
+
 #include "f3d_ctx.h"
 
 namespace f3d {
@@ -692,9 +695,11 @@ bool flux_supported(TileShape ts)
     return (ts.ty == 10 || ts.ty == 18 || ts.ty == 26) && (ts.mc == 0 || ts.mc == 1 || ts.mc == 2);
 }
 
-int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
+// Kernel parameters of one (member-)scenario step: grid, state buffers,
+// field offsets, dt.
+static int make_params(const StepArgs &a, const Fields &f, FluxParams &p)
 {
-    FluxParams p{};
+    p = FluxParams{};
     p.g = a.g;
     p.cin = a.cin;
     p.cout = a.cout;
@@ -707,6 +712,10 @@ int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
     // 32-bit
     if (f.bytes >= (size_t)1 << 32 || (size_t)a.g.G * 4 >= (size_t)1 << 32) {
         set_error("flux kernel: fields (%zu B) or a class slab exceed 4 GiB", f.bytes);
+        return F3D_EINVAL;
+    }
+    if (a.g.nz > kMaxLevels) {
+        set_error("flux kernel: nz=%d exceeds the supported %d levels", a.g.nz, kMaxLevels);
         return F3D_EINVAL;
     }
     p.fbase = f.base;
@@ -723,6 +732,13 @@ int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
     p.dt = (float)a.dt;
     p.dtdx2 = (float)(a.dt / (a.g.dx * a.g.dx));
     p.dtdy2 = (float)(a.dt / (a.g.dy * a.g.dy));
+    return F3D_OK;
+}
+
+int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
+{
+    FluxParams p;
+    TRY(make_params(a, f, p));
     const bool odd = (a.iiter % 2) != 0;
     const int mc = ts.mc ? ts.mc : 1;
     if (ts.ty == 10 && mc == 1) return launch_flux<16, 8, 1>(s, a, p, odd);
@@ -733,6 +749,211 @@ int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
     if (ts.ty == 26 && mc == 2) return launch_flux<32, 16, 2>(s, a, p, odd);
     set_error("flux kernel: unsupported tile %dx%d (%d slabs per group)", ts.tx, ts.ty, mc);
     return F3D_EINVAL;
+}
+
+// ---------------- batched form: many scenarios, one persistent launch ----------------
+namespace {
+
+__global__ void k_zero_counters(unsigned *c) { c[threadIdx.x] = 0u; }
+
+// Persistent form of k_flux over a work list: item = (scenario, bx, by,
+// slab group).  Each workgroup draws items from an atomic counter until the
+// list is exhausted; every item writes cells no other item writes, so the
+// result does not depend on which workgroup takes which item.
+template <int RY, int NW, int MC, bool ODD, bool EDGEK>
+__global__ void __launch_bounds__(NW * 64)
+    k_flux_batch(const FluxParams *scen, const int4 *items, int nitems, unsigned *counter)
+{
+    constexpr int PW = sizeof(FluxParams) / sizeof(int);
+    static_assert(sizeof(FluxParams) % sizeof(int) == 0, "parameter block of whole dwords");
+    static_assert(PW <= NW * 64, "one thread per parameter dword");
+    __shared__ float sm[3 * MC * RX * RY];
+    __shared__ ZLev zt[kMaxLevels + 2];
+    __shared__ int sp[PW];
+    __shared__ int sitem[4];
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const int it = (int)atomicAdd(counter, 1u);
+            const int4 w = (it < nitems) ? items[it] : make_int4(-1, 0, 0, 0);
+            sitem[0] = w.x;
+            sitem[1] = w.y;
+            sitem[2] = w.z;
+            sitem[3] = w.w;
+        }
+        __syncthreads();
+        const int sc = __builtin_amdgcn_readfirstlane(sitem[0]);
+        if (sc < 0) return;  // list exhausted (the same value for every wave)
+        const int bx = __builtin_amdgcn_readfirstlane(sitem[1]);
+        const int by = __builtin_amdgcn_readfirstlane(sitem[2]);
+        const int grp = __builtin_amdgcn_readfirstlane(sitem[3]);
+        if (threadIdx.x < PW) sp[threadIdx.x] = reinterpret_cast<const int *>(scen + sc)[threadIdx.x];
+        __syncthreads();
+        FluxParams p;
+#pragma unroll
+        for (int k = 0; k < PW; ++k)
+            reinterpret_cast<int *>(&p)[k] = __builtin_amdgcn_readfirstlane(sp[k]);
+        for (int k = threadIdx.x; k <= p.g.nz + 1; k += NW * 64) zt[k] = p.zl[k];
+        __syncthreads();
+        int qv[MC];
+#pragma unroll
+        for (int c = 0; c < MC; ++c) qv[c] = grp * MC + c;
+        const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * (RY - 6) - 3;
+        if (!EDGEK) {
+            flux_tile<RY, NW, MC, ODD, false, false>(p, qv, gi0, gj0, sm, zt);
+        } else {
+            const bool ex = gi0 < 1 || gi0 + RX - 1 > p.g.nx;
+            const bool ey = gj0 < 1 || gj0 + RY - 1 > p.g.ny;
+            if (ex && ey)
+                flux_tile<RY, NW, MC, ODD, true, true>(p, qv, gi0, gj0, sm, zt);
+            else if (ex)
+                flux_tile<RY, NW, MC, ODD, true, false>(p, qv, gi0, gj0, sm, zt);
+            else
+                flux_tile<RY, NW, MC, ODD, false, true>(p, qv, gi0, gj0, sm, zt);
+        }
+        __syncthreads();  // LDS is reused by the next item
+    }
+}
+
+template <int RY, int NW, int MC, bool EDGEK>
+int batch_grid(int &grid)
+{
+    int occ = 0;
+    F3D_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_flux_batch<RY, NW, MC, false, EDGEK>,
+                                                         NW * 64, 0));
+    int dev = 0, ncu = 0;
+    F3D_HIP(hipGetDevice(&dev));
+    F3D_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    grid = std::max(1, occ) * ncu;
+    return F3D_OK;
+}
+
+template <int RY, int NW, int MC>
+int batch_launch(hipStream_t s, hipStream_t se, const FluxBatchPlan &b, bool odd)
+{
+    constexpr int MCE = (NW * 64 > 768) ? 1 : MC;
+    const FluxParams *par = b.par[odd ? 1 : 0].as<FluxParams>();
+    const int4 *items = b.items.as<int4>();
+    unsigned *ctr = b.counter.as<unsigned>();
+    if (b.nedge > 0) {
+        const unsigned g = (unsigned)std::min(b.grid_edge, b.nedge);
+        if (odd)
+            hipLaunchKernelGGL((k_flux_batch<RY, NW, MCE, true, true>), dim3(g), dim3(NW * 64), 0, se,
+                               par, items + b.nint, b.nedge, ctr + 1);
+        else
+            hipLaunchKernelGGL((k_flux_batch<RY, NW, MCE, false, true>), dim3(g), dim3(NW * 64), 0, se,
+                               par, items + b.nint, b.nedge, ctr + 1);
+        F3D_CHECK_LAUNCH();
+    }
+    if (b.nint > 0) {
+        const unsigned g = (unsigned)std::min(b.grid_int, b.nint);
+        if (odd)
+            hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, true, false>), dim3(g), dim3(NW * 64), 0, s,
+                               par, items, b.nint, ctr);
+        else
+            hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, false, false>), dim3(g), dim3(NW * 64), 0, s,
+                               par, items, b.nint, ctr);
+        F3D_CHECK_LAUNCH();
+    }
+    return F3D_OK;
+}
+
+// item lists of one scenario for region rows RY: interior tiles in groups of
+// mc slabs, edge tiles in groups of mce slabs
+void scenario_items(int sc, const Grid &g, int RY, int mc, int mce, std::vector<int4> &iint,
+                    std::vector<int4> &iedge)
+{
+    const int ty = RY - 6;
+    const int ntx = (g.nx + TXR - 1) / TXR, nty = (g.ny + ty - 1) / ty;
+    const int nq = g.nc * g.nm;
+    for (int bx = 0; bx < ntx; ++bx)
+        for (int by = 0; by < nty; ++by) {
+            const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * ty - 3;
+            const bool edge = gi0 < 1 || gi0 + RX - 1 > g.nx || gj0 < 1 || gj0 + RY - 1 > g.ny;
+            const int m = edge ? mce : mc;
+            for (int gr = 0; gr < (nq + m - 1) / m; ++gr)
+                (edge ? iedge : iint).push_back(make_int4(sc, bx, by, gr));
+        }
+}
+
+template <int RY, int NW, int MC>
+int batch_plan_t(FluxBatchPlan &b, const std::vector<StepArgs> &ae, const std::vector<StepArgs> &ao,
+                 const std::vector<Fields> &f)
+{
+    constexpr int MCE = (NW * 64 > 768) ? 1 : MC;
+    const size_t n = ae.size();
+    std::vector<FluxParams> pe(n), po(n);
+    std::vector<int4> iint, iedge;
+    for (size_t k = 0; k < n; ++k) {
+        TRY(make_params(ae[k], f[k], pe[k]));
+        TRY(make_params(ao[k], f[k], po[k]));
+        scenario_items((int)k, ae[k].g, RY, MC, MCE, iint, iedge);
+    }
+    // deeper columns first (cost per item grows with nz)
+    auto deeper = [&](const int4 &x, const int4 &y) { return ae[x.x].g.nz > ae[y.x].g.nz; };
+    std::stable_sort(iint.begin(), iint.end(), deeper);
+    std::stable_sort(iedge.begin(), iedge.end(), deeper);
+    b.nint = (int)iint.size();
+    b.nedge = (int)iedge.size();
+    iint.insert(iint.end(), iedge.begin(), iedge.end());
+    if (b.par[0].alloc(sizeof(FluxParams) * n, nullptr) || b.par[1].alloc(sizeof(FluxParams) * n, nullptr) ||
+        b.items.alloc(sizeof(int4) * std::max<size_t>(1, iint.size()), nullptr) ||
+        b.counter.alloc(2 * sizeof(unsigned), nullptr))
+        return F3D_ENOMEM;
+    F3D_HIP(hipMemcpy(b.par[0].p, pe.data(), sizeof(FluxParams) * n, hipMemcpyHostToDevice));
+    F3D_HIP(hipMemcpy(b.par[1].p, po.data(), sizeof(FluxParams) * n, hipMemcpyHostToDevice));
+    if (!iint.empty())
+        F3D_HIP(hipMemcpy(b.items.p, iint.data(), sizeof(int4) * iint.size(), hipMemcpyHostToDevice));
+    TRY((batch_grid<RY, NW, MC, false>(b.grid_int)));
+    TRY((batch_grid<RY, NW, MCE, true>(b.grid_edge)));
+    return F3D_OK;
+}
+
+}  // namespace
+
+int flux_batch_plan(FluxBatchPlan &b, const std::vector<StepArgs> &even,
+                    const std::vector<StepArgs> &odd, const std::vector<Fields> &f, TileShape ts)
+{
+    if (even.empty() || even.size() != odd.size() || even.size() != f.size()) {
+        set_error("batch plan: inconsistent scenario lists");
+        return F3D_EINVAL;
+    }
+    if (!flux_supported(ts)) {
+        set_error("batch plan: the batched step runs the 58-column kernel (got %dx%d)", ts.tx, ts.ty);
+        return F3D_EINVAL;
+    }
+    b.ts = ts;
+    const int mc = ts.mc ? ts.mc : 1;
+    if (ts.ty == 10) return mc == 1 ? batch_plan_t<16, 8, 1>(b, even, odd, f) : batch_plan_t<16, 8, 2>(b, even, odd, f);
+    if (ts.ty == 18) return mc == 1 ? batch_plan_t<24, 12, 1>(b, even, odd, f) : batch_plan_t<24, 12, 2>(b, even, odd, f);
+    return mc == 1 ? batch_plan_t<32, 16, 1>(b, even, odd, f) : batch_plan_t<32, 16, 2>(b, even, odd, f);
+}
+
+int flux_batch_step(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join,
+                    const FluxBatchPlan &b, bool odd)
+{
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(2), 0, s, b.counter.as<unsigned>());
+    F3D_CHECK_LAUNCH();
+    const bool split = side && fork && join && b.nint > 0 && b.nedge > 0;
+    hipStream_t se = split ? side : s;
+    if (split) {
+        F3D_HIP(hipEventRecord(fork, s));
+        F3D_HIP(hipStreamWaitEvent(se, fork, 0));
+    }
+    const TileShape ts = b.ts;
+    const int mc = ts.mc ? ts.mc : 1;
+    int rc;
+    if (ts.ty == 10)
+        rc = mc == 1 ? batch_launch<16, 8, 1>(s, se, b, odd) : batch_launch<16, 8, 2>(s, se, b, odd);
+    else if (ts.ty == 18)
+        rc = mc == 1 ? batch_launch<24, 12, 1>(s, se, b, odd) : batch_launch<24, 12, 2>(s, se, b, odd);
+    else
+        rc = mc == 1 ? batch_launch<32, 16, 1>(s, se, b, odd) : batch_launch<32, 16, 2>(s, se, b, odd);
+    if (rc) return rc;
+    if (split) {
+        F3D_HIP(hipEventRecord(join, se));
+        F3D_HIP(hipStreamWaitEvent(s, join, 0));
+    }
+    return F3D_OK;
 }
 
 }  // namespace f32

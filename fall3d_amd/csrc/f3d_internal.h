@@ -20,6 +20,13 @@ namespace f3d {
 
 void set_error(const char *fmt, ...);
 
+// propagate a nonzero F3D_* status
+#define TRY(x)                   \
+    do {                         \
+        int rc_ = (x);           \
+        if (rc_) return rc_;     \
+    } while (0)
+
 #define F3D_HIP(call)                                                                     \
     do {                                                                                  \
         hipError_t e_ = (call);                                                           \

@@ -31,7 +31,9 @@ CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_get_clock",
                "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
                "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time", "f3d_device_bytes",
-               "f3d_last_error", "f3d_hbm_copy"]
+               "f3d_last_error", "f3d_hbm_copy", "f3d_batch_create", "f3d_batch_destroy",
+               "f3d_batch_set_tile", "f3d_batch_get_tile", "f3d_batch_use_graph",
+               "f3d_batch_step", "f3d_batch_sync", "f3d_batch_autotune", "f3d_autotune"]
 
 
 class F3DError(RuntimeError):
@@ -85,6 +87,20 @@ def lib():
         L.f3d_device_bytes.restype = ctypes.c_longlong
         L.f3d_hbm_copy.argtypes = [ctypes.c_longlong, i, dp]
         L.f3d_hbm_copy.restype = c
+        L.f3d_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), i]
+        L.f3d_batch_destroy.argtypes = [vp]
+        L.f3d_batch_destroy.restype = None
+        L.f3d_batch_set_tile.argtypes = [vp, i, i, i]
+        L.f3d_batch_get_tile.argtypes = [vp, ip, ip, ip]
+        L.f3d_batch_use_graph.argtypes = [vp, i]
+        L.f3d_batch_step.argtypes = [vp, i]
+        L.f3d_batch_sync.argtypes = [vp]
+        L.f3d_batch_autotune.argtypes = [vp, i, ip, ip, ip, dp]
+        L.f3d_autotune.argtypes = [vp, i, ip, ip, ip, dp]
+        for name in ("f3d_batch_create", "f3d_batch_set_tile", "f3d_batch_get_tile",
+                     "f3d_batch_use_graph", "f3d_batch_step", "f3d_batch_sync",
+                     "f3d_batch_autotune", "f3d_autotune"):
+            getattr(L, name).restype = c
         for name in ("f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                      "f3d_ensemble_budget", "f3d_get_member_state", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
                      "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
@@ -274,6 +290,13 @@ class Context:
     def device_bytes(self):
         return int(lib().f3d_device_bytes(self.h))
 
+    def autotune(self, trials=3):
+        """Pick the fastest tile shape for this context (f3d_autotune)."""
+        tx, ty, sl, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().f3d_autotune(self.h, int(trials), ctypes.byref(tx), ctypes.byref(ty),
+                                  ctypes.byref(sl), ctypes.byref(ms)), "f3d_autotune")
+        return (tx.value, ty.value, sl.value), ms.value
+
 
 class Run:
     """Driver over one device context with the reference's control flow:
@@ -357,3 +380,54 @@ class Ensemble:
 
     def close(self):
         self.ctx.close()
+
+
+class Batch:
+    """A set of FP32 contexts stepped together (f3d_batch): one persistent
+    launch per step over all their work items, steps replayed from HIP
+    graphs.  Each context keeps its own clock, sources and budgets."""
+
+    def __init__(self, contexts, tile=None, graph=True):
+        self.contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = ctypes.c_void_p()
+        _check(lib().f3d_batch_create(ctypes.byref(h), arr, len(self.contexts)),
+               "f3d_batch_create")
+        self.h = h
+        if tile is not None:
+            self.set_tile(*tile)
+        if not graph:
+            _check(lib().f3d_batch_use_graph(self.h, 0), "f3d_batch_use_graph")
+
+    def set_tile(self, tx, ty, slabs=2):
+        _check(lib().f3d_batch_set_tile(self.h, int(tx), int(ty), int(slabs)),
+               "f3d_batch_set_tile")
+
+    def tile(self):
+        tx, ty, sl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().f3d_batch_get_tile(self.h, ctypes.byref(tx), ctypes.byref(ty),
+                                        ctypes.byref(sl)), "f3d_batch_get_tile")
+        return tx.value, ty.value, sl.value
+
+    def autotune(self, trials=3):
+        tx, ty, sl, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().f3d_batch_autotune(self.h, int(trials), ctypes.byref(tx), ctypes.byref(ty),
+                                        ctypes.byref(sl), ctypes.byref(ms)), "f3d_batch_autotune")
+        return (tx.value, ty.value, sl.value), ms.value
+
+    def step(self, n=1):
+        _check(lib().f3d_batch_step(self.h, int(n)), "f3d_batch_step")
+
+    def sync(self):
+        _check(lib().f3d_batch_sync(self.h), "f3d_batch_sync")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().f3d_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -179,6 +179,35 @@ long long f3d_device_bytes(const f3d_ctx *ctx);
 
 const char *f3d_last_error(void);
 
+/* ---------------- Layer 3: batched steps and tile autotuning ----------------
+ * No counterpart in the reference (its parallel form is one scenario split
+ * over MPI ranks).  A batch steps a set of FP32 contexts -- any grid sizes,
+ * class and member counts, each with its own meteorology, sources and dt --
+ * together: per step one persistent launch draws every (scenario, tile,
+ * slab group) work item from a queue, and the step's device work is
+ * captured in a HIP graph and replayed.  Each context keeps its own clock,
+ * sources and budgets (f3d_budget, f3d_get_state work on it as usual); a
+ * context's result in a batch equals the one it gets from f3d_step with the
+ * same tile shape.  The contexts must stay alive while the batch exists and
+ * be on the same iiter parity when stepped. */
+typedef struct f3d_batch f3d_batch;
+int f3d_batch_create(f3d_batch **batch, f3d_ctx *const *ctxs, int n);
+void f3d_batch_destroy(f3d_batch *batch);
+/* tile shape of the batched flux kernel: 58x10, 58x18 or 58x26, 1 or 2
+ * slabs per workgroup (default 58x18, 2) */
+int f3d_batch_set_tile(f3d_batch *batch, int tx, int ty, int slabs);
+int f3d_batch_get_tile(const f3d_batch *batch, int *tx, int *ty, int *slabs);
+/* capture steps in HIP graphs (default on) or launch them directly */
+int f3d_batch_use_graph(f3d_batch *batch, int on);
+int f3d_batch_step(f3d_batch *batch, int nsteps);
+int f3d_batch_sync(f3d_batch *batch);
+/* time each candidate tile shape over `trials` flux steps of the batch (no
+ * state changes) and keep the fastest; outputs may be null */
+int f3d_batch_autotune(f3d_batch *batch, int trials, int *tx, int *ty, int *slabs,
+                       double *best_ms);
+/* the same for one context and its f3d_step (column-tile shapes included) */
+int f3d_autotune(f3d_ctx *ctx, int trials, int *tx, int *ty, int *slabs, double *best_ms);
+
 /* No counterpart in the reference: a plain device copy of nbytes (read) +
  * nbytes (write), repeated reps times; *avg_ms is the average launch time.
  * It is the known-size kernel against which HBM counters (rocprofv3
