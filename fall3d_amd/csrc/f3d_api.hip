@@ -1,5 +1,6 @@
 // C ABI of libfall3d_amd: the reference-ABI per-routine drop-ins and the
 // device-resident context (include/fall3d_amd.h).
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -424,6 +425,17 @@ const char *f3d_last_error(void) { return f3d::g_err; }
 }  // extern "C"
 
 namespace {
+// dst(nx,ny,nz) = src(i0+1.., j0+1.., 1..) of a (snx,sny,*) array
+__global__ void k_window(double *dst, const double *src, int nx, int ny, int nz, int snx, int sny,
+                         int i0, int j0)
+{
+    const long n = (long)nx * ny * nz;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((long)nx * ny));
+        dst[t] = src[(i0 + i) + (long)snx * ((j0 + j) + (long)sny * k)];
+    }
+}
+
 // plain copy, one float4 per thread: the known-size kernel for HBM calibration
 __global__ void k_hbm_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long n)
 {
@@ -431,6 +443,27 @@ __global__ void k_hbm_copy(const float4 *__restrict__ src, float4 *__restrict__ 
     if (t < n) dst[t] = src[t];
 }
 }  // namespace
+
+namespace f3d {
+int window_copy(hipStream_t s, double *dst, const double *src, int nx, int ny, int nz, int snx,
+                int sny, int i0, int j0)
+{
+    const long n = (long)nx * ny * nz;
+    const unsigned nb = (unsigned)std::min<long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_window, dim3(nb), dim3(256), 0, s, dst, src, nx, ny, nz, snx, sny, i0, j0);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+}  // namespace f3d
+
+extern "C" int f3d_mem_info(long long *free_bytes, long long *total_bytes)
+{
+    size_t fr = 0, tot = 0;
+    F3D_HIP(hipMemGetInfo(&fr, &tot));
+    if (free_bytes) *free_bytes = (long long)fr;
+    if (total_bytes) *total_bytes = (long long)tot;
+    return F3D_OK;
+}
 
 extern "C" int f3d_hbm_copy(long long nbytes, int reps, double *avg_ms)
 {

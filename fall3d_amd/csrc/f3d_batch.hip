@@ -33,6 +33,7 @@ struct f3d_batch {
     bool planned = false;
     std::vector<unsigned> gen;     // context generations the plan was built for
     std::vector<int> cur_odd;      // state buffer read by an odd step, per context
+    std::vector<int> group;        // 1: on the other iiter parity than context 0
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // device work of an even / odd step
     bool use_graph = true;
 };
@@ -60,20 +61,14 @@ void drop_graphs(f3d_batch *b)
         }
 }
 
-// every context ready and on the same iiter parity
+// every context has its meteorology
 int check_members(const f3d_batch *b)
 {
-    const int par = b->ctx[0]->iiter & 1;
-    for (const f3d_ctx *c : b->ctx) {
+    for (const f3d_ctx *c : b->ctx)
         if (!c->meteo_ready) {
             set_error("batch: a context has no meteo read yet");
             return F3D_ESTATE;
         }
-        if ((c->iiter & 1) != par) {
-            set_error("batch: contexts must be on the same iiter parity (%d vs %d)", c->iiter, b->ctx[0]->iiter);
-            return F3D_ESTATE;
-        }
-    }
     return F3D_OK;
 }
 
@@ -88,12 +83,15 @@ void step_lists(f3d_batch *b, std::vector<f32::StepArgs> &even, std::vector<f32:
     odd.resize(n);
     f.resize(n);
     b->cur_odd.resize(n);
+    b->group.resize(n);
+    const int next0 = (b->ctx[0]->iiter + 1) & 1;
     for (size_t k = 0; k < n; ++k) {
         f3d_ctx *c = b->ctx[k];
         // the next step has parity (iiter+1)&1 and reads buffer cur
         const int next_odd = (c->iiter + 1) & 1;
         const int cur_odd = next_odd ? c->cur : 1 - c->cur;
         b->cur_odd[k] = cur_odd;
+        b->group[k] = next_odd != next0;
         odd[k] = fp32_args(c, cur_odd, 1);
         even[k] = fp32_args(c, 1 - cur_odd, 0);
         if (cdry_alt) odd[k].cdry = even[k].cdry = (*cdry_alt)[k];
@@ -108,7 +106,7 @@ int replan(f3d_batch *b)
     std::vector<f32::Fields> f;
     step_lists(b, even, odd, f, nullptr);
     b->planned = false;
-    TRY(f32::flux_batch_plan(b->plan, even, odd, f, b->ts));
+    TRY(f32::flux_batch_plan(b->plan, even, odd, f, b->group, b->ts));
     b->gen.resize(b->ctx.size());
     for (size_t k = 0; k < b->ctx.size(); ++k) b->gen[k] = b->ctx[k]->gen;
     b->planned = true;
@@ -118,26 +116,30 @@ int replan(f3d_batch *b)
 bool plan_current(const f3d_batch *b)
 {
     if (!b->planned) return false;
+    const int next0 = (b->ctx[0]->iiter + 1) & 1;
     for (size_t k = 0; k < b->ctx.size(); ++k) {
         const f3d_ctx *c = b->ctx[k];
         if (c->gen != b->gen[k]) return false;
         const int next_odd = (c->iiter + 1) & 1;
         if ((next_odd ? c->cur : 1 - c->cur) != b->cur_odd[k]) return false;
+        if ((next_odd != next0) != (b->group[k] != 0)) return false;
     }
     return true;
 }
 
-// device work of one step of parity odd
-int record_step(f3d_batch *b, bool odd)
+// device work of one step in which context 0 (and its parity group) takes
+// parity odd0
+int record_step(f3d_batch *b, bool odd0)
 {
     hipStream_t s = b->stream;
     std::vector<f32::StepArgs> args(b->ctx.size());
     for (size_t k = 0; k < b->ctx.size(); ++k) {
+        const bool odd = b->group[k] ? !odd0 : odd0;
         const int cur_in = odd ? b->cur_odd[k] : 1 - b->cur_odd[k];
         args[k] = fp32_args(b->ctx[k], cur_in, odd ? 1 : 0);
         TRY(fp32_setsrc(b->ctx[k], s, args[k]));
     }
-    TRY(f32::flux_batch_step(s, b->side, b->fork, b->join, b->plan, odd));
+    TRY(f32::flux_batch_step(s, b->side, b->fork, b->join, b->plan, odd0));
     for (size_t k = 0; k < b->ctx.size(); ++k) TRY(fp32_frame(b->ctx[k], s, args[k]));
     return F3D_OK;
 }
@@ -175,7 +177,7 @@ int time_candidate(f3d_batch *b, f32::TileShape ts, int trials, const std::vecto
     std::vector<f32::Fields> f;
     step_lists(b, even, odd, f, &scr);
     f32::FluxBatchPlan plan;
-    TRY(f32::flux_batch_plan(plan, even, odd, f, ts));
+    TRY(f32::flux_batch_plan(plan, even, odd, f, b->group, ts));
     const bool next_odd = ((b->ctx[0]->iiter + 1) & 1) != 0;
     hipEvent_t e0, e1;
     F3D_HIP(hipEventCreate(&e0));

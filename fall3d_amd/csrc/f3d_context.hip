@@ -356,6 +356,8 @@ int f3d_set_models(f3d_ctx *c, int modkv, int modkh, double rkv0, double rkh0, d
     return F3D_OK;
 }
 
+static int meteo_stage(f3d_ctx *c);
+
 int f3d_set_meteo(f3d_ctx *c, const double *vx, const double *vy, const double *vz,
                   const double *tempe, const double *vptemp, const double *rho,
                   const double *ustar, const double *zi, const double *rl, const double *vdep)
@@ -388,6 +390,15 @@ int f3d_set_meteo(f3d_ctx *c, const double *vx, const double *vy, const double *
     TRY(up_or_zero(c->d_zi, zi, g.NXY));
     TRY(up_or_zero(c->d_rl, rl, g.NXY));
     TRY(up_or_zero(c->d_vdep, vdep, g.NXY * g.nc));
+    return meteo_stage(c);
+}
+
+// kappa3, setvset, Hm1 scaling and finddt on the meteo fields in HBM, then
+// the FP32 step fields (meteo.f90:27-149)
+static int meteo_stage(f3d_ctx *c)
+{
+    const Grid &g = c->g;
+    hipStream_t s = c->stream;
     // meteo.f90:27 kappa3 (on the unscaled wind)
     TRY(met::kappa3(s, g, c->modkv, c->modkh, c->d_dz.as<double>(), c->d_zlayer.as<double>(),
                     c->d_vx.as<double>(), c->d_vy.as<double>(), c->d_vptemp.as<double>(),
@@ -431,6 +442,59 @@ int f3d_set_meteo(f3d_ctx *c, const double *vx, const double *vy, const double *
     TRY(rebuild_fields(c));
     c->meteo_ready = true;
     return F3D_OK;
+}
+
+int f3d_set_meteo_window(f3d_ctx *c, const f3d_ctx *region, int i0, int j0)
+{
+    if (!check_ctx(c) || !check_ctx(region)) return F3D_EINVAL;
+    if (!c->classes_set) {
+        set_error("f3d_set_meteo_window: f3d_set_classes must come first");
+        return F3D_ESTATE;
+    }
+    if (!region->meteo_ready) {
+        set_error("f3d_set_meteo_window: the region has no meteo");
+        return F3D_ESTATE;
+    }
+    const Grid &g = c->g, &r = region->g;
+    // the region's wind and density are stored after the Hm1 scaling of
+    // meteo.f90:108-112, which is the identity only for a uniform Hm1 = 1
+    if (!region->uniform_hm1 || !c->uniform_hm1 || region->hm1[0] != 1.0) {
+        set_error("f3d_set_meteo_window: needs Hm1 = 1 (UTM grids)");
+        return F3D_EINVAL;
+    }
+    if (i0 < 0 || j0 < 0 || i0 + g.nx > r.nx || j0 + g.ny > r.ny || g.nz > r.nz ||
+        g.dx != r.dx || g.dy != r.dy) {
+        set_error("f3d_set_meteo_window: window %d+%d x %d+%d x %d does not fit the region %dx%dx%d",
+                  i0, g.nx, j0, g.ny, g.nz, r.nx, r.ny, r.nz);
+        return F3D_EINVAL;
+    }
+    for (int k = 0; k < g.nz; ++k)
+        if (c->zlayer[k] != region->zlayer[k]) {
+            set_error("f3d_set_meteo_window: levels differ from the region's");
+            return F3D_EINVAL;
+        }
+    hipStream_t s = c->stream;
+    // the region's own stream may still be writing its fields
+    F3D_HIP(hipStreamSynchronize(region->stream));
+    auto win3 = [&](DeviceBuffer &d, const DeviceBuffer &src, int nz) {
+        return window_copy(s, d.as<double>(), src.as<double>(), g.nx, g.ny, nz, r.nx, r.ny, i0, j0);
+    };
+    TRY(win3(c->d_vx, region->d_vx, g.nz));
+    TRY(win3(c->d_vy, region->d_vy, g.nz));
+    TRY(win3(c->d_vz, region->d_vz, g.nz));
+    TRY(win3(c->d_tempe, region->d_tempe, g.nz));
+    TRY(win3(c->d_vptemp, region->d_vptemp, g.nz));
+    TRY(win3(c->d_rho, region->d_rho, g.nz));
+    TRY(win3(c->d_ustar, region->d_ustar, 1));
+    TRY(win3(c->d_zi, region->d_zi, 1));
+    TRY(win3(c->d_rl, region->d_rl, 1));
+    // dry deposition velocities are per class: taken when the class sets
+    // agree in size, else zero
+    if (g.nc == r.nc)
+        TRY(win3(c->d_vdep, region->d_vdep, g.nc));
+    else
+        F3D_HIP(hipMemsetAsync(c->d_vdep.p, 0, sizeof(double) * g.NXY * g.nc, s));
+    return meteo_stage(c);
 }
 
 int f3d_get_dt(f3d_ctx *c, double *dt, double *dtc)

@@ -29,6 +29,9 @@ int class_constants(int modv, double diam, double rhop, double sphe, met::ClassC
 Grid make_grid(int nx, int ny, int nz, int nc, double dx, double dy);
 void vertical_coefs(int nz, const double *zlayer, std::vector<double> &dz,
                     std::vector<ZCoef> &zc, std::vector<double> &ldz);
+// dst(nx,ny,nz) = the window (i0+1.., j0+1.., 1..) of src(snx,sny,*)
+int window_copy(hipStream_t s, double *dst, const double *src, int nx, int ny, int nz, int snx,
+                int sny, int i0, int j0);
 
 // ---------------- FP32 fused path (f3d_fused32.hip) ----------------
 namespace f32 {
@@ -108,19 +111,25 @@ bool flux_supported(TileShape ts);
 int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts);
 // Batched form of the flux step: every (scenario, tile, slab group) of a set
 // of FP32 contexts as one work list, drawn by persistent workgroups.
+// Scenarios fall in two parity groups (group[k]): in a step one group takes
+// an odd and the other an even step, so contexts on different iiter
+// parities share the batch.
 struct FluxBatchPlan {
     TileShape ts{58, 18, 0, 2};
     DeviceBuffer par[2];   // kernel parameters per scenario, even / odd iiter
-    DeviceBuffer items;    // interior items, then edge items (int4 each)
-    DeviceBuffer counter;  // work counters of the two lists
-    int nint = 0, nedge = 0;
+    DeviceBuffer items;    // per group: interior items, then edge items (int4 each)
+    DeviceBuffer counter;  // work counters of the four lists
+    int nint[2] = {0, 0}, nedge[2] = {0, 0};
+    int off_int[2] = {0, 0}, off_edge[2] = {0, 0};
     int grid_int = 0, grid_edge = 0;  // persistent grid sizes (resident capacity)
 };
 int flux_batch_plan(FluxBatchPlan &b, const std::vector<StepArgs> &even,
-                    const std::vector<StepArgs> &odd, const std::vector<Fields> &f, TileShape ts);
-// zero the counters, interior list on s, edge list on side (fork/join)
+                    const std::vector<StepArgs> &odd, const std::vector<Fields> &f,
+                    const std::vector<int> &group, TileShape ts);
+// zero the counters; group 0 steps with parity odd0, group 1 with the other;
+// interior lists on s, edge lists on side (fork/join)
 int flux_batch_step(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join,
-                    const FluxBatchPlan &b, bool odd);
+                    const FluxBatchPlan &b, bool odd0);
 // Ghost frame of c_out (setbcc values of c_in, clipped) and cmass2d
 // outflux; outmass[m] += member m's outflux (deterministic per member).
 int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,

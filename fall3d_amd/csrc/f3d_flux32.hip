@@ -203,6 +203,10 @@ struct FluxTile {
     float nc2[MC][NS], nwz[MC][NS];
     float b0[MC][NS], q0[MC][NS], gprev[MC][NS];
     float dc[MC][NS], qc[MC][NS], P[MC][NS], phi[MC][NS];
+    // edge rows (EY): the y fluxes with W2 clamped to the line, formed in
+    // phase 2 (phase 3 must not read the SP plane: the next level's phase 1
+    // may already be rewriting it)
+    float apl[MC][NS], apr[MC][NS];
 
     __device__ __forceinline__ FluxTile(const FluxParams &pp, const int *qv, int gi0, int gj0,
                                         float *sm, const ZLev *zt_)
@@ -307,6 +311,7 @@ struct FluxTile {
                 b0[c][sl] = q0[c][sl] = 0.f;
                 dc[c][sl] = qc[c][sl] = 0.f;
                 P[c][sl] = phi[c][sl] = 0.f;
+                apl[c][sl] = apr[c][sl] = 0.f;
             }
         }
     }
@@ -385,6 +390,12 @@ struct FluxTile {
                     const float Pp = (sl < NS - 1) ? P[c][sl + 1] : *row(SP, c, lj[sl] + 1);
                     const float w2 = AY[sl].y * minmod(Pp - P[c][sl], P[c][sl] - Pm);
                     phi[c][sl] = fmaf(AY[sl].x, P[c][sl], w2);
+                    if (EY) {
+                        // advcty.f90:49-51: W2 clamps to the line at j = 1 / ny,
+                        // A at the ghost row is v(1) / v(ny)
+                        apl[c][sl] = fmaf(AY[sl].x, Pm, w2);
+                        apr[c][sl] = fmaf(AY[sl].x, Pp, w2);
+                    }
                     if (sl == 0 || sl == NS - 1) *row(SF, c, lj[sl]) = phi[c][sl];
                 }
             }
@@ -451,13 +462,8 @@ struct FluxTile {
                     float pl = (sl > 0) ? phi[c][sl - 1] : *row(SF, c, lj[sl] - 1);
                     float pr = (sl < NS - 1) ? phi[c][sl + 1] : *row(SF, c, lj[sl] + 1);
                     if (EY) {
-                        // advcty.f90:49-51: W2 clamps to the line at j = 1 / ny,
-                        // A at the ghost row is v(1) / v(ny)
-                        const float Pm = (sl > 0) ? P[c][sl - 1] : *row(SP, c, lj[sl] - 1);
-                        const float Pp = (sl < NS - 1) ? P[c][sl + 1] : *row(SP, c, lj[sl] + 1);
-                        const float w2 = ph - ay * Pc;
-                        pl = has(sl, F_YLO) ? fmaf(ay, Pm, w2) : pl;
-                        pr = has(sl, F_YHI) ? fmaf(ay, Pp, w2) : pr;
+                        pl = has(sl, F_YLO) ? apl[c][sl] : pl;
+                        pr = has(sl, F_YHI) ? apr[c][sl] : pr;
                     }
                     bnew = (ay > 0.f) ? Pc + (pl - ph) : Pc + (ph - pr);
                     if (EDGE) bnew = inner(sl) ? bnew : Pc;
@@ -827,31 +833,36 @@ int batch_grid(int &grid)
     return F3D_OK;
 }
 
+// parity group gr steps with parity odd: its edge list on se, interior list
+// on s
 template <int RY, int NW, int MC>
-int batch_launch(hipStream_t s, hipStream_t se, const FluxBatchPlan &b, bool odd)
+int batch_launch(hipStream_t s, hipStream_t se, const FluxBatchPlan &b, int gr, bool odd)
 {
     constexpr int MCE = (NW * 64 > 768) ? 1 : MC;
     const FluxParams *par = b.par[odd ? 1 : 0].as<FluxParams>();
     const int4 *items = b.items.as<int4>();
-    unsigned *ctr = b.counter.as<unsigned>();
-    if (b.nedge > 0) {
-        const unsigned g = (unsigned)std::min(b.grid_edge, b.nedge);
+    unsigned *ctr = b.counter.as<unsigned>() + 2 * gr;
+    const int ne = b.nedge[gr], ni = b.nint[gr];
+    if (ne > 0) {
+        const unsigned g = (unsigned)std::min(b.grid_edge, ne);
+        const int4 *it = items + b.off_edge[gr];
         if (odd)
             hipLaunchKernelGGL((k_flux_batch<RY, NW, MCE, true, true>), dim3(g), dim3(NW * 64), 0, se,
-                               par, items + b.nint, b.nedge, ctr + 1);
+                               par, it, ne, ctr + 1);
         else
             hipLaunchKernelGGL((k_flux_batch<RY, NW, MCE, false, true>), dim3(g), dim3(NW * 64), 0, se,
-                               par, items + b.nint, b.nedge, ctr + 1);
+                               par, it, ne, ctr + 1);
         F3D_CHECK_LAUNCH();
     }
-    if (b.nint > 0) {
-        const unsigned g = (unsigned)std::min(b.grid_int, b.nint);
+    if (ni > 0) {
+        const unsigned g = (unsigned)std::min(b.grid_int, ni);
+        const int4 *it = items + b.off_int[gr];
         if (odd)
             hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, true, false>), dim3(g), dim3(NW * 64), 0, s,
-                               par, items, b.nint, ctr);
+                               par, it, ni, ctr);
         else
             hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, false, false>), dim3(g), dim3(NW * 64), 0, s,
-                               par, items, b.nint, ctr);
+                               par, it, ni, ctr);
         F3D_CHECK_LAUNCH();
     }
     return F3D_OK;
@@ -877,32 +888,40 @@ void scenario_items(int sc, const Grid &g, int RY, int mc, int mce, std::vector<
 
 template <int RY, int NW, int MC>
 int batch_plan_t(FluxBatchPlan &b, const std::vector<StepArgs> &ae, const std::vector<StepArgs> &ao,
-                 const std::vector<Fields> &f)
+                 const std::vector<Fields> &f, const std::vector<int> &group)
 {
     constexpr int MCE = (NW * 64 > 768) ? 1 : MC;
     const size_t n = ae.size();
     std::vector<FluxParams> pe(n), po(n);
-    std::vector<int4> iint, iedge;
+    std::vector<int4> iint[2], iedge[2];
     for (size_t k = 0; k < n; ++k) {
         TRY(make_params(ae[k], f[k], pe[k]));
         TRY(make_params(ao[k], f[k], po[k]));
-        scenario_items((int)k, ae[k].g, RY, MC, MCE, iint, iedge);
+        const int gr = group[k] ? 1 : 0;
+        scenario_items((int)k, ae[k].g, RY, MC, MCE, iint[gr], iedge[gr]);
     }
-    // deeper columns first (cost per item grows with nz)
+    // deeper columns first (cost per item grows with nz); lists laid out as
+    // [group 0 interior | group 0 edge | group 1 interior | group 1 edge]
     auto deeper = [&](const int4 &x, const int4 &y) { return ae[x.x].g.nz > ae[y.x].g.nz; };
-    std::stable_sort(iint.begin(), iint.end(), deeper);
-    std::stable_sort(iedge.begin(), iedge.end(), deeper);
-    b.nint = (int)iint.size();
-    b.nedge = (int)iedge.size();
-    iint.insert(iint.end(), iedge.begin(), iedge.end());
+    std::vector<int4> all;
+    for (int gr = 0; gr < 2; ++gr) {
+        std::stable_sort(iint[gr].begin(), iint[gr].end(), deeper);
+        std::stable_sort(iedge[gr].begin(), iedge[gr].end(), deeper);
+        b.nint[gr] = (int)iint[gr].size();
+        b.nedge[gr] = (int)iedge[gr].size();
+        b.off_int[gr] = (int)all.size();
+        all.insert(all.end(), iint[gr].begin(), iint[gr].end());
+        b.off_edge[gr] = (int)all.size();
+        all.insert(all.end(), iedge[gr].begin(), iedge[gr].end());
+    }
     if (b.par[0].alloc(sizeof(FluxParams) * n, nullptr) || b.par[1].alloc(sizeof(FluxParams) * n, nullptr) ||
-        b.items.alloc(sizeof(int4) * std::max<size_t>(1, iint.size()), nullptr) ||
-        b.counter.alloc(2 * sizeof(unsigned), nullptr))
+        b.items.alloc(sizeof(int4) * std::max<size_t>(1, all.size()), nullptr) ||
+        b.counter.alloc(4 * sizeof(unsigned), nullptr))
         return F3D_ENOMEM;
     F3D_HIP(hipMemcpy(b.par[0].p, pe.data(), sizeof(FluxParams) * n, hipMemcpyHostToDevice));
     F3D_HIP(hipMemcpy(b.par[1].p, po.data(), sizeof(FluxParams) * n, hipMemcpyHostToDevice));
-    if (!iint.empty())
-        F3D_HIP(hipMemcpy(b.items.p, iint.data(), sizeof(int4) * iint.size(), hipMemcpyHostToDevice));
+    if (!all.empty())
+        F3D_HIP(hipMemcpy(b.items.p, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice));
     TRY((batch_grid<RY, NW, MC, false>(b.grid_int)));
     TRY((batch_grid<RY, NW, MCE, true>(b.grid_edge)));
     return F3D_OK;
@@ -911,9 +930,11 @@ int batch_plan_t(FluxBatchPlan &b, const std::vector<StepArgs> &ae, const std::v
 }  // namespace
 
 int flux_batch_plan(FluxBatchPlan &b, const std::vector<StepArgs> &even,
-                    const std::vector<StepArgs> &odd, const std::vector<Fields> &f, TileShape ts)
+                    const std::vector<StepArgs> &odd, const std::vector<Fields> &f,
+                    const std::vector<int> &group, TileShape ts)
 {
-    if (even.empty() || even.size() != odd.size() || even.size() != f.size()) {
+    if (even.empty() || even.size() != odd.size() || even.size() != f.size() ||
+        group.size() != f.size()) {
         set_error("batch plan: inconsistent scenario lists");
         return F3D_EINVAL;
     }
@@ -923,17 +944,18 @@ int flux_batch_plan(FluxBatchPlan &b, const std::vector<StepArgs> &even,
     }
     b.ts = ts;
     const int mc = ts.mc ? ts.mc : 1;
-    if (ts.ty == 10) return mc == 1 ? batch_plan_t<16, 8, 1>(b, even, odd, f) : batch_plan_t<16, 8, 2>(b, even, odd, f);
-    if (ts.ty == 18) return mc == 1 ? batch_plan_t<24, 12, 1>(b, even, odd, f) : batch_plan_t<24, 12, 2>(b, even, odd, f);
-    return mc == 1 ? batch_plan_t<32, 16, 1>(b, even, odd, f) : batch_plan_t<32, 16, 2>(b, even, odd, f);
+    if (ts.ty == 10) return mc == 1 ? batch_plan_t<16, 8, 1>(b, even, odd, f, group) : batch_plan_t<16, 8, 2>(b, even, odd, f, group);
+    if (ts.ty == 18) return mc == 1 ? batch_plan_t<24, 12, 1>(b, even, odd, f, group) : batch_plan_t<24, 12, 2>(b, even, odd, f, group);
+    return mc == 1 ? batch_plan_t<32, 16, 1>(b, even, odd, f, group) : batch_plan_t<32, 16, 2>(b, even, odd, f, group);
 }
 
 int flux_batch_step(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join,
-                    const FluxBatchPlan &b, bool odd)
+                    const FluxBatchPlan &b, bool odd0)
 {
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(2), 0, s, b.counter.as<unsigned>());
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(4), 0, s, b.counter.as<unsigned>());
     F3D_CHECK_LAUNCH();
-    const bool split = side && fork && join && b.nint > 0 && b.nedge > 0;
+    const int ne = b.nedge[0] + b.nedge[1], ni = b.nint[0] + b.nint[1];
+    const bool split = side && fork && join && ni > 0 && ne > 0;
     hipStream_t se = split ? side : s;
     if (split) {
         F3D_HIP(hipEventRecord(fork, s));
@@ -941,14 +963,17 @@ int flux_batch_step(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t
     }
     const TileShape ts = b.ts;
     const int mc = ts.mc ? ts.mc : 1;
-    int rc;
-    if (ts.ty == 10)
-        rc = mc == 1 ? batch_launch<16, 8, 1>(s, se, b, odd) : batch_launch<16, 8, 2>(s, se, b, odd);
-    else if (ts.ty == 18)
-        rc = mc == 1 ? batch_launch<24, 12, 1>(s, se, b, odd) : batch_launch<24, 12, 2>(s, se, b, odd);
-    else
-        rc = mc == 1 ? batch_launch<32, 16, 1>(s, se, b, odd) : batch_launch<32, 16, 2>(s, se, b, odd);
-    if (rc) return rc;
+    for (int gr = 0; gr < 2; ++gr) {
+        const bool odd = gr == 0 ? odd0 : !odd0;
+        int rc;
+        if (ts.ty == 10)
+            rc = mc == 1 ? batch_launch<16, 8, 1>(s, se, b, gr, odd) : batch_launch<16, 8, 2>(s, se, b, gr, odd);
+        else if (ts.ty == 18)
+            rc = mc == 1 ? batch_launch<24, 12, 1>(s, se, b, gr, odd) : batch_launch<24, 12, 2>(s, se, b, gr, odd);
+        else
+            rc = mc == 1 ? batch_launch<32, 16, 1>(s, se, b, gr, odd) : batch_launch<32, 16, 2>(s, se, b, gr, odd);
+        if (rc) return rc;
+    }
     if (split) {
         F3D_HIP(hipEventRecord(join, se));
         F3D_HIP(hipStreamWaitEvent(s, join, 0));

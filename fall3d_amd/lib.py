@@ -33,7 +33,8 @@ CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time", "f3d_device_bytes",
                "f3d_last_error", "f3d_hbm_copy", "f3d_batch_create", "f3d_batch_destroy",
                "f3d_batch_set_tile", "f3d_batch_get_tile", "f3d_batch_use_graph",
-               "f3d_batch_step", "f3d_batch_sync", "f3d_batch_autotune", "f3d_autotune"]
+               "f3d_batch_step", "f3d_batch_sync", "f3d_batch_autotune", "f3d_autotune",
+               "f3d_set_meteo_window", "f3d_mem_info"]
 
 
 class F3DError(RuntimeError):
@@ -66,6 +67,10 @@ def lib():
         L.f3d_set_classes.argtypes = [vp, dp, dp, dp, i]
         L.f3d_set_models.argtypes = [vp, i, i, d, d, d]
         L.f3d_set_meteo.argtypes = [vp] + [dp] * 10
+        L.f3d_set_meteo_window.argtypes = [vp, vp, i, i]
+        L.f3d_mem_info.argtypes = [ctypes.POINTER(ctypes.c_longlong)] * 2
+        L.f3d_mem_info.restype = c
+        L.f3d_set_meteo_window.restype = c
         L.f3d_get_dt.argtypes = [vp, dp, dp]
         L.f3d_set_dt.argtypes = [vp, d]
         L.f3d_set_source.argtypes = [vp, i, ip, ip, ip, dp, d]
@@ -118,6 +123,13 @@ def hbm_copy(nbytes=1 << 30, reps=5):
     ms = ctypes.c_double()
     _check(lib().f3d_hbm_copy(int(nbytes), int(reps), ctypes.byref(ms)), "f3d_hbm_copy")
     return ms.value
+
+
+def mem_info():
+    """(free, total) device memory in bytes (f3d_mem_info)."""
+    fr, tot = ctypes.c_longlong(), ctypes.c_longlong()
+    _check(lib().f3d_mem_info(ctypes.byref(fr), ctypes.byref(tot)), "f3d_mem_info")
+    return fr.value, tot.value
 
 
 def last_error():
@@ -188,6 +200,13 @@ class Context:
         _check(lib().f3d_set_meteo(self.h, *[_P(a) for a in arrs]), "f3d_set_meteo")
         return self.dt
 
+    def set_meteo_window(self, region, i0, j0):
+        """Meteorology from the window (i0+1.., j0+1.., 1..nz) of a region
+        context's fields in HBM (f3d_set_meteo_window)."""
+        _check(lib().f3d_set_meteo_window(self.h, region.h, int(i0), int(j0)),
+               "f3d_set_meteo_window")
+        return self.dt
+
     @property
     def dt(self):
         dt, dtc = ctypes.c_double(), ctypes.c_double()
@@ -234,6 +253,13 @@ class Context:
         _check(lib().f3d_get_member_state(self.h, int(member), _P(c), _P(cdry)),
                "f3d_get_member_state")
         return c, cdry
+
+    def ground_load(self, member=0):
+        """Deposit per class and its total, cdry(nx,ny,0:nc) (no state copy)."""
+        cdry = np.zeros((self.nx, self.ny, self.nc + 1), order="F")
+        _check(lib().f3d_get_member_state(self.h, int(member), None, _P(cdry)),
+               "f3d_get_member_state")
+        return cdry
 
     def ensemble_budget(self):
         out = np.zeros(6 * self.members)

@@ -116,6 +116,13 @@ int f3d_set_models(f3d_ctx *ctx, int modkv, int modkh, double rkv0, double rkh0,
 int f3d_set_meteo(f3d_ctx *ctx, const double *vx, const double *vy, const double *vz,
                   const double *tempe, const double *vptemp, const double *rho,
                   const double *ustar, const double *zi, const double *rl, const double *vdep);
+/* The same from meteorology already in HBM: the window (i0+1..i0+nx,
+ * j0+1..j0+ny, 1..nz) of a larger context's fields (same dx, dy and
+ * levels; Hm1 = 1).  No counterpart in the reference: it lets many
+ * forecasts on one resident regional meteorology start without a host copy.
+ * Deposition velocities come from the region when the class counts agree,
+ * else they are zero. */
+int f3d_set_meteo_window(f3d_ctx *ctx, const f3d_ctx *region, int i0, int j0);
 int f3d_get_dt(f3d_ctx *ctx, double *dt, double *dtc);
 /* Override dt (the reference keeps dt until the next meteo read). */
 int f3d_set_dt(f3d_ctx *ctx, double dt);
@@ -188,8 +195,8 @@ const char *f3d_last_error(void);
  * captured in a HIP graph and replayed.  Each context keeps its own clock,
  * sources and budgets (f3d_budget, f3d_get_state work on it as usual); a
  * context's result in a batch equals the one it gets from f3d_step with the
- * same tile shape.  The contexts must stay alive while the batch exists and
- * be on the same iiter parity when stepped. */
+ * same tile shape.  The contexts must stay alive while the batch exists;
+ * they may be on different iiter parities (each keeps its own sweep order). */
 typedef struct f3d_batch f3d_batch;
 int f3d_batch_create(f3d_batch **batch, f3d_ctx *const *ctxs, int n);
 void f3d_batch_destroy(f3d_batch *batch);
@@ -207,6 +214,10 @@ int f3d_batch_autotune(f3d_batch *batch, int trials, int *tx, int *ty, int *slab
                        double *best_ms);
 /* the same for one context and its f3d_step (column-tile shapes included) */
 int f3d_autotune(f3d_ctx *ctx, int trials, int *tx, int *ty, int *slabs, double *best_ms);
+
+/* Free and total device memory (admission control of the serving loop).  No
+ * counterpart in the reference. */
+int f3d_mem_info(long long *free_bytes, long long *total_bytes);
 
 /* No counterpart in the reference: a plain device copy of nbytes (read) +
  * nbytes (write), repeated reps times; *avg_ms is the average launch time.

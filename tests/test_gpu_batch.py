@@ -159,14 +159,25 @@ def test_autotune_keeps_state_and_picks_a_candidate():
         alone.close()
 
 
-def test_batch_rejects_mixed_parity():
-    scs = _mixture(2, seed=5)
+def test_batch_mixed_parities_equal_alone():
+    # contexts that joined at different times sit on different iiter
+    # parities; each keeps its own sweep order in the batch
+    scs = _mixture(3, seed=5)
     runs = _runs(scs, None)
     try:
         runs[0].step(1)
+        runs[2].step(3)
         b = F.Batch([r.ctx for r in runs])
-        with pytest.raises(F.F3DError):
-            b.step(1)
+        b.step(5)
+        b.sync()
+        got = [r.state()[0] for r in runs]
         b.close()
     finally:
         _close(runs)
+    for k, n in ((0, 6), (1, 5), (2, 8)):
+        alone = _runs([scs[k]], None)[0]
+        try:
+            alone.step(n)
+            assert np.array_equal(alone.state()[0], got[k]), k
+        finally:
+            alone.close()

@@ -211,19 +211,25 @@ def test_fp32_tile_edges_partial_tiles(tile):
     assert m["c_l1"] <= 1e-4 and m["load_l1"] <= 1e-4, m
 
 
-def test_fp32_deterministic():
-    sc = S.config_small(nx=33, ny=29, nz=9, nc=3, seed=9, mean_u=5.0)
+@pytest.mark.parametrize("tile", [(58, 18, 0, 2), (58, 10, 0, 2), (58, 26, 0, 1)],
+                         ids=lambda t: f"{t[0]}x{t[1]}m{t[3]}")
+def test_fp32_deterministic(tile):
+    # grids where every tile is an edge tile in x and y, many steps: repeated
+    # runs are bit-identical (regression test for a cross-phase LDS hazard
+    # on the y-edge rows)
+    sc = S.config_small(nx=41, ny=37, nz=9, nc=3, seed=9, mean_u=5.0, mean_v=-4.0)
     outs = []
-    for _ in range(2):
-        r = F.Run(sc, precision=F.FP32)
+    for _ in range(3):
+        r = F.Run(sc, precision=F.FP32, tile=tile)
         r.meteo_update()
-        r.step(15)
+        r.step(40)
         c, cdry = r.state()
         outs.append((c, cdry, r.budget()))
         r.close()
-    assert np.array_equal(outs[0][0], outs[1][0])
-    assert np.array_equal(outs[0][1], outs[1][1])
-    assert outs[0][2] == outs[1][2]
+    for o in outs[1:]:
+        assert np.array_equal(outs[0][0], o[0])
+        assert np.array_equal(outs[0][1], o[1])
+        assert outs[0][2] == o[2]
 
 
 def test_settling_chain_matches_serial_oracle():
