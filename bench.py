@@ -7,11 +7,20 @@ setsrc + sizloop (advection, diffusion, boundaries, deposit, clip, outflux).
 
 value     = cell*class updates per second of the FP32 fused path (inputs
             resident in HBM, meteo stage excluded and reported separately)
-roofline  = the fused step kernel's algorithmic HBM bytes per launch over its
-            HIP-event average launch time, against 8 TB/s
+roofline  = the fused step kernels' algorithmic HBM bytes per step over their
+            HIP-event average time, against 8 TB/s
 fp64_plain= the routine-by-routine FP64 drop-in on the same workload
 cpu_baseline = the oracle's FP64 C restatement (class-parallel threads) on
             this host, on a bounded number of steps of the same workload
+ensemble  = 64 members x 200x200x30 x 16 classes on one meteorology, one
+            fused launch per step
+uneven_batch = scenarios of 80..320 squared x 16..48 levels x 4..24 classes,
+            each with its own meteorology, source and dt: the persistent
+            batched step with HIP-graph replay and the autotuned tile, beside
+            per-scenario f3d_step launches
+serving   = Poisson arrivals of 6 h forecasts on windows of three resident
+            400x400x40 regions, continuous batching: p50/p99 latency,
+            throughput, error against the FP64 drop-in on a sampled request
 """
 from __future__ import annotations
 
@@ -52,6 +61,14 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--tile", default="58x18")
+    ap.add_argument("--skip-ensemble", action="store_true")
+    ap.add_argument("--ensemble-members", type=int, default=64)
+    ap.add_argument("--skip-uneven", action="store_true")
+    ap.add_argument("--uneven-n", type=int, default=128)
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=30.0,
+                    help="arrival window of the serving run (2 req/s)")
+    ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--slabs", type=int, default=2,
                     help="member*class slabs per workgroup of the 58-column kernel")
     args = ap.parse_args()
@@ -107,6 +124,7 @@ def main():
 
     # ---------------- FP64 plain drop-in ----------------
     fp64 = None
+    coef = None
     if not args.skip_fp64:
         r64 = F.Run(sc, precision=F.FP64)
         r64.meteo_update()
@@ -181,7 +199,163 @@ def main():
         "hbm_peak_frac_wall": value * b_alg / 1e9 / HBM_PEAK_GBS,
         "hbm_copy_gbs": copy_gbs,
     }
+    if not args.skip_ensemble:
+        out["ensemble"] = bench_ensemble(args, fp64)
+    if not args.skip_uneven:
+        out["uneven_batch"] = bench_uneven(args, fp64)
+    if not args.skip_serve:
+        out["serving"] = bench_serving(args, fp64)
     print(json.dumps(out), flush=True)
+
+
+def _speedup(v, fp64):
+    return (v / fp64) if fp64 else None
+
+
+def bench_ensemble(args, fp64, nsteps=10):
+    """64 members on one 200x200x30 grid, 16 classes: one fused launch per
+    step for all members (FP32 state, FP64 deposits and budgets)."""
+    from fall3d_amd import lib as F
+    from fall3d_amd import scenario as S
+    base = S.config_c2(nx=200, ny=200, nz=30, nc=16, seed=0xF3D0003)
+    mem = [S.config_ensemble_member(base, 5000 + k) for k in range(args.ensemble_members)]
+    ens = F.Ensemble(base, mem)
+    try:
+        ens.meteo_update()
+        ens.step(2)
+        ens.ctx.sync()
+        ens.ctx.timing(True)
+        ens.ctx.kernel_time(reset=True)
+        t0 = time.perf_counter()
+        ens.step(nsteps)
+        ens.ctx.sync()
+        el = time.perf_counter() - t0
+        kms, _ = ens.ctx.kernel_time()
+    finally:
+        ens.close()
+    upd = base.nx * base.ny * base.nz * base.nc * len(mem)
+    value = upd * nsteps / el
+    b_alg = alg_bytes_per_update(base.nc, n_share=len(mem))
+    ach = b_alg * upd / (kms * 1e-3) / 1e9
+    log(f"ensemble: {len(mem)} members, {el * 1e3 / nsteps:.2f} ms/step, {value:.3e} updates/s")
+    return {"workload": f"{len(mem)} members x {base.nx}x{base.ny}x{base.nz} x {base.nc} classes",
+            "updates_per_s": value, "ms_per_step": el * 1e3 / nsteps, "kernel_avg_ms": kms,
+            "bytes_per_update": b_alg, "achieved_gbs": ach, "hbm_frac": ach / HBM_PEAK_GBS,
+            "speedup_vs_fp64_plain": _speedup(value, fp64)}
+
+
+def _uneven_scenario(seed):
+    from fall3d_amd import scenario as S
+    return S.config_uneven(seed)
+
+
+def bench_uneven(args, fp64, nsteps=10):
+    """Uneven batch: scenarios of their own size, class count, meteorology,
+    source and dt; persistent batched step (autotuned tile, HIP graphs)
+    against per-scenario f3d_step launches of the same kernels."""
+    import concurrent.futures as cf
+    from fall3d_amd import lib as F
+    t0 = time.time()
+    nproc = max(1, min(16, len(os.sched_getaffinity(0))))
+    with cf.ProcessPoolExecutor(nproc) as ex:
+        scs = list(ex.map(_uneven_scenario, range(1000, 1000 + args.uneven_n)))
+    log(f"uneven: {len(scs)} scenarios built in {time.time() - t0:.1f}s")
+    runs = []
+    try:
+        for sc in scs:
+            r = F.Run(sc, precision=F.FP32)
+            r.meteo_update()
+            runs.append(r)
+        upd = sum(sc.updates_per_step for sc in scs)
+        # per-scenario launches (f3d_step on each context in turn)
+        for r in runs:
+            r.step(1)
+        for r in runs:
+            r.ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            for r in runs:
+                r.step(1)
+        for r in runs:
+            r.ctx.sync()
+        el_plain = time.perf_counter() - t0
+        # batched: autotune, then graph-replayed persistent steps
+        b = F.Batch([r.ctx for r in runs])
+        tile, tune_ms = b.autotune(trials=2)
+        b.step(2)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(nsteps)
+        b.sync()
+        el_batch = time.perf_counter() - t0
+        b.close()
+    finally:
+        for r in runs:
+            r.close()
+    v_batch, v_plain = upd * nsteps / el_batch, upd * nsteps / el_plain
+    log(f"uneven: batched {el_batch * 1e3 / nsteps:.2f} ms/step ({v_batch:.3e} updates/s), "
+        f"per-scenario {el_plain * 1e3 / nsteps:.2f} ms/step ({v_plain:.3e}), tile {tile}")
+    return {"workload": f"{len(scs)} scenarios, nx=ny 80..320, nz 16..48, nc 4..24",
+            "updates_per_step": upd, "batched_updates_per_s": v_batch,
+            "per_scenario_updates_per_s": v_plain, "batched_ms_per_step": el_batch * 1e3 / nsteps,
+            "per_scenario_ms_per_step": el_plain * 1e3 / nsteps,
+            "autotuned_tile": "%dx%d slabs %d" % tile, "autotune_best_flux_ms": tune_ms,
+            "speedup_vs_fp64_plain": _speedup(v_batch, fp64)}
+
+
+def bench_serving(args, fp64):
+    """Continuous-batching serving of 6 h forecasts on windows of three
+    resident 400x400x40 regions; latency from arrival to final ground load."""
+    import copy
+
+    import numpy as np
+
+    from fall3d_amd import lib as F
+    from fall3d_amd import scenario as S
+    from fall3d_amd import serve
+    t0 = time.time()
+    regions = [serve.Region(S.config_c2(nx=400, ny=400, nz=40, nc=16, seed=0xF3D0500 + k))
+               for k in range(3)]
+    rng = np.random.default_rng(0xF3D0005)
+    times = serve.poisson_arrivals(args.serve_rate, args.serve_seconds, rng)
+    reqs = [serve.make_request(k, t, regions, rng) for k, t in enumerate(times)]
+    sample = copy.deepcopy(reqs[len(reqs) // 2]) if reqs else None
+    log(f"serving: {len(reqs)} requests over {args.serve_seconds:.0f}s, setup {time.time() - t0:.1f}s")
+    srv = serve.Server(regions)
+    done = srv.run(reqs, log=None)
+    st = serve.latency_stats(done)
+    thr = srv.updates / srv.wall_s
+    log(f"serving: p50 {st['p50']:.2f}s p99 {st['p99']:.2f}s, {thr:.3e} updates/s over "
+        f"{srv.wall_s:.1f}s (GPU busy {srv.busy_s:.1f}s)")
+    err = None
+    if sample is not None:
+        # the sampled request again, stepped by the FP64 drop-in
+        got = next(r for r in done if r.rid == sample.rid)
+        reg = regions[sample.region]
+        c64 = F.Context(sample.nx, sample.ny, sample.nz, sample.nc, reg.sc.dx, reg.sc.dy,
+                        reg.sc.zlayer[:sample.nz], F.FP64)
+        try:
+            c64.set_classes(sample.diam, sample.rhop, sample.sphe, reg.sc.modv)
+            c64.set_models(reg.sc.modkv, reg.sc.modkh, reg.sc.rkv0, reg.sc.rkh0, reg.sc.csh)
+            c64.set_meteo_window(reg.ctx, sample.i0, sample.j0)
+            c64.set_source(sample.isrc, sample.jsrc, sample.ksrc, sample.tmrat, sample.flowrate)
+            c64.step(got.nsteps)
+            ref = c64.ground_load()
+        finally:
+            c64.close()
+        a, b = got.cdry[:, :, 1:].sum(-1), ref[:, :, 1:].sum(-1)
+        err = {"request": sample.rid, "steps": got.nsteps,
+               "load_rel_l1": float(np.abs(a - b).sum() / np.abs(b).sum()),
+               "load_rel_linf": float(np.abs(a - b).max() / np.abs(b).max())}
+    return {"workload": f"Poisson {args.serve_rate} req/s over {args.serve_seconds:.0f}s, "
+                        f"6 h forecasts of 200x200x30 x 16 classes on windows of 3 resident "
+                        f"400x400x40 regions",
+            "requests": st["n"], "latency_p50_s": st["p50"], "latency_p99_s": st["p99"],
+            "latency_max_s": st["max"], "throughput_updates_per_s": thr,
+            "busy_updates_per_s": srv.updates / srv.busy_s if srv.busy_s else None,
+            "gpu_busy_frac": srv.busy_s / srv.wall_s, "wall_s": srv.wall_s,
+            "speedup_vs_fp64_plain": _speedup(srv.updates / srv.busy_s if srv.busy_s else 0, fp64),
+            "fp64_check": err}
 
 
 def measured_traffic(tile):

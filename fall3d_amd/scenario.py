@@ -239,15 +239,19 @@ def _fourier_wind(rng, nx, ny, zlayer, dx, dy, nmodes, rms_h, rms_v):
     z = np.asarray(zlayer)
 
     def field(rms):
-        f = np.zeros((nx, ny, len(z)), order="F")
-        for _ in range(nmodes):
+        X = np.empty((nmodes, nx))
+        Y = np.empty((nmodes, ny))
+        Z = np.empty((nmodes, len(z)))
+        for m in range(nmodes):
             kx, ky, kz = rng.integers(1, nmodes + 1, size=3)
             px, py, pz = rng.uniform(0, 2 * np.pi, size=3)
             a = rng.standard_normal()
-            fx = np.cos(2 * np.pi * kx * x / Lx + px)
-            fy = np.cos(2 * np.pi * ky * y / Ly + py)
-            fz = np.cos(np.pi * kz * z / H + pz)
-            f += a * fx[:, None, None] * fy[None, :, None] * fz[None, None, :]
+            X[m] = a * np.cos(2 * np.pi * kx * x / Lx + px)
+            Y[m] = np.cos(2 * np.pi * ky * y / Ly + py)
+            Z[m] = np.cos(np.pi * kz * z / H + pz)
+        # sum over modes of a*fx(x)*fy(y)*fz(z), as one contraction
+        XY = X[:, :, None] * Y[:, None, :]
+        f = np.asfortranarray(np.tensordot(XY, Z, axes=([0], [0])))
         r = math.sqrt(float(np.mean(f * f)))
         return f * (rms / r if r > 0 else 0.0)
 
