@@ -61,10 +61,10 @@ struct Fields {
     const double *vsum1;        // min(0, vset(:,:,1)+vdep) (nx,ny,nc), FP64 deposit factor
     // class-independent face coefficients (dt-dependent), see k_fields_face
     const float *ax, *fx, *ay, *fy, *dfx, *dfy;
-    // the same, interleaved per cell for the flux kernel: one 16-byte load
-    // per cell for each group
+    // the same, interleaved per cell for the flux kernel: one load per cell
+    // for each group
     const float4 *cA;  // (ax, fx, ay, fy)
-    const float4 *cD;  // (dfx, dfy, 1/rho, rho*rkver)
+    const float *cD;   // (dfx, 1/rho, rho*rkver): 12-byte records
     const float2 *cE;  // (rho*rkh1, rho*rkhor)
     const ZLev *zl;    // per level 0..nz+1
     const float *base;  // the buffer all FP32 fields live in, and its size

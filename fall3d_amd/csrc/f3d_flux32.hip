@@ -48,12 +48,14 @@ struct FluxParams {
     float *cout;
     double *cdry;
     // all FP32 fields live in one buffer; byte offsets of the groups in it:
-    // cA (ax, fx, ay, fy), cD (dfx, dfy, 1/rho, rho*rkver), cE (rho*rkh1,
-    // rho*rkhor: raw products for line ends), wz (vz+vset, class 0), bfac and
-    // tfac (class 0)
+    // cA (ax, fx, ay, fy), cD (dfx, 1/rho, rho*rkver: 12-byte records), dfy,
+    // cE (rho*rkh1, rho*rkhor: raw products for line ends), wz (vz+vset,
+    // class 0), bfac and tfac (class 0).  The kernel reads each record whole,
+    // once per level: a partial read of a cache line would fetch the line
+    // again for the rest.
     const float *fbase;
     unsigned fbytes;
-    unsigned oA, oD, oE, oW, oBF, oTF;
+    unsigned oA, oD, oY, oE, oW, oBF, oTF;
     const double *vsum1;
     const ZLev *zl;
     double dt64;
@@ -137,6 +139,12 @@ __device__ __forceinline__ float2 bld2(rsrc_t r, unsigned voff, unsigned soff)
     const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
     return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
 }
+typedef unsigned u32x3 __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ float3 bld3(rsrc_t r, unsigned voff, unsigned soff)
+{
+    const u32x3 v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)voff, (int)soff, 0);
+    return make_float3(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z));
+}
 __device__ __forceinline__ float4 bld4(rsrc_t r, unsigned voff, unsigned soff)
 {
     const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
@@ -161,6 +169,9 @@ template <int RY, int NW, int MC, bool ODD, bool EX, bool EY>
 struct FluxTile {
     static constexpr int NS = RY / NW;
     static constexpr bool EDGE = EX || EY;
+    // interior tiles of groups of at most 768 lanes (a budget of 168
+    // registers) run the steady levels in pairs and prefetch c a level early
+    static constexpr bool PIPE = !EDGE && NW * 64 <= 768;
     static constexpr int PL = RX * RY;  // one LDS plane
     static_assert(RY % NW == 0, "region rows must divide evenly over the waves");
 
@@ -181,19 +192,23 @@ struct FluxTile {
 
     // per-row geometry: byte offsets of the (clamped) column in c, in the
     // meteo arrays and of the rows above/below; edge flags
-    unsigned co4[NS], m4[NS], m16[NS], mup16[NS], mdn8[NS], mup8[NS], fl[NS];
+    unsigned co4[NS], m4[NS], m12[NS], m16[NS], mup4[NS], mdn8[NS], mup8[NS], fl[NS];
     int lj[NS];
-    // Class-independent coefficients of the current level: (ax, fx), (ay,
-    // fy), (dfx, dfy, lr, pz) and the latter of the level below.  Each
-    // register is reloaded with the next level's value right after its last
-    // use in this level, so a load is never copied while in flight (a copy
-    // would wait for it).  gv: wind at the face cell of a ghost (edge tiles),
-    // one level ahead.
-    float2 AX[NS], AY[NS];
-    float4 Dn[NS], D0[NS];
+    // Class-independent coefficients: (ax, fx, ay, fy) of the current and
+    // the next level, (dfx, lr, pz) of the current level and the one below.
+    // Every load fills a register after the last use of the value it
+    // replaces, every loaded lane is used (a dead lane is written back late
+    // and makes its register's next use wait), and each value is loaded a
+    // level before it is needed.  gv: wind at the face cell of a ghost (edge
+    // tiles), one level ahead.
+    float4 CA[NS], CAn[NS];
+    float3 Dn[NS], D0[NS];
     float gv[NS];
+    // dfy of the level in the last sweep (own rows, and the row above the
+    // band) and of the level after it
+    float dyl[NS], dyu[NS], dyl1[NS], dyu1[NS];
     // carried between phase 3 and the next phase 2 (class-independent)
-    float dyl[NS], dyu[NS], pyo[NS], pym[NS], pyp[NS], lrc[NS], dxc[NS], pxc[NS];
+    float pyo[NS], pym[NS], pyp[NS], lrc[NS], dxc[NS], pxc[NS];
     // per class: advctzc window c(s), c(s+1), w(s), U(s), W2(s), delta(s) (after
     // level nz, c0 holds the top ghost c(nz+1)), prefetched c(s+2), w(s+1);
     // diffz window b(k), q(k) = b/rho and the face flux it leaves for the
@@ -247,10 +262,11 @@ struct FluxTile {
             const unsigned mo = (unsigned)((ci - 1) + nx * (cj - 1));
             co4[sl] = 4u * (unsigned)cidx(nx, ny, ci, cj, 0);
             m4[sl] = 4u * mo;
+            m12[sl] = 12u * mo;
             m16[sl] = 16u * mo;
             const unsigned mu = (unsigned)((ci - 1) + nx * (min(cj + 1, ny) - 1));
             const unsigned md = (unsigned)((ci - 1) + nx * (max(cj - 1, 1) - 1));
-            mup16[sl] = 16u * mu;
+            mup4[sl] = 4u * mu;
             mup8[sl] = 8u * mu;
             mdn8[sl] = 8u * md;
             unsigned f = 0;
@@ -283,10 +299,11 @@ struct FluxTile {
         dtop = p.dt * zt[nz].ldz;
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
-            AX[sl] = bld2(rf, m16[sl], p.oA);  // level 1
-            AY[sl] = bld2(rf, m16[sl] + 8u, p.oA);
-            Dn[sl] = bld4(rf, m16[sl], p.oD);
-            D0[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
+            CA[sl] = bld4(rf, m16[sl], p.oA);  // level 1
+            CAn[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
+            Dn[sl] = bld3(rf, m12[sl], p.oD);
+            D0[sl] = make_float3(0.f, 0.f, 0.f);
+            dyl1[sl] = dyu1[sl] = 0.f;
             gv[sl] = EDGE ? bld(rf, m16[sl] + (has(sl, F_GX) ? 0u : 8u), p.oA) : 0.f;
             dyl[sl] = dyu[sl] = pyo[sl] = pym[sl] = pyp[sl] = 0.f;
             lrc[sl] = dxc[sl] = pxc[sl] = 0.f;
@@ -327,6 +344,18 @@ struct FluxTile {
         if (STEADY || s <= nzl) {
             const ZLev zn = zt[s + 1];
             const unsigned lz = 4u * NXY * (unsigned)s;  // byte offset of level s+1 in a 3-D field
+            // c(s+3) first: carrying it into the window copies the load, so
+            // it is issued a whole level before that copy
+            float nc3[MC][NS];
+            if (PIPE) {
+#pragma unroll
+                for (int sl = 0; sl < NS; ++sl)
+#pragma unroll
+                    for (int c = 0; c < MC; ++c)
+                        nc3[c][sl] = (STEADY || s + 3 <= nzl)
+                                         ? bld(rin[c], co4[sl], 4u * SZ * (unsigned)(s + 3)) : 0.f;
+                if (STEADY) __builtin_amdgcn_sched_barrier(0);
+            }
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 bool zero = false;
@@ -335,8 +364,6 @@ struct FluxTile {
                     // out of the domain through it, else the adjacent cell
                     const float v = gv[sl];
                     zero = has(sl, F_GLO) ? v >= 0.f : v <= 0.f;
-                    if (STEADY || s + 1 <= nzl)
-                        gv[sl] = bld(rf, m16[sl] + (has(sl, F_GX) ? 0u : 8u), p.oA + 4u * lz);
                 }
 #pragma unroll
                 for (int c = 0; c < MC; ++c) {
@@ -363,38 +390,63 @@ struct FluxTile {
                     Uc[c][sl] = Un;
                     W2k[c][sl] = W2n;
                     dprev[c][sl] = dn;
-                    if (STEADY || s + 3 <= nzl)
-                        nc2[c][sl] = bld(rin[c], co4[sl], 4u * SZ * (unsigned)(s + 3));
-                    if (STEADY || s + 2 <= nzl) nwz[c][sl] = bld(rf, m4[sl], oW[c] + lz + 4u * NXY);
-                    if (!ODD) a = adv_x<EX>(a, AX[sl].x, AX[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
+                    if (!ODD) a = adv_x<EX>(a, CA[sl].x, CA[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
                     P[c][sl] = a;
                 }
-                if (!ODD && (STEADY || s + 1 <= nzl)) AX[sl] = bld2(rf, m16[sl], p.oA + 4u * lz);
             }
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl)
                 if (sl == 0 || sl == NS - 1)
 #pragma unroll
                     for (int c = 0; c < MC; ++c) *row(SP, c, lj[sl]) = P[c][sl];
+            // Reloads go after the last use of the registers they refill: a
+            // load issued while the old value is still live lands in another
+            // register and is copied back, and the copy waits for the load.
+            if (STEADY) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl) {
+                if (EDGE && (STEADY || s + 1 <= nzl))
+                    gv[sl] = bld(rf, m16[sl] + (has(sl, F_GX) ? 0u : 8u), p.oA + 4u * lz);
+#pragma unroll
+                for (int c = 0; c < MC; ++c) {
+                    if (PIPE)
+                        nc2[c][sl] = nc3[c][sl];
+                    else if (STEADY || s + 3 <= nzl)
+                        nc2[c][sl] = bld(rin[c], co4[sl], 4u * SZ * (unsigned)(s + 3));
+                    if (STEADY || s + 2 <= nzl) nwz[c][sl] = bld(rf, m4[sl], oW[c] + lz + 4u * NXY);
+                }
+                if (STEADY || s + 1 <= nzl) CAn[sl] = bld4(rf, m16[sl], p.oA + 4u * lz);
+            }
         }
         __syncthreads();
 
         // ======== phase 2: advcty fluxes at level s; last sweep of level s-2 ========
+        // The band-edge neighbours are read first so the LDS reads overlap.
+        // Every row is computed (no wave-uniform skips, so no loop-carried
+        // value is assigned under a branch); the region's first and last
+        // rows read a clamped neighbour and their results are never used.
+        float ePm[MC], ePp[MC], eqm[MC], eqp[MC];
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            ePm[c] = *row(SP, c, max(lj[0] - 1, 0));
+            ePp[c] = *row(SP, c, min(lj[NS - 1] + 1, RY - 1));
+            eqm[c] = *row(SQ, c, max(lj[0] - 1, 0));
+            eqp[c] = *row(SQ, c, min(lj[NS - 1] + 1, RY - 1));
+        }
         if (STEADY || s <= nzl) {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
-                if (lj[sl] < 1 || lj[sl] >= RY - 1) continue;  // wave-uniform
 #pragma unroll
                 for (int c = 0; c < MC; ++c) {
-                    const float Pm = (sl > 0) ? P[c][sl - 1] : *row(SP, c, lj[sl] - 1);
-                    const float Pp = (sl < NS - 1) ? P[c][sl + 1] : *row(SP, c, lj[sl] + 1);
-                    const float w2 = AY[sl].y * minmod(Pp - P[c][sl], P[c][sl] - Pm);
-                    phi[c][sl] = fmaf(AY[sl].x, P[c][sl], w2);
+                    const float Pm = (sl > 0) ? P[c][sl - 1] : ePm[c];
+                    const float Pp = (sl < NS - 1) ? P[c][sl + 1] : ePp[c];
+                    const float w2 = CA[sl].w * minmod(Pp - P[c][sl], P[c][sl] - Pm);
+                    phi[c][sl] = fmaf(CA[sl].z, P[c][sl], w2);
                     if (EY) {
                         // advcty.f90:49-51: W2 clamps to the line at j = 1 / ny,
                         // A at the ghost row is v(1) / v(ny)
-                        apl[c][sl] = fmaf(AY[sl].x, Pm, w2);
-                        apr[c][sl] = fmaf(AY[sl].x, Pp, w2);
+                        apl[c][sl] = fmaf(CA[sl].z, Pm, w2);
+                        apr[c][sl] = fmaf(CA[sl].z, Pp, w2);
                     }
                     if (sl == 0 || sl == NS - 1) *row(SF, c, lj[sl]) = phi[c][sl];
                 }
@@ -410,8 +462,8 @@ struct FluxTile {
 #pragma unroll
                 for (int c = 0; c < MC; ++c) {
                     // diffy (diffy.f90:30-44; line ends 20-29, 45-53)
-                    const float qm = (sl > 0) ? qc[c][sl - 1] : *row(SQ, c, lj[sl] - 1);
-                    const float qp = (sl < NS - 1) ? qc[c][sl + 1] : *row(SQ, c, lj[sl] + 1);
+                    const float qm = (sl > 0) ? qc[c][sl - 1] : eqm[c];
+                    const float qp = (sl < NS - 1) ? qc[c][sl + 1] : eqp[c];
                     const float q = qc[c][sl];
                     float glo = dyl[sl] * (q - qm);
                     float ghi = Du * (qp - q);
@@ -439,6 +491,19 @@ struct FluxTile {
                 }
             }
         }
+        // dfy: the level just swept is done with; level s is loaded for the
+        // sweep two levels on
+        if (STEADY) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int sl = 0; sl < NS; ++sl) {
+            dyl[sl] = dyl1[sl];
+            if (sl == NS - 1) dyu[sl] = dyu1[sl];
+            if (STEADY || s <= nzl) {
+                const unsigned ly = p.oY + 4u * NXY * (unsigned)(s - 1);
+                dyl1[sl] = bld(rf, m4[sl], ly);
+                if (sl == NS - 1) dyu1[sl] = bld(rf, mup4[sl], ly);
+            }
+        }
         __syncthreads();
 
         // ======== phase 3: advcty (+ advctx on odd steps) at level s; diffz and
@@ -447,20 +512,26 @@ struct FluxTile {
         const int k = s - 1;
         const ZLev zk = zt[k > 0 ? k : 0];
         const unsigned lk = 4u * NXY * (unsigned)(k > 0 ? k - 1 : 0);  // level k in a 3-D field
+        // band-edge fluxes first; every row is computed, as in phase 2
+        float epl[MC], epr[MC];
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            epl[c] = *row(SF, c, max(lj[0] - 1, 0));
+            epr[c] = *row(SF, c, min(lj[NS - 1] + 1, RY - 1));
+        }
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
-            if (lj[sl] < 2 || lj[sl] >= RY - 2) continue;  // wave-uniform
-            const float lrn = Dn[sl].z, pzn = Dn[sl].w;  // level s (unused at s = nz+1)
-            const float4 Dk = D0[sl];                    // level k (unused at s = 1)
-            const float lr0 = Dk.z, pz0 = Dk.w;
-            const float ay = AY[sl].x;
+            const float lrn = Dn[sl].y, pzn = Dn[sl].z;  // level s (unused at s = nz+1)
+            const float3 Dk = D0[sl];                    // level k (unused at s = 1)
+            const float lr0 = Dk.y, pz0 = Dk.z;
+            const float ay = CA[sl].z;
 #pragma unroll
             for (int c = 0; c < MC; ++c) {
                 float bnew;
                 if (STEADY || s <= nzl) {
                     const float Pc = P[c][sl], ph = phi[c][sl];
-                    float pl = (sl > 0) ? phi[c][sl - 1] : *row(SF, c, lj[sl] - 1);
-                    float pr = (sl < NS - 1) ? phi[c][sl + 1] : *row(SF, c, lj[sl] + 1);
+                    float pl = (sl > 0) ? phi[c][sl - 1] : epl[c];
+                    float pr = (sl < NS - 1) ? phi[c][sl + 1] : epr[c];
                     if (EY) {
                         pl = has(sl, F_YLO) ? apl[c][sl] : pl;
                         pr = has(sl, F_YHI) ? apr[c][sl] : pr;
@@ -468,7 +539,7 @@ struct FluxTile {
                     bnew = (ay > 0.f) ? Pc + (pl - ph) : Pc + (ph - pr);
                     if (EDGE) bnew = inner(sl) ? bnew : Pc;
                     if (ODD)
-                        bnew = adv_x<EX>(bnew, AX[sl].x, AX[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
+                        bnew = adv_x<EX>(bnew, CA[sl].x, CA[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
                 } else {
                     bnew = c0[c][sl];  // b(nz+1): the top ghost of c_in
                 }
@@ -502,14 +573,13 @@ struct FluxTile {
             }
             if (!STEADY && s == 1) {
                 D0[sl] = Dn[sl];
-                Dn[sl] = bld4(rf, m16[sl], p.oD + 16u * NXY);  // level 2 (nz >= 2)
+                Dn[sl] = bld3(rf, m12[sl], p.oD + 12u * NXY);  // level 2 (nz >= 2)
                 continue;
             }
             D0[sl] = Dn[sl];
-            if (STEADY || s + 1 <= nzl) Dn[sl] = bld4(rf, m16[sl], p.oD + 16u * NXY * (unsigned)s);
             // first horizontal diffusion at level k
             if (!ODD) {
-                const float pk = EX ? bld2(rf, m4[sl] * 2u, p.oE + 2u * lk).x : 0.f;
+                const float pk = EX ? bld(rf, m4[sl] * 2u, p.oE + 2u * lk) : 0.f;
 #pragma unroll
                 for (int c = 0; c < MC; ++c)
                     dc[c][sl] = diff_x<EX>(dc[c][sl], lr0, Dk.x, pk, p.dtdx2, inner(sl), has(sl, F_XLO),
@@ -517,30 +587,29 @@ struct FluxTile {
             } else {
                 lrc[sl] = lr0;
                 dxc[sl] = Dk.x;
-                if (EX) pxc[sl] = bld2(rf, m4[sl] * 2u, p.oE + 2u * lk).x;
+                if (EX) pxc[sl] = bld(rf, m4[sl] * 2u, p.oE + 2u * lk);
             }
 #pragma unroll
             for (int c = 0; c < MC; ++c) qc[c][sl] = dc[c][sl] * lr0;
-            dyl[sl] = Dk.y;
-            if (sl == NS - 1) dyu[sl] = bld(rf, mup16[sl] + 4u, p.oD + 4u * lk);
             if (EY) {
                 pyo[sl] = bld(rf, m4[sl] * 2u + 4u, p.oE + 2u * lk);
                 pym[sl] = bld(rf, mdn8[sl] + 4u, p.oE + 2u * lk);
                 pyp[sl] = bld(rf, mup8[sl] + 4u, p.oE + 2u * lk);
             }
         }
-        // (ay, fy) of every row, and (ax, fx) on odd steps, for level s+1
+        // after their last use: (dfx, lr, pz) of level s+1; its (ax, fx, ay,
+        // fy) arrived during phase 1
+        if (STEADY) __builtin_amdgcn_sched_barrier(0);
         if (STEADY || s + 1 <= nzl) {
-            const unsigned la = p.oA + 16u * NXY * (unsigned)s;
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
-                AY[sl] = bld2(rf, m16[sl] + 8u, la);
-                if (ODD) AX[sl] = bld2(rf, m16[sl], la);
+                if (STEADY || s >= 2) Dn[sl] = bld3(rf, m12[sl], p.oD + 12u * NXY * (unsigned)s);
+                CA[sl] = CAn[sl];
             }
         }
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl)
-            if ((sl == 0 || sl == NS - 1) && lj[sl] >= 2 && lj[sl] < RY - 2 && (STEADY || s >= 2))
+            if ((sl == 0 || sl == NS - 1) && (STEADY || s >= 2))
 #pragma unroll
                 for (int c = 0; c < MC; ++c) *row(SQ, c, lj[sl]) = qc[c][sl];
     }
@@ -558,6 +627,14 @@ struct FluxTile {
             // with the same pending loads as its own back edge, and its
             // waits count only the loads issued one level back
             __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            // two levels per iteration: a register reloaded in one level
+            // and its partner in the next swap roles, so no loaded value is
+            // copied (a copy would wait for the load)
+            if (PIPE)
+                for (; s + 1 <= nzl - 3; s += 2) {
+                    level<true>(s);
+                    level<true>(s + 1);
+                }
             for (; s <= nzl - 3; ++s) level<true>(s);
         }
         for (; s <= nzl + 2; ++s) level<false>(s);
@@ -728,6 +805,7 @@ static int make_params(const StepArgs &a, const Fields &f, FluxParams &p)
     p.fbytes = (unsigned)f.bytes;
     p.oA = off(f.cA);
     p.oD = off(f.cD);
+    p.oY = off(f.dfy);
     p.oE = off(f.cE);
     p.oW = off(f.wz);
     p.oBF = off(f.bfac);

@@ -630,12 +630,14 @@ __global__ void k_fields_face(Grid g, const double *vx, const double *vy, const 
 }
 
 // interleave the per-cell groups read together by the flux kernel
-__global__ void k_fields_pack(long n, Fields f, float4 *cA, float4 *cD, float2 *cE)
+__global__ void k_fields_pack(long n, Fields f, float4 *cA, float *cD, float2 *cE)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     cA[t] = make_float4(f.ax[t], f.fx[t], f.ay[t], f.fy[t]);
-    cD[t] = make_float4(f.dfx[t], f.dfy[t], f.lr[t], f.pkz[t]);
+    cD[3 * t] = f.dfx[t];
+    cD[3 * t + 1] = f.lr[t];
+    cD[3 * t + 2] = f.pkz[t];
     cE[t] = make_float2(f.pkx[t], f.pky[t]);
 }
 
@@ -761,8 +763,8 @@ Fields fields_view(const Grid &g, const float *b0, const double *vsum1)
 {
     Fields f;
     f.cA = reinterpret_cast<const float4 *>(b0);
-    f.cD = reinterpret_cast<const float4 *>(b0 + 4 * g.N);
-    f.cE = reinterpret_cast<const float2 *>(b0 + 8 * g.N);
+    f.cE = reinterpret_cast<const float2 *>(b0 + 4 * g.N);
+    f.cD = b0 + 6 * g.N;
     f.zl = reinterpret_cast<const ZLev *>(b0 + 10 * g.N);
     f.base = b0;
     f.bytes = sizeof(float) * fields_floats(g);
@@ -809,7 +811,7 @@ int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *v
                        const_cast<float *>(f.dfy));
     F3D_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_fields_pack, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g.N, f,
-                       const_cast<float4 *>(f.cA), const_cast<float4 *>(f.cD),
+                       const_cast<float4 *>(f.cA), const_cast<float *>(f.cD),
                        const_cast<float2 *>(f.cE));
     F3D_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_zlev, dim3(1), dim3(64), 0, s, g.nz, zc, dt, const_cast<ZLev *>(f.zl));
