@@ -179,6 +179,10 @@ struct FluxTile {
     int nz, li, w;
     unsigned NXY, SZ;
     float *SP, *SF, *SQ;  // MC planes each
+    // odd steps: (dfx, 1/rho) of the level awaiting its last sweep, own rows
+    // only (kept in LDS rather than registers, which would keep the loaded
+    // record live into a third level)
+    float *SC;
     rsrc_t rf;            // the field buffer
     const ZLev *zt;       // the z-level table, staged in LDS
     float dtop, ldz0;
@@ -208,7 +212,7 @@ struct FluxTile {
     // band) and of the level after it
     float dyl[NS], dyu[NS], dyl1[NS], dyu1[NS];
     // carried between phase 3 and the next phase 2 (class-independent)
-    float pyo[NS], pym[NS], pyp[NS], lrc[NS], dxc[NS], pxc[NS];
+    float pyo[NS], pym[NS], pyp[NS], pxc[NS];
     // per class: advctzc window c(s), c(s+1), w(s), U(s), W2(s), delta(s) (after
     // level nz, c0 holds the top ghost c(nz+1)), prefetched c(s+2), w(s+1);
     // diffz window b(k), q(k) = b/rho and the face flux it leaves for the
@@ -237,6 +241,7 @@ struct FluxTile {
         SP = sm;
         SF = sm + MC * PL;
         SQ = sm + 2 * MC * PL;
+        SC = sm + 3 * MC * PL;
         const size_t G = (size_t)p.g.G, N = (size_t)p.g.N;
         rf = mkbuf(p.fbase, p.fbytes);
         const int nq = nc * p.g.nm;
@@ -306,7 +311,7 @@ struct FluxTile {
             dyl1[sl] = dyu1[sl] = 0.f;
             gv[sl] = EDGE ? bld(rf, m16[sl] + (has(sl, F_GX) ? 0u : 8u), p.oA) : 0.f;
             dyl[sl] = dyu[sl] = pyo[sl] = pym[sl] = pyp[sl] = 0.f;
-            lrc[sl] = dxc[sl] = pxc[sl] = 0.f;
+            pxc[sl] = 0.f;
 #pragma unroll
             for (int c = 0; c < MC; ++c) {
                 const float c1 = bld(rin[c], co4[sl], 4 * SZ);
@@ -425,7 +430,14 @@ struct FluxTile {
         // Every row is computed (no wave-uniform skips, so no loop-carried
         // value is assigned under a branch); the region's first and last
         // rows read a clamped neighbour and their results are never used.
-        float ePm[MC], ePp[MC], eqm[MC], eqp[MC];
+        float ePm[MC], ePp[MC], eqm[MC], eqp[MC], edx[NS], elr[NS];
+        if (ODD) {
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl) {
+                edx[sl] = *row(SC, 0, lj[sl]);
+                elr[sl] = *row(SC, 1, lj[sl]);
+            }
+        }
 #pragma unroll
         for (int c = 0; c < MC; ++c) {
             ePm[c] = *row(SP, c, max(lj[0] - 1, 0));
@@ -476,7 +488,7 @@ struct FluxTile {
                     float out = dc[c][sl] + (ghi - glo);
                     if (EDGE) out = inner(sl) ? out : dc[c][sl];
                     if (ODD)
-                        out = diff_x<EX>(out, lrc[sl], dxc[sl], pxc[sl], p.dtdx2, inner(sl),
+                        out = diff_x<EX>(out, elr[sl], edx[sl], pxc[sl], p.dtdx2, inner(sl),
                                          has(sl, F_XLO), has(sl, F_XHI));
                     const bool store = li >= 3 && li < RX - 3 && inner(sl) && (c == 0 || live[c]);
                     if (store) {
@@ -585,8 +597,8 @@ struct FluxTile {
                     dc[c][sl] = diff_x<EX>(dc[c][sl], lr0, Dk.x, pk, p.dtdx2, inner(sl), has(sl, F_XLO),
                                            has(sl, F_XHI));
             } else {
-                lrc[sl] = lr0;
-                dxc[sl] = Dk.x;
+                *row(SC, 0, lj[sl]) = Dk.x;
+                *row(SC, 1, lj[sl]) = lr0;
                 if (EX) pxc[sl] = bld(rf, m4[sl] * 2u, p.oE + 2u * lk);
             }
 #pragma unroll
@@ -692,7 +704,7 @@ __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
         }
     }
     const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * (RY - 6) - 3;
-    __shared__ float sm[3 * MC * RX * RY];
+    __shared__ float sm[(3 * MC + (ODD ? 2 : 0)) * RX * RY];
     __shared__ ZLev zt[kMaxLevels + 2];
     for (int k = threadIdx.x; k <= p.g.nz + 1; k += NW * 64) zt[k] = p.zl[k];
     __syncthreads();
@@ -851,7 +863,7 @@ __global__ void __launch_bounds__(NW * 64)
     constexpr int PW = sizeof(FluxParams) / sizeof(int);
     static_assert(sizeof(FluxParams) % sizeof(int) == 0, "parameter block of whole dwords");
     static_assert(PW <= NW * 64, "one thread per parameter dword");
-    __shared__ float sm[3 * MC * RX * RY];
+    __shared__ float sm[(3 * MC + (ODD ? 2 : 0)) * RX * RY];
     __shared__ ZLev zt[kMaxLevels + 2];
     __shared__ int sp[PW];
     __shared__ int sitem[4];
