@@ -29,6 +29,9 @@ module fall3d_amd
   public :: f3d_create, f3d_destroy, f3d_set_classes, f3d_set_models, f3d_set_meteo, &
             f3d_get_dt, f3d_set_source, f3d_step, f3d_budget, f3d_get_state, f3d_sync, &
             f3d_get_clock, f3d_error_message
+  ! batching and tuning (no counterpart in the reference)
+  public :: f3d_batch_create, f3d_batch_destroy, f3d_batch_step, f3d_batch_sync, &
+            f3d_batch_autotune, f3d_autotune
 
   interface
      integer(c_int) function f3d_create(ctx, grid, precision) bind(C, name='f3d_create')
@@ -113,6 +116,48 @@ module fall3d_amd
        real(c_double), intent(out) :: time, erumass, outmass
        integer(c_int), intent(out) :: iiter
      end function f3d_get_clock
+
+     ! several contexts stepped together (ctxs: array of n context handles)
+     integer(c_int) function f3d_batch_create(batch, ctxs, n) bind(C, name='f3d_batch_create')
+       import :: c_int, c_ptr
+       type(c_ptr), intent(out) :: batch
+       type(c_ptr), intent(in) :: ctxs(*)
+       integer(c_int), value :: n
+     end function f3d_batch_create
+
+     subroutine f3d_batch_destroy(batch) bind(C, name='f3d_batch_destroy')
+       import :: c_ptr
+       type(c_ptr), value :: batch
+     end subroutine f3d_batch_destroy
+
+     integer(c_int) function f3d_batch_step(batch, nsteps) bind(C, name='f3d_batch_step')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+       integer(c_int), value :: nsteps
+     end function f3d_batch_step
+
+     integer(c_int) function f3d_batch_sync(batch) bind(C, name='f3d_batch_sync')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+     end function f3d_batch_sync
+
+     integer(c_int) function f3d_batch_autotune(batch, trials, tx, ty, slabs, best_ms) &
+          bind(C, name='f3d_batch_autotune')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: batch
+       integer(c_int), value :: trials
+       integer(c_int), intent(out) :: tx, ty, slabs
+       real(c_double), intent(out) :: best_ms
+     end function f3d_batch_autotune
+
+     integer(c_int) function f3d_autotune(ctx, trials, tx, ty, slabs, best_ms) &
+          bind(C, name='f3d_autotune')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: trials
+       integer(c_int), intent(out) :: tx, ty, slabs
+       real(c_double), intent(out) :: best_ms
+     end function f3d_autotune
 
      type(c_ptr) function f3d_last_error_c() bind(C, name='f3d_last_error')
        import :: c_ptr
