@@ -140,11 +140,11 @@ int record_step(f3d_batch *b, bool odd0)
         TRY(fp32_setsrc(b->ctx[k], s, args[k]));
     }
     // frames (c_in -> c_out's ghost frame, disjoint from what the flux
-    // kernels touch) on the side stream ahead of the edge lists
+    // kernels touch) on the side stream after the edge lists
     F3D_HIP(hipEventRecord(b->fork, s));
     F3D_HIP(hipStreamWaitEvent(b->side, b->fork, 0));
-    for (size_t k = 0; k < b->ctx.size(); ++k) TRY(fp32_frame(b->ctx[k], b->side, args[k]));
     TRY(f32::flux_batch_step(s, b->side, b->fork, b->join, b->plan, odd0));
+    for (size_t k = 0; k < b->ctx.size(); ++k) TRY(fp32_frame(b->ctx[k], b->side, args[k]));
     F3D_HIP(hipEventRecord(b->join, b->side));
     F3D_HIP(hipStreamWaitEvent(s, b->join, 0));
     return F3D_OK;

@@ -144,19 +144,19 @@ int step_fp32(f3d_ctx *c)
     TRY(fp32_setsrc(c, s, a));
     // The frame reads only c_in and writes only c_out's ghost frame, which
     // the flux kernels neither read nor write: it runs on the side stream
-    // ahead of the edge tiles, beside the interior tiles.
+    // after the edge tiles, beside the interior tiles.
     const bool side = c->side && c->ev_fork && c->ev_join;
     if (side) {
+        // fork here as well: the flux launch forks only when it has both
+        // interior and edge tiles
         F3D_HIP(hipEventRecord(c->ev_fork, s));
         F3D_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-        TRY(fp32_frame(c, c->side, a));
     }
     record_start(c);
     TRY(f32::fused_step(s, a, f, c->tile));
     record_stop(c);
     if (side) {
-        // the flux launch joins the side stream when it forks; join here too
-        // for the shapes that do not fork (no edge or no interior tiles)
+        TRY(fp32_frame(c, c->side, a));
         F3D_HIP(hipEventRecord(c->ev_join, c->side));
         F3D_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
     } else {
