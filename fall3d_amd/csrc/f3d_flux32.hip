@@ -212,7 +212,10 @@ struct FluxTile {
     // band) and of the level after it
     float dyl[NS], dyu[NS], dyl1[NS], dyu1[NS];
     // carried between phase 3 and the next phase 2 (class-independent)
-    float pyo[NS], pym[NS], pyp[NS], pxc[NS];
+    float pyo[NS], pym[NS], pyp[NS];
+    // x-side tiles: rho*rkh1 of the level awaiting its diffusion, loaded a
+    // level ahead (the line-end diffx fluxes)
+    float pkk[NS];
     // per class: advctzc window c(s), c(s+1), w(s), U(s), W2(s), delta(s) (after
     // level nz, c0 holds the top ghost c(nz+1)), prefetched c(s+2), w(s+1);
     // diffz window b(k), q(k) = b/rho and the face flux it leaves for the
@@ -311,7 +314,7 @@ struct FluxTile {
             dyl1[sl] = dyu1[sl] = 0.f;
             gv[sl] = EDGE ? bld(rf, m16[sl] + (has(sl, F_GX) ? 0u : 8u), p.oA) : 0.f;
             dyl[sl] = dyu[sl] = pyo[sl] = pym[sl] = pyp[sl] = 0.f;
-            pxc[sl] = 0.f;
+            pkk[sl] = 0.f;
 #pragma unroll
             for (int c = 0; c < MC; ++c) {
                 const float c1 = bld(rin[c], co4[sl], 4 * SZ);
@@ -430,12 +433,13 @@ struct FluxTile {
         // Every row is computed (no wave-uniform skips, so no loop-carried
         // value is assigned under a branch); the region's first and last
         // rows read a clamped neighbour and their results are never used.
-        float ePm[MC], ePp[MC], eqm[MC], eqp[MC], edx[NS], elr[NS];
+        float ePm[MC], ePp[MC], eqm[MC], eqp[MC], edx[NS], elr[NS], epk[NS];
         if (ODD) {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 edx[sl] = *row(SC, 0, lj[sl]);
                 elr[sl] = *row(SC, 1, lj[sl]);
+                epk[sl] = EX ? *row(SC, 2, lj[sl]) : 0.f;
             }
         }
 #pragma unroll
@@ -488,7 +492,7 @@ struct FluxTile {
                     float out = dc[c][sl] + (ghi - glo);
                     if (EDGE) out = inner(sl) ? out : dc[c][sl];
                     if (ODD)
-                        out = diff_x<EX>(out, elr[sl], edx[sl], pxc[sl], p.dtdx2, inner(sl),
+                        out = diff_x<EX>(out, elr[sl], edx[sl], epk[sl], p.dtdx2, inner(sl),
                                          has(sl, F_XLO), has(sl, F_XHI));
                     const bool store = li >= 3 && li < RX - 3 && inner(sl) && (c == 0 || live[c]);
                     if (store) {
@@ -591,7 +595,7 @@ struct FluxTile {
             D0[sl] = Dn[sl];
             // first horizontal diffusion at level k
             if (!ODD) {
-                const float pk = EX ? bld(rf, m4[sl] * 2u, p.oE + 2u * lk) : 0.f;
+                const float pk = EX ? pkk[sl] : 0.f;
 #pragma unroll
                 for (int c = 0; c < MC; ++c)
                     dc[c][sl] = diff_x<EX>(dc[c][sl], lr0, Dk.x, pk, p.dtdx2, inner(sl), has(sl, F_XLO),
@@ -599,7 +603,7 @@ struct FluxTile {
             } else {
                 *row(SC, 0, lj[sl]) = Dk.x;
                 *row(SC, 1, lj[sl]) = lr0;
-                if (EX) pxc[sl] = bld(rf, m4[sl] * 2u, p.oE + 2u * lk);
+                if (EX) *row(SC, 2, lj[sl]) = pkk[sl];
             }
 #pragma unroll
             for (int c = 0; c < MC; ++c) qc[c][sl] = dc[c][sl] * lr0;
@@ -610,8 +614,14 @@ struct FluxTile {
             }
         }
         // after their last use: (dfx, lr, pz) of level s+1; its (ax, fx, ay,
-        // fy) arrived during phase 1
+        // fy) arrived during phase 1; on x-side tiles rho*rkh1 of level s,
+        // which the next level diffuses
         if (STEADY) __builtin_amdgcn_sched_barrier(0);
+        if (EX && (STEADY || s <= nzl)) {
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl)
+                pkk[sl] = bld(rf, m4[sl] * 2u, p.oE + 8u * NXY * (unsigned)(s - 1));
+        }
         if (STEADY || s + 1 <= nzl) {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
@@ -661,11 +671,28 @@ __device__ __forceinline__ void flux_tile(const FluxParams &p, const int *qv, co
     t.run();
 }
 
-// One workgroup per (tile, group of MC consecutive member*class slabs).
-// Interior tiles (the rectangle [bx0, bx1] x [by0, by1] of tiles whose region
-// lies inside the domain) and edge tiles run as two kernels, so the interior
-// one keeps the register budget of the plain path.
-template <int RY, int NW, int MC, bool ODD, bool EDGEK>
+// Tile kinds.  The rectangle [bx0, bx1] x [by0, by1] holds the interior
+// tiles (region inside the domain); a column outside [bx0, bx1] meets an x
+// face of the frame, a row outside [by0, by1] a y face.  Interior tiles are
+// one kernel; the x-side, y-side and corner tiles share a second one (one
+// launch, so the small kinds do not each wait for a launch's tail), which
+// picks the tile's code path per workgroup.
+enum : int { K_INTERIOR = 0, K_EDGE = 1 };
+
+// the j-th edge column (outside [bx0, bx1]) / edge row (outside [by0, by1])
+__device__ __forceinline__ int edge_col(const FluxParams &p, int c)
+{
+    return (c < p.bx0) ? c : p.bx1 + 1 + (c - p.bx0);
+}
+__device__ __forceinline__ int edge_row(const FluxParams &p, int r)
+{
+    return (r < p.by0) ? r : p.by1 + 1 + (r - p.by0);
+}
+
+// One workgroup per (tile of kind KIND, group of MC consecutive
+// member*class slabs).  Consecutive tiles run down y, so the 6-row halos of
+// neighbours meet in L2.
+template <int RY, int NW, int MC, bool ODD, int KIND>
 __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
 {
     const int b = blockIdx.x;
@@ -678,51 +705,48 @@ __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
 #pragma unroll
     for (int c = 0; c < MC; ++c) qv[c] = grp * MC + c;
     const int t = L / ngrp;
-    int bx, by;
-    if (!EDGEK) {
-        // consecutive tiles run down y so the 6-row halos are shared in L2
-        const int nyi = p.by1 - p.by0 + 1;
-        by = p.by0 + t % nyi;
-        bx = p.bx0 + t / nyi;
+    const int nmy = p.by1 - p.by0 + 1;          // interior rows (>= 0)
+    const int nry = p.by0 + (p.nty - 1 - p.by1);  // edge rows
+    const int nmx = p.bx1 - p.bx0 + 1;
+    const int nex = p.ntx - nmx;
+    int bx, by, kind = 0;  // 0 interior, 1 x side, 2 y side, 3 corner
+    if (KIND == K_INTERIOR) {
+        by = p.by0 + t % nmy;
+        bx = p.bx0 + t / nmy;
     } else {
-        // edge tiles: full rows below by0, the side columns of rows by0..by1,
-        // full rows above by1
-        const int nside = p.bx0 + (p.ntx - 1 - p.bx1);
-        const int nmid = (p.by1 >= p.by0) ? (p.by1 - p.by0 + 1) : 0;
+        // x-side tiles first, then y-side, then corners, each down y
         int e = t;
-        if (e < p.by0 * p.ntx) {
-            by = e / p.ntx;
-            bx = e % p.ntx;
-        } else if ((e -= p.by0 * p.ntx) < nmid * nside) {
-            by = p.by0 + e / nside;
-            const int c = e % nside;
-            bx = (c < p.bx0) ? c : p.bx1 + 1 + (c - p.bx0);
+        if (e < nex * nmy) {
+            kind = 1;
+            by = p.by0 + e % nmy;
+            bx = edge_col(p, e / nmy);
+        } else if ((e -= nex * nmy) < nmx * nry) {
+            kind = 2;
+            by = edge_row(p, e % nry);
+            bx = p.bx0 + e / nry;
         } else {
-            e -= nmid * nside;
-            by = p.by0 + nmid + e / p.ntx;
-            bx = e % p.ntx;
+            e -= nmx * nry;
+            kind = 3;
+            by = edge_row(p, e % nry);
+            bx = edge_col(p, e / nry);
         }
     }
     const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * (RY - 6) - 3;
-    __shared__ float sm[(3 * MC + (ODD ? 2 : 0)) * RX * RY];
+    __shared__ float sm[(3 * MC + (ODD ? 3 : 0)) * RX * RY];
     __shared__ ZLev zt[kMaxLevels + 2];
     for (int k = threadIdx.x; k <= p.g.nz + 1; k += NW * 64) zt[k] = p.zl[k];
     __syncthreads();
-    if (!EDGEK) {
+    if (KIND == K_INTERIOR)
         flux_tile<RY, NW, MC, ODD, false, false>(p, qv, gi0, gj0, sm, zt);
-    } else {
-        const bool ex = gi0 < 1 || gi0 + RX - 1 > p.g.nx;
-        const bool ey = gj0 < 1 || gj0 + RY - 1 > p.g.ny;
-        if (ex && ey)
-            flux_tile<RY, NW, MC, ODD, true, true>(p, qv, gi0, gj0, sm, zt);
-        else if (ex)
-            flux_tile<RY, NW, MC, ODD, true, false>(p, qv, gi0, gj0, sm, zt);
-        else
-            flux_tile<RY, NW, MC, ODD, false, true>(p, qv, gi0, gj0, sm, zt);
-    }
+    else if (kind == 1)
+        flux_tile<RY, NW, MC, ODD, true, false>(p, qv, gi0, gj0, sm, zt);
+    else if (kind == 2)
+        flux_tile<RY, NW, MC, ODD, false, true>(p, qv, gi0, gj0, sm, zt);
+    else
+        flux_tile<RY, NW, MC, ODD, true, true>(p, qv, gi0, gj0, sm, zt);
 }
 
-template <int RY, int NW, int MC, bool EDGEK>
+template <int RY, int NW, int MC, int KIND>
 void launch_part(hipStream_t s, FluxParams p, int ntiles, bool odd)
 {
     if (ntiles <= 0) return;
@@ -731,9 +755,9 @@ void launch_part(hipStream_t s, FluxParams p, int ntiles, bool odd)
     p.per_xcd = (p.nlogical + kXCD - 1) / kXCD;
     const unsigned nb = (unsigned)(p.per_xcd * kXCD);
     if (odd)
-        hipLaunchKernelGGL((k_flux<RY, NW, MC, true, EDGEK>), dim3(nb), dim3(NW * 64), 0, s, p);
+        hipLaunchKernelGGL((k_flux<RY, NW, MC, true, KIND>), dim3(nb), dim3(NW * 64), 0, s, p);
     else
-        hipLaunchKernelGGL((k_flux<RY, NW, MC, false, EDGEK>), dim3(nb), dim3(NW * 64), 0, s, p);
+        hipLaunchKernelGGL((k_flux<RY, NW, MC, false, KIND>), dim3(nb), dim3(NW * 64), 0, s, p);
 }
 
 template <int RY, int NW, int MC>
@@ -749,14 +773,18 @@ int launch_flux(hipStream_t s, const StepArgs &a, const FluxParams &p0, bool odd
     p.bx1 = min(p.ntx - 1, (nx - (RX - 3)) >= 0 ? (nx - (RX - 3)) / TXR : -1);
     p.by0 = 1;
     p.by1 = min(p.nty - 1, (ny - (RY - 3)) >= 0 ? (ny - (RY - 3)) / ty : -1);
-    if (p.bx1 < p.bx0 || p.by1 < p.by0) {
-        // no interior tile: every tile is an edge tile (all rows "below by0")
-        p.bx0 = 0;
+    // an axis without interior tiles: every column (row) is an edge one
+    if (p.bx1 < p.bx0) {
+        p.bx0 = p.ntx;
         p.bx1 = p.ntx - 1;
+    }
+    if (p.by1 < p.by0) {
         p.by0 = p.nty;
         p.by1 = p.nty - 1;
     }
-    const int nint = (p.bx1 - p.bx0 + 1) * (p.by1 >= p.by0 ? p.by1 - p.by0 + 1 : 0);
+    const int nmx = p.bx1 - p.bx0 + 1, nmy = p.by1 - p.by0 + 1;
+    const int nex = p.ntx - nmx, ney = p.nty - nmy;
+    const int nint = nmx * nmy;
     const int nedge = p.ntx * p.nty - nint;
     // edge tiles on the side stream when there is one; 1024-lane groups take
     // one slab per group there (two would exceed the register budget)
@@ -767,9 +795,9 @@ int launch_flux(hipStream_t s, const StepArgs &a, const FluxParams &p0, bool odd
         F3D_HIP(hipEventRecord(a.fork, s));
         F3D_HIP(hipStreamWaitEvent(se, a.fork, 0));
     }
-    launch_part<RY, NW, MCE, true>(se, p, nedge, odd);
+    launch_part<RY, NW, MCE, K_EDGE>(se, p, nedge, odd);
     F3D_CHECK_LAUNCH();
-    launch_part<RY, NW, MC, false>(s, p, nint, odd);
+    launch_part<RY, NW, MC, K_INTERIOR>(s, p, nint, odd);
     F3D_CHECK_LAUNCH();
     if (fork) {
         F3D_HIP(hipEventRecord(a.join, se));
@@ -863,7 +891,7 @@ __global__ void __launch_bounds__(NW * 64)
     constexpr int PW = sizeof(FluxParams) / sizeof(int);
     static_assert(sizeof(FluxParams) % sizeof(int) == 0, "parameter block of whole dwords");
     static_assert(PW <= NW * 64, "one thread per parameter dword");
-    __shared__ float sm[(3 * MC + (ODD ? 2 : 0)) * RX * RY];
+    __shared__ float sm[(3 * MC + (ODD ? 3 : 0)) * RX * RY];
     __shared__ ZLev zt[kMaxLevels + 2];
     __shared__ int sp[PW];
     __shared__ int sitem[4];
