@@ -41,6 +41,8 @@ constexpr int TXR = RX - 6;  // output columns per tile
 constexpr int kMaxLevels = 64;
 // s_waitcnt immediate of gfx9: vmcnt(0), expcnt and lgkmcnt left open
 constexpr int kWaitVm0 = 0x0F70;
+// cache-policy operand of the buffer builtins on gfx950: the nt bit
+constexpr int kNonTemporal = 2;
 
 struct FluxParams {
     Grid g;
@@ -134,6 +136,13 @@ __device__ __forceinline__ float bld(rsrc_t r, unsigned voff, unsigned soff)
 {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
+// c_in, read once per step apart from the halo rows of neighbouring tiles:
+// non-temporal, so it does not displace the coefficient records and vz+vset
+// (read by every slab group of a tile, and by every ensemble member) in L2
+__device__ __forceinline__ float bldnt(rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, kNonTemporal));
+}
 __device__ __forceinline__ float2 bld2(rsrc_t r, unsigned voff, unsigned soff)
 {
     const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
@@ -151,9 +160,10 @@ __device__ __forceinline__ float4 bld4(rsrc_t r, unsigned voff, unsigned soff)
     return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
                        __uint_as_float(v.w));
 }
+// c_out is written once per step and read only by the next one: non-temporal
 __device__ __forceinline__ void bst(float x, rsrc_t r, unsigned voff, unsigned soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, (int)voff, (int)soff, kNonTemporal);
 }
 
 // per-row flags of edge tiles
@@ -225,6 +235,8 @@ struct FluxTile {
     float nc2[MC][NS], nwz[MC][NS];
     float b0[MC][NS], q0[MC][NS], gprev[MC][NS];
     float dc[MC][NS], qc[MC][NS], P[MC][NS], phi[MC][NS];
+    // clipped outputs of the last sweep, awaiting flush()
+    float sto[MC][NS];
     // edge rows (EY): the y fluxes with W2 clamped to the line, formed in
     // phase 2 (phase 3 must not read the SP plane: the next level's phase 1
     // may already be rewriting it)
@@ -336,6 +348,7 @@ struct FluxTile {
                 b0[c][sl] = q0[c][sl] = 0.f;
                 dc[c][sl] = qc[c][sl] = 0.f;
                 P[c][sl] = phi[c][sl] = 0.f;
+                sto[c][sl] = 0.f;
                 apl[c][sl] = apr[c][sl] = 0.f;
             }
         }
@@ -361,7 +374,7 @@ struct FluxTile {
 #pragma unroll
                     for (int c = 0; c < MC; ++c)
                         nc3[c][sl] = (STEADY || s + 3 <= nzl)
-                                         ? bld(rin[c], co4[sl], 4u * SZ * (unsigned)(s + 3)) : 0.f;
+                                         ? bldnt(rin[c], co4[sl], 4u * SZ * (unsigned)(s + 3)) : 0.f;
                 if (STEADY) __builtin_amdgcn_sched_barrier(0);
             }
 #pragma unroll
@@ -420,7 +433,7 @@ struct FluxTile {
                     if (PIPE)
                         nc2[c][sl] = nc3[c][sl];
                     else if (STEADY || s + 3 <= nzl)
-                        nc2[c][sl] = bld(rin[c], co4[sl], 4u * SZ * (unsigned)(s + 3));
+                        nc2[c][sl] = bldnt(rin[c], co4[sl], 4u * SZ * (unsigned)(s + 3));
                     if (STEADY || s + 2 <= nzl) nwz[c][sl] = bld(rf, m4[sl], oW[c] + lz + 4u * NXY);
                 }
                 if (STEADY || s + 1 <= nzl) CAn[sl] = bld4(rf, m16[sl], p.oA + 4u * lz);
@@ -470,7 +483,6 @@ struct FluxTile {
         }
         if (STEADY || s >= 3) {
             const int k = s - 2;
-            const unsigned ko = 4u * SZ * (unsigned)k;
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 if (lj[sl] < 3 || lj[sl] >= RY - 3) continue;  // wave-uniform
@@ -494,15 +506,16 @@ struct FluxTile {
                     if (ODD)
                         out = diff_x<EX>(out, elr[sl], edx[sl], epk[sl], p.dtdx2, inner(sl),
                                          has(sl, F_XLO), has(sl, F_XHI));
-                    const bool store = li >= 3 && li < RX - 3 && inner(sl) && (c == 0 || live[c]);
-                    if (store) {
-                        bst(fmaxf(0.f, out), rout[c], co4[sl], ko);  // sizloop.f90:77-78
-                        if (!STEADY && k == 1) {
-                            // accum.f90:22: deposit from c(:,:,1) before the clip, FP64
-                            const size_t t2 = m4[sl] / 4u;
-                            double *cd = p.cdry + ((size_t)memv[c] * (p.g.nc + 1) + icv[c] + 1) * NXY + t2;
-                            *cd = *cd - p.dt64 * (double)out * p.vsum1[(size_t)icv[c] * NXY + t2];
-                        }
+                    // sizloop.f90:77-78; interior tiles store in flush()
+                    if (PIPE)
+                        sto[c][sl] = fmaxf(0.f, out);
+                    else if (stored(sl, c))
+                        bst(fmaxf(0.f, out), rout[c], co4[sl], 4u * SZ * (unsigned)k);
+                    if (!STEADY && k == 1 && stored(sl, c)) {
+                        // accum.f90:22: deposit from c(:,:,1) before the clip, FP64
+                        const size_t t2 = m4[sl] / 4u;
+                        double *cd = p.cdry + ((size_t)memv[c] * (p.g.nc + 1) + icv[c] + 1) * NXY + t2;
+                        *cd = *cd - p.dt64 * (double)out * p.vsum1[(size_t)icv[c] * NXY + t2];
                     }
                 }
             }
@@ -524,7 +537,10 @@ struct FluxTile {
 
         // ======== phase 3: advcty (+ advctx on odd steps) at level s; diffz and
         // the first horizontal diffusion at level s-1 ========
-        if (!STEADY && s >= nzl + 2) return;
+        if (!STEADY && s >= nzl + 2) {
+            flush(s);
+            return;
+        }
         const int k = s - 1;
         const ZLev zk = zt[k > 0 ? k : 0];
         const unsigned lk = 4u * NXY * (unsigned)(k > 0 ? k - 1 : 0);  // level k in a 3-D field
@@ -634,6 +650,29 @@ struct FluxTile {
             if ((sl == 0 || sl == NS - 1) && (STEADY || s >= 2))
 #pragma unroll
                 for (int c = 0; c < MC; ++c) *row(SQ, c, lj[sl]) = qc[c][sl];
+        flush(s);
+    }
+
+    // Interior tiles store the outputs of the last sweep (level s-2) after
+    // every load of the level is issued: vector-memory counters complete in
+    // order, so a store issued before a load makes every wait for that load
+    // also wait for the store's write.  (Edge tiles store at once: holding
+    // the outputs costs registers they do not have.)
+    __device__ __forceinline__ bool stored(int sl, int c) const
+    {
+        return li >= 3 && li < RX - 3 && inner(sl) && (c == 0 || live[c]);
+    }
+    __device__ __forceinline__ void flush(const int s)
+    {
+        if (!PIPE || !(s >= 3)) return;
+        const unsigned ko = 4u * SZ * (unsigned)(s - 2);
+#pragma unroll
+        for (int sl = 0; sl < NS; ++sl) {
+            if (lj[sl] < 3 || lj[sl] >= RY - 3) continue;  // wave-uniform
+#pragma unroll
+            for (int c = 0; c < MC; ++c)
+                if (stored(sl, c)) bst(sto[c][sl], rout[c], co4[sl], ko);
+        }
     }
 
     // Prologue, steady and epilogue levels run as three loops so each keeps

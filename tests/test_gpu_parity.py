@@ -211,6 +211,17 @@ def test_fp32_tile_edges_partial_tiles(tile):
     assert m["c_l1"] <= 1e-4 and m["load_l1"] <= 1e-4, m
 
 
+@pytest.mark.parametrize("shape", [(150, 20), (60, 61), (128, 64)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_fp32_tile_kinds_one_axis_without_interior_tiles(shape):
+    # 58x18 tiles: 150x20 has interior columns but no interior rows (y-side
+    # and corner tiles only), 60x61 the reverse (x-side and corner tiles),
+    # 128x64 one tile column and two tile rows of interior tiles
+    nx, ny = shape
+    sc = S.config_small(nx=nx, ny=ny, nz=7, nc=3, seed=41, mean_u=6.0, mean_v=-5.0)
+    m = _compare_fp32_with_oracle(sc, 9, (58, 18, 0, 2))
+    assert m["c_l1"] <= 1e-4 and m["load_l1"] <= 1e-4, m
+
+
 @pytest.mark.parametrize("tile", [(58, 18, 0, 2), (58, 10, 0, 2), (58, 26, 0, 1)],
                          ids=lambda t: f"{t[0]}x{t[1]}m{t[3]}")
 def test_fp32_deterministic(tile):
