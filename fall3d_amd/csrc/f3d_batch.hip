@@ -36,6 +36,14 @@ struct f3d_batch {
     std::vector<int> group;        // 1: on the other iiter parity than context 0
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // device work of an even / odd step
     bool use_graph = true;
+    // frame arguments of every context for a step in which context 0 takes
+    // an even / odd step, and the (context, member) rows of the frame launch
+    DeviceBuffer fdesc[2], frows;
+    int nfrows = 0;
+    // source arguments of every context (even / odd as above) and the first
+    // item of each context in the one setsrc launch
+    DeviceBuffer sdesc[2], sfirst;
+    int nsrc = 0;
 };
 
 namespace {
@@ -107,6 +115,32 @@ int replan(f3d_batch *b)
     step_lists(b, even, odd, f, nullptr);
     b->planned = false;
     TRY(f32::flux_batch_plan(b->plan, even, odd, f, b->group, b->ts));
+    const size_t n = b->ctx.size();
+    std::vector<int2> rows;
+    for (size_t k = 0; k < n; ++k)
+        for (int m = 0; m < b->ctx[k]->g.nm; ++m) rows.push_back(make_int2((int)k, m));
+    b->nfrows = (int)rows.size();
+    if (b->frows.alloc(sizeof(int2) * rows.size(), nullptr)) return F3D_ENOMEM;
+    F3D_HIP(hipMemcpy(b->frows.p, rows.data(), sizeof(int2) * rows.size(), hipMemcpyHostToDevice));
+    std::vector<int> first(n + 1, 0);
+    for (size_t k = 0; k < n; ++k) first[k + 1] = first[k] + b->ctx[k]->nsrcgrp * b->ctx[k]->g.nc;
+    b->nsrc = first[n];
+    if (b->sfirst.alloc(sizeof(int) * (n + 1), nullptr)) return F3D_ENOMEM;
+    F3D_HIP(hipMemcpy(b->sfirst.p, first.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    for (int o = 0; o < 2; ++o) {
+        std::vector<f32::FrameArgs> fa(n);
+        std::vector<f32::SrcArgs> sa(n);
+        for (size_t k = 0; k < n; ++k) {
+            const bool odd_k = b->group[k] ? !o : (o != 0);
+            fa[k] = fp32_frame_args(b->ctx[k], odd_k ? odd[k] : even[k]);
+            sa[k] = fp32_src_args(b->ctx[k], odd_k ? odd[k] : even[k]);
+        }
+        if (b->fdesc[o].alloc(sizeof(f32::FrameArgs) * n, nullptr) ||
+            b->sdesc[o].alloc(sizeof(f32::SrcArgs) * n, nullptr))
+            return F3D_ENOMEM;
+        F3D_HIP(hipMemcpy(b->fdesc[o].p, fa.data(), sizeof(f32::FrameArgs) * n, hipMemcpyHostToDevice));
+        F3D_HIP(hipMemcpy(b->sdesc[o].p, sa.data(), sizeof(f32::SrcArgs) * n, hipMemcpyHostToDevice));
+    }
     b->gen.resize(b->ctx.size());
     for (size_t k = 0; k < b->ctx.size(); ++k) b->gen[k] = b->ctx[k]->gen;
     b->planned = true;
@@ -132,21 +166,15 @@ bool plan_current(const f3d_batch *b)
 int record_step(f3d_batch *b, bool odd0)
 {
     hipStream_t s = b->stream;
-    std::vector<f32::StepArgs> args(b->ctx.size());
-    for (size_t k = 0; k < b->ctx.size(); ++k) {
-        const bool odd = b->group[k] ? !odd0 : odd0;
-        const int cur_in = odd ? b->cur_odd[k] : 1 - b->cur_odd[k];
-        args[k] = fp32_args(b->ctx[k], cur_in, odd ? 1 : 0);
-        TRY(fp32_setsrc(b->ctx[k], s, args[k]));
-    }
-    // frames (c_in -> c_out's ghost frame, disjoint from what the flux
-    // kernels touch) on the side stream after the edge lists
-    F3D_HIP(hipEventRecord(b->fork, s));
-    F3D_HIP(hipStreamWaitEvent(b->side, b->fork, 0));
+    const int o = odd0 ? 1 : 0;
+    // the sources of every context, then their frames (c_in -> c_out's
+    // ghost frame), one launch each, then the flux lists: a persistent flux
+    // launch holds every CU until its lists are drawn, so nothing runs well
+    // beside it
+    TRY(f32::setsrc_batch(s, b->sdesc[o].as<f32::SrcArgs>(), b->sfirst.as<int>(), (int)b->ctx.size(),
+                          b->nsrc));
+    TRY(f32::frame_batch(s, b->fdesc[o].as<f32::FrameArgs>(), b->frows.as<int2>(), b->nfrows));
     TRY(f32::flux_batch_step(s, b->side, b->fork, b->join, b->plan, odd0));
-    for (size_t k = 0; k < b->ctx.size(); ++k) TRY(fp32_frame(b->ctx[k], b->side, args[k]));
-    F3D_HIP(hipEventRecord(b->join, b->side));
-    F3D_HIP(hipStreamWaitEvent(s, b->join, 0));
     return F3D_OK;
 }
 

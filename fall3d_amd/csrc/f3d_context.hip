@@ -28,6 +28,15 @@ f32::StepArgs fp32_args(f3d_ctx *c, int cur_in, int iiter)
     return a;
 }
 
+f32::SrcArgs fp32_src_args(f3d_ctx *c, const f32::StepArgs &a)
+{
+    return f32::src_args(c->g, const_cast<float *>(a.cin), c->d_srcoff.as<int>(),
+                         c->d_srcgrp.as<int4>(), c->nsrcgrp, c->d_srcperm.as<int>(),
+                         c->d_isrc.as<int>(), c->d_jsrc.as<int>(), c->d_ksrc.as<int>(),
+                         c->d_tmrat.as<double>(), c->d_zlayer.as<double>(), c->d_dz.as<double>(),
+                         c->d_hm1.as<double>(), c->dt);
+}
+
 int fp32_setsrc(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a)
 {
     return f32::setsrc(s, c->g, const_cast<float *>(a.cin), c->d_srcoff.as<int>(),
@@ -35,6 +44,13 @@ int fp32_setsrc(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a)
                        c->d_isrc.as<int>(), c->d_jsrc.as<int>(), c->d_ksrc.as<int>(),
                        c->d_tmrat.as<double>(), c->d_zlayer.as<double>(), c->d_dz.as<double>(),
                        c->d_hm1.as<double>(), c->dt);
+}
+
+f32::FrameArgs fp32_frame_args(f3d_ctx *c, const f32::StepArgs &a)
+{
+    return f32::frame_args(a, fp32_fields(c), c->d_vx.as<double>(), c->d_vy.as<double>(),
+                           c->d_vz.as<double>(), c->d_vset.as<double>(), c->d_dz.as<double>(),
+                           c->d_scratch.as<double>(), c->d_outmass.as<double>());
 }
 
 int fp32_frame(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a)

@@ -122,25 +122,61 @@ struct FluxBatchPlan {
     int nint[2] = {0, 0}, nedge[2] = {0, 0};
     int off_int[2] = {0, 0}, off_edge[2] = {0, 0};
     int grid_int = 0, grid_edge = 0;  // persistent grid sizes (resident capacity)
+    bool mixed = false;  // one list of interior and edge items per group
 };
 int flux_batch_plan(FluxBatchPlan &b, const std::vector<StepArgs> &even,
                     const std::vector<StepArgs> &odd, const std::vector<Fields> &f,
                     const std::vector<int> &group, TileShape ts);
 // zero the counters; group 0 steps with parity odd0, group 1 with the other;
-// interior lists on s, edge lists on side (fork/join)
+// interior lists on s, edge lists on side (fork, then join unless the caller
+// joins the side stream itself)
 int flux_batch_step(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join,
-                    const FluxBatchPlan &b, bool odd0);
+                    const FluxBatchPlan &b, bool odd0, bool join_side = true);
 // Ghost frame of c_out (setbcc values of c_in, clipped) and cmass2d
 // outflux; outmass[m] += member m's outflux (deterministic per member).
 int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,
                       const double *vy64, const double *vz64, const double *vset64,
                       const double *dz, double *scratch, double *outmass);
+// the same for many contexts in two launches: fa[k] describes context k's
+// frame step, rows[y] = (context, member) for y < nrows; each member's
+// partials and tree are the ones frame_and_outflux uses
+struct FrameArgs {
+    Grid g;
+    const float *cin;
+    float *cout;
+    const float *bfac, *tfac;
+    const double *vx, *vy, *vz, *vset, *dz;
+    double dt;
+    double *part;     // kFramePartials per member
+    double *outmass;  // per member
+};
+FrameArgs frame_args(const StepArgs &a, const Fields &f, const double *vx64, const double *vy64,
+                     const double *vz64, const double *vset64, const double *dz, double *scratch,
+                     double *outmass);
+int frame_batch(hipStream_t s, const FrameArgs *fa, const int2 *rows, int nrows);
 // setsrc for every (member, class); off[m]..off[m+1] are member m's points,
 // grp/perm group them by target cell in point order
 int setsrc(hipStream_t s, const Grid &g, float *c, const int *off, const int4 *grp, int ngrp,
            const int *perm, const int *isrc, const int *jsrc, const int *ksrc,
            const double *tmrat, const double *zlayer, const double *dz, const double *hm1,
            double dt);
+// the same for many contexts in one launch: sa[k] describes context k's
+// sources, its items are [first[k], first[k+1]) of nitems = first[nctx]
+struct SrcArgs {
+    Grid g;
+    float *c;
+    const int *off;
+    const int4 *grp;
+    int ngrp;
+    const int *perm, *isrc, *jsrc, *ksrc;
+    const double *tmrat, *zlayer, *dz, *hm1;
+    double dt;
+};
+SrcArgs src_args(const Grid &g, float *c, const int *off, const int4 *grp, int ngrp,
+                 const int *perm, const int *isrc, const int *jsrc, const int *ksrc,
+                 const double *tmrat, const double *zlayer, const double *dz, const double *hm1,
+                 double dt);
+int setsrc_batch(hipStream_t s, const SrcArgs *sa, const int *first, int nctx, int nitems);
 int to_fp64(hipStream_t s, const float *c32, double *c64, long n);
 int from_fp64(hipStream_t s, const double *c64, float *c32, long n);
 
@@ -219,6 +255,8 @@ f32::Fields fp32_fields(const f3d_ctx *c);
 f32::StepArgs fp32_args(f3d_ctx *c, int cur_in, int iiter);
 int fp32_setsrc(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a);
 int fp32_frame(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a);
+f32::FrameArgs fp32_frame_args(f3d_ctx *c, const f32::StepArgs &a);
+f32::SrcArgs fp32_src_args(f3d_ctx *c, const f32::StepArgs &a);
 }  // namespace f3d
 
 // non-null and alive; sets the error message otherwise

@@ -922,8 +922,11 @@ __global__ void k_zero_counters(unsigned *c) { c[threadIdx.x] = 0u; }
 // Persistent form of k_flux over a work list: item = (scenario, bx, by,
 // slab group).  Each workgroup draws items from an atomic counter until the
 // list is exhausted; every item writes cells no other item writes, so the
-// result does not depend on which workgroup takes which item.
-template <int RY, int NW, int MC, bool ODD, bool EDGEK>
+// result does not depend on which workgroup takes which item.  LIST: 0
+// interior items only, 1 edge items only, 2 both (the tile's path is picked
+// per item).
+enum : int { L_INTERIOR = 0, L_EDGE = 1, L_MIXED = 2 };
+template <int RY, int NW, int MC, bool ODD, int LIST>
 __global__ void __launch_bounds__(NW * 64)
     k_flux_batch(const FluxParams *scen, const int4 *items, int nitems, unsigned *counter)
 {
@@ -961,11 +964,11 @@ __global__ void __launch_bounds__(NW * 64)
 #pragma unroll
         for (int c = 0; c < MC; ++c) qv[c] = grp * MC + c;
         const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * (RY - 6) - 3;
-        if (!EDGEK) {
+        const bool ex = gi0 < 1 || gi0 + RX - 1 > p.g.nx;
+        const bool ey = gj0 < 1 || gj0 + RY - 1 > p.g.ny;
+        if (LIST == L_INTERIOR || (LIST == L_MIXED && !ex && !ey)) {
             flux_tile<RY, NW, MC, ODD, false, false>(p, qv, gi0, gj0, sm, zt);
         } else {
-            const bool ex = gi0 < 1 || gi0 + RX - 1 > p.g.nx;
-            const bool ey = gj0 < 1 || gj0 + RY - 1 > p.g.ny;
             if (ex && ey)
                 flux_tile<RY, NW, MC, ODD, true, true>(p, qv, gi0, gj0, sm, zt);
             else if (ex)
@@ -977,11 +980,11 @@ __global__ void __launch_bounds__(NW * 64)
     }
 }
 
-template <int RY, int NW, int MC, bool EDGEK>
+template <int RY, int NW, int MC, int LIST>
 int batch_grid(int &grid)
 {
     int occ = 0;
-    F3D_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_flux_batch<RY, NW, MC, false, EDGEK>,
+    F3D_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_flux_batch<RY, NW, MC, false, LIST>,
                                                          NW * 64, 0));
     int dev = 0, ncu = 0;
     F3D_HIP(hipGetDevice(&dev));
@@ -990,8 +993,8 @@ int batch_grid(int &grid)
     return F3D_OK;
 }
 
-// parity group gr steps with parity odd: its edge list on se, interior list
-// on s
+// parity group gr steps with parity odd: one mixed list on s, or its edge
+// list on se and its interior list on s
 template <int RY, int NW, int MC>
 int batch_launch(hipStream_t s, hipStream_t se, const FluxBatchPlan &b, int gr, bool odd)
 {
@@ -1000,14 +1003,28 @@ int batch_launch(hipStream_t s, hipStream_t se, const FluxBatchPlan &b, int gr, 
     const int4 *items = b.items.as<int4>();
     unsigned *ctr = b.counter.as<unsigned>() + 2 * gr;
     const int ne = b.nedge[gr], ni = b.nint[gr];
+    if (b.mixed) {
+        if (ni > 0) {
+            const unsigned g = (unsigned)std::min(b.grid_int, ni);
+            const int4 *it = items + b.off_int[gr];
+            if (odd)
+                hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, true, L_MIXED>), dim3(g), dim3(NW * 64), 0, s,
+                                   par, it, ni, ctr);
+            else
+                hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, false, L_MIXED>), dim3(g), dim3(NW * 64), 0, s,
+                                   par, it, ni, ctr);
+            F3D_CHECK_LAUNCH();
+        }
+        return F3D_OK;
+    }
     if (ne > 0) {
         const unsigned g = (unsigned)std::min(b.grid_edge, ne);
         const int4 *it = items + b.off_edge[gr];
         if (odd)
-            hipLaunchKernelGGL((k_flux_batch<RY, NW, MCE, true, true>), dim3(g), dim3(NW * 64), 0, se,
+            hipLaunchKernelGGL((k_flux_batch<RY, NW, MCE, true, L_EDGE>), dim3(g), dim3(NW * 64), 0, se,
                                par, it, ne, ctr + 1);
         else
-            hipLaunchKernelGGL((k_flux_batch<RY, NW, MCE, false, true>), dim3(g), dim3(NW * 64), 0, se,
+            hipLaunchKernelGGL((k_flux_batch<RY, NW, MCE, false, L_EDGE>), dim3(g), dim3(NW * 64), 0, se,
                                par, it, ne, ctr + 1);
         F3D_CHECK_LAUNCH();
     }
@@ -1015,10 +1032,10 @@ int batch_launch(hipStream_t s, hipStream_t se, const FluxBatchPlan &b, int gr, 
         const unsigned g = (unsigned)std::min(b.grid_int, ni);
         const int4 *it = items + b.off_int[gr];
         if (odd)
-            hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, true, false>), dim3(g), dim3(NW * 64), 0, s,
+            hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, true, L_INTERIOR>), dim3(g), dim3(NW * 64), 0, s,
                                par, it, ni, ctr);
         else
-            hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, false, false>), dim3(g), dim3(NW * 64), 0, s,
+            hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, false, L_INTERIOR>), dim3(g), dim3(NW * 64), 0, s,
                                par, it, ni, ctr);
         F3D_CHECK_LAUNCH();
     }
@@ -1058,10 +1075,18 @@ int batch_plan_t(FluxBatchPlan &b, const std::vector<StepArgs> &ae, const std::v
         scenario_items((int)k, ae[k].g, RY, MC, MCE, iint[gr], iedge[gr]);
     }
     // deeper columns first (cost per item grows with nz); lists laid out as
-    // [group 0 interior | group 0 edge | group 1 interior | group 1 edge]
+    // [group 0 interior | group 0 edge | group 1 interior | group 1 edge].
+    // When interior and edge tiles take the same slab count, each group is
+    // one mixed list drawn by one persistent launch: two persistent launches
+    // side by side would split the device by arrival, not by work.
     auto deeper = [&](const int4 &x, const int4 &y) { return ae[x.x].g.nz > ae[y.x].g.nz; };
+    b.mixed = MCE == MC;
     std::vector<int4> all;
     for (int gr = 0; gr < 2; ++gr) {
+        if (b.mixed) {
+            iint[gr].insert(iint[gr].end(), iedge[gr].begin(), iedge[gr].end());
+            iedge[gr].clear();
+        }
         std::stable_sort(iint[gr].begin(), iint[gr].end(), deeper);
         std::stable_sort(iedge[gr].begin(), iedge[gr].end(), deeper);
         b.nint[gr] = (int)iint[gr].size();
@@ -1079,8 +1104,13 @@ int batch_plan_t(FluxBatchPlan &b, const std::vector<StepArgs> &ae, const std::v
     F3D_HIP(hipMemcpy(b.par[1].p, po.data(), sizeof(FluxParams) * n, hipMemcpyHostToDevice));
     if (!all.empty())
         F3D_HIP(hipMemcpy(b.items.p, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice));
-    TRY((batch_grid<RY, NW, MC, false>(b.grid_int)));
-    TRY((batch_grid<RY, NW, MCE, true>(b.grid_edge)));
+    if (b.mixed) {
+        TRY((batch_grid<RY, NW, MC, L_MIXED>(b.grid_int)));
+        b.grid_edge = 0;
+    } else {
+        TRY((batch_grid<RY, NW, MC, L_INTERIOR>(b.grid_int)));
+        TRY((batch_grid<RY, NW, MCE, L_EDGE>(b.grid_edge)));
+    }
     return F3D_OK;
 }
 
@@ -1107,7 +1137,7 @@ int flux_batch_plan(FluxBatchPlan &b, const std::vector<StepArgs> &even,
 }
 
 int flux_batch_step(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join,
-                    const FluxBatchPlan &b, bool odd0)
+                    const FluxBatchPlan &b, bool odd0, bool join_side)
 {
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(4), 0, s, b.counter.as<unsigned>());
     F3D_CHECK_LAUNCH();
@@ -1131,7 +1161,7 @@ int flux_batch_step(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t
             rc = mc == 1 ? batch_launch<32, 16, 1>(s, se, b, gr, odd) : batch_launch<32, 16, 2>(s, se, b, gr, odd);
         if (rc) return rc;
     }
-    if (split) {
+    if (split && join_side) {
         F3D_HIP(hipEventRecord(join, se));
         F3D_HIP(hipStreamWaitEvent(s, join, 0));
     }

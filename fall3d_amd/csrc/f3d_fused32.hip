@@ -488,20 +488,23 @@ __global__ void __launch_bounds__(kNT) k_fused(Params p)
 // ---------------- ghost frame + cmass2d outflux ----------------
 // setbcc (setbcc.f90:19-109) values of c_in written to c_out's frame, then
 // clipped; cmass2d terms (cmass2d.f90:68-127) of the clipped ghosts, FP64.
-__global__ void __launch_bounds__(256) k_frame(Grid g, const float *cin, float *cout, Fields f,
-                                               const double *vx, const double *vy,
-                                               const double *vz, const double *vset,
-                                               const double *dz, double dt, double *part)
+// Block bx of nbx for member mem: a grid-stride share of the frame cells of
+// every class; the block's outflux sum goes to part[mem*nbx + bx].
+__device__ __forceinline__ void frame_block(const FrameArgs &a, int bx, int nbx, int mem)
 {
     __shared__ double sh[256];
+    const Grid &g = a.g;
+    const float *cin = a.cin;
+    float *cout = a.cout;
+    const double *vx = a.vx, *vy = a.vy, *vz = a.vz, *vset = a.vset, *dz = a.dz;
+    const double dt = a.dt;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long nxf = (long)ny * nz, nyf = (long)nx * nz, nxy = g.NXY;
     const long nedge = 4L * nz + 4L * ny + 4L * nx + 8;
     const long per = nxf + nyf + nxy + nedge;
     const long total = per * g.nc;
-    const int mem = blockIdx.y;  // member: its own partials, its own outflux row
     double acc = 0.0;
-    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < total; t += (long)gridDim.x * 256) {
+    for (long t = (long)bx * 256 + threadIdx.x; t < total; t += (long)nbx * 256) {
         const int q = (int)(t / per);  // class
         long r = t - (long)q * per;
         const float *ci = cin + ((long)mem * g.nc + q) * g.G;
@@ -534,8 +537,8 @@ __global__ void __launch_bounds__(256) k_frame(Grid g, const float *cin, float *
         r -= nyf;
         if (r < nxy) {
             const int i = (int)(r % nx) + 1, j = (int)(r / nx) + 1;
-            const float gb = fmaxf(0.f, ci[cidx(nx, ny, i, j, 1)] * f.bfac[(long)q * nxy + r]);
-            const float gt = fmaxf(0.f, f.tfac[(long)q * nxy + r] * ci[cidx(nx, ny, i, j, nz)]);
+            const float gb = fmaxf(0.f, ci[cidx(nx, ny, i, j, 1)] * a.bfac[(long)q * nxy + r]);
+            const float gt = fmaxf(0.f, a.tfac[(long)q * nxy + r] * ci[cidx(nx, ny, i, j, nz)]);
             co[cidx(nx, ny, i, j, 0)] = gb;
             co[cidx(nx, ny, i, j, nz + 1)] = gt;
             const double fatx = (i == 1 || i == nx) ? 0.5 : 1.0;
@@ -580,7 +583,29 @@ __global__ void __launch_bounds__(256) k_frame(Grid g, const float *cin, float *
         if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[(long)mem * gridDim.x + blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) a.part[(long)mem * nbx + bx] = sh[0];
+}
+
+__global__ void __launch_bounds__(256) k_frame(FrameArgs a)
+{
+    frame_block(a, blockIdx.x, gridDim.x, blockIdx.y);  // y: member
+}
+
+// the frames of many contexts: row y of the grid is (context, member)
+__global__ void __launch_bounds__(256) k_frame_batch(const FrameArgs *fa, const int2 *rows)
+{
+    const int2 r = rows[blockIdx.y];
+    frame_block(fa[r.x], blockIdx.x, gridDim.x, r.y);
+}
+
+// outmass[member] += the sum of its partials, for many contexts (the tree
+// of red::finish_rows, row by row)
+__global__ void __launch_bounds__(1024) k_finish_batch(const FrameArgs *fa, const int2 *rows)
+{
+    const int2 r = rows[blockIdx.x];
+    const FrameArgs &a = fa[r.x];
+    red::block_sum_row(a.part + (long)r.y * red::kFramePartials, red::kFramePartials,
+                       a.outmass + r.y, 1);
 }
 
 // FP32 step fields from the FP64 coefficients
@@ -687,13 +712,15 @@ __global__ void k_fields_class(Grid g, const double *vz, const double *vset, con
 // target cell (grp[g] = {member, first, count} into perm, in point order):
 // one thread per (group, class) adds its points in the reference order, so
 // every cell sees the same sequence of roundings as the serial loop.
-__global__ void k_setsrc32(Grid g, float *c, const int *off, const int4 *grp, int ngrp,
-                           const int *perm, const int *isrc, const int *jsrc, const int *ksrc,
-                           const double *tmrat, const double *zlayer, const double *dz,
-                           const double *hm1, double dt)
+// item t = (cell group, class) of one context's sources
+__device__ __forceinline__ void setsrc_item(const SrcArgs &a, int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ngrp * g.nc) return;
+    const Grid &g = a.g;
+    float *c = a.c;
+    const int *off = a.off, *perm = a.perm, *isrc = a.isrc, *jsrc = a.jsrc, *ksrc = a.ksrc;
+    const int4 *grp = a.grp;
+    const double *tmrat = a.tmrat, *zlayer = a.zlayer, *dz = a.dz, *hm1 = a.hm1;
+    const double dt = a.dt;
     const int q = t % g.nc;
     const int4 gr = grp[t / g.nc];
     const int mem = gr.x;
@@ -712,6 +739,27 @@ __global__ void k_setsrc32(Grid g, float *c, const int *off, const int4 *grp, in
         v = (float)((double)v + dt * tm[is + (long)nsou * q] / vol0);
     }
     *cc = v;
+}
+
+__global__ void k_setsrc32(SrcArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ngrp * a.g.nc) return;
+    setsrc_item(a, t);
+}
+
+// the sources of many contexts: items [first[k], first[k+1]) are context k's
+__global__ void k_setsrc_batch(const SrcArgs *sa, const int *first, int nctx)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= first[nctx]) return;
+    int lo = 0, hi = nctx - 1;  // the context whose range holds t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (first[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    setsrc_item(sa[lo], t - first[lo]);
 }
 
 __global__ void k_to64(const float *a, double *b, long n)
@@ -855,14 +903,68 @@ int fused_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
     return F3D_EINVAL;
 }
 
+FrameArgs frame_args(const StepArgs &a, const Fields &f, const double *vx64, const double *vy64,
+                     const double *vz64, const double *vset64, const double *dz, double *scratch,
+                     double *outmass)
+{
+    FrameArgs fa;
+    fa.g = a.g;
+    fa.cin = a.cin;
+    fa.cout = a.cout;
+    fa.bfac = f.bfac;
+    fa.tfac = f.tfac;
+    fa.vx = vx64;
+    fa.vy = vy64;
+    fa.vz = vz64;
+    fa.vset = vset64;
+    fa.dz = dz;
+    fa.dt = a.dt;
+    fa.part = scratch;
+    fa.outmass = outmass;
+    return fa;
+}
+
 int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,
                       const double *vy64, const double *vz64, const double *vset64,
                       const double *dz, double *scratch, double *outmass)
 {
-    hipLaunchKernelGGL(k_frame, dim3(red::kFramePartials, a.g.nm), dim3(256), 0, s, a.g, a.cin,
-                       a.cout, f, vx64, vy64, vz64, vset64, dz, a.dt, scratch);
+    const FrameArgs fa = frame_args(a, f, vx64, vy64, vz64, vset64, dz, scratch, outmass);
+    hipLaunchKernelGGL(k_frame, dim3(red::kFramePartials, a.g.nm), dim3(256), 0, s, fa);
     F3D_CHECK_LAUNCH();
     return red::finish_rows(s, scratch, red::kFramePartials, a.g.nm, outmass, 1);
+}
+
+int frame_batch(hipStream_t s, const FrameArgs *fa, const int2 *rows, int nrows)
+{
+    if (nrows <= 0) return F3D_OK;
+    hipLaunchKernelGGL(k_frame_batch, dim3(red::kFramePartials, nrows), dim3(256), 0, s, fa, rows);
+    F3D_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_finish_batch, dim3(nrows), dim3(1024), 0, s, fa, rows);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+SrcArgs src_args(const Grid &g, float *c, const int *off, const int4 *grp, int ngrp,
+                 const int *perm, const int *isrc, const int *jsrc, const int *ksrc,
+                 const double *tmrat, const double *zlayer, const double *dz, const double *hm1,
+                 double dt)
+{
+    SrcArgs a;
+    a.g = g;
+    a.c = c;
+    a.off = off;
+    a.grp = grp;
+    a.ngrp = ngrp;
+    a.perm = perm;
+    a.isrc = isrc;
+    a.jsrc = jsrc;
+    a.ksrc = ksrc;
+    a.tmrat = tmrat;
+    a.zlayer = zlayer;
+    a.dz = dz;
+    a.hm1 = hm1;
+    a.dt = dt;
+    return a;
 }
 
 int setsrc(hipStream_t s, const Grid &g, float *c, const int *off, const int4 *grp, int ngrp,
@@ -872,8 +974,16 @@ int setsrc(hipStream_t s, const Grid &g, float *c, const int *off, const int4 *g
 {
     const int n = ngrp * g.nc;
     if (n == 0) return F3D_OK;
-    hipLaunchKernelGGL(k_setsrc32, dim3((n + 255) / 256), dim3(256), 0, s, g, c, off, grp, ngrp,
-                       perm, isrc, jsrc, ksrc, tmrat, zlayer, dz, hm1, dt);
+    const SrcArgs a = src_args(g, c, off, grp, ngrp, perm, isrc, jsrc, ksrc, tmrat, zlayer, dz, hm1, dt);
+    hipLaunchKernelGGL(k_setsrc32, dim3((n + 255) / 256), dim3(256), 0, s, a);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+int setsrc_batch(hipStream_t s, const SrcArgs *sa, const int *first, int nctx, int nitems)
+{
+    if (nitems <= 0) return F3D_OK;
+    hipLaunchKernelGGL(k_setsrc_batch, dim3((nitems + 255) / 256), dim3(256), 0, s, sa, first, nctx);
     F3D_CHECK_LAUNCH();
     return F3D_OK;
 }

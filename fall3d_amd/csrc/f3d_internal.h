@@ -116,7 +116,7 @@ int cmass2d64(hipStream_t s, const Grid &g, int ncls, const double *c, const dou
 // per-member partial counts: a member's reduction tree depends only on its
 // own sizes, never on how many members share the launch
 constexpr int kMemberPartials = 256;
-constexpr int kFramePartials = 1024;
+constexpr int kFramePartials = 256;
 // getc0 (endstep.f90:80-106) for every member: slot 0 of cdry/cwet = sum of
 // the class slots; member m's deposit is cdry + m*(nc+1)*NXY
 int getc0(hipStream_t s, const Grid &g, double *cdry, double *cwet);
@@ -134,6 +134,21 @@ int cmass3d(hipStream_t s, const Grid &g, const float *c, const double *cload0,
 // final stage: dst[r] (+)= sum(partials[r*n .. r*n+n)) for r < nrows, fixed tree
 int finish_rows(hipStream_t s, const double *partials, int n, int nrows, double *dst,
                 int accumulate);
+// the tree of finish_rows for one row, by a block of 1024 threads: *dst (+)=
+// sum of row[0 .. n)
+__device__ __forceinline__ void block_sum_row(const double *row, int n, double *dst, int accumulate)
+{
+    __shared__ double sh[1024];
+    double v = 0.0;
+    for (int t = threadIdx.x; t < n; t += 1024) v += row[t];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *dst = accumulate ? *dst + sh[0] : sh[0];
+}
 inline int finish_sum(hipStream_t s, const double *partials, int n, double *dst, int accumulate)
 {
     return finish_rows(s, partials, n, 1, dst, accumulate);

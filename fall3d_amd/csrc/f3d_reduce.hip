@@ -143,17 +143,8 @@ __global__ void __launch_bounds__(kBS) k_cmass3d(Grid g, const T *c, const doubl
 __global__ void __launch_bounds__(1024) k_finish(const double *part, int n, double *dst,
                                                  int accumulate)
 {
-    __shared__ double sh[1024];
     const int r = blockIdx.x;
-    double v = 0.0;
-    for (int t = threadIdx.x; t < n; t += 1024) v += part[(long)r * n + t];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) dst[r] = accumulate ? dst[r] + sh[0] : sh[0];
+    red::block_sum_row(part + (long)r * n, n, dst + r, accumulate);
 }
 
 }  // namespace
