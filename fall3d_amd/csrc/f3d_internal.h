@@ -116,7 +116,7 @@ int cmass2d64(hipStream_t s, const Grid &g, int ncls, const double *c, const dou
 // per-member partial counts: a member's reduction tree depends only on its
 // own sizes, never on how many members share the launch
 constexpr int kMemberPartials = 256;
-constexpr int kFramePartials = 256;
+constexpr int kFramePartials = 1024;
 // getc0 (endstep.f90:80-106) for every member: slot 0 of cdry/cwet = sum of
 // the class slots; member m's deposit is cdry + m*(nc+1)*NXY
 int getc0(hipStream_t s, const Grid &g, double *cdry, double *cwet);
