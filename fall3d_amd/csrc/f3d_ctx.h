@@ -59,13 +59,12 @@ struct Fields {
     const float *bfac;          // bottom ghost factor (nx,ny,nc): 0, or 1+dz0*vdep/rkver(1)
     const float *tfac;          // top ghost factor (nx,ny,nc): 0 or 1 (setbcc.f90:126-130)
     const double *vsum1;        // min(0, vset(:,:,1)+vdep) (nx,ny,nc), FP64 deposit factor
-    // class-independent face coefficients (dt-dependent), see k_fields_face
-    const float *ax, *fx, *ay, *fy, *dfx, *dfy;
-    // the same, interleaved per cell for the flux kernel: one load per cell
-    // for each group
-    const float4 *cA;  // (ax, fx, ay, fy)
-    const float *cD;   // (dfx, 1/rho, rho*rkver): 12-byte records
-    const float2 *cE;  // (rho*rkh1, rho*rkhor)
+    // Courant numbers dt/dx*vx, dt/dy*vy (dt-dependent), see k_fields_face
+    const float *ax, *ay;
+    // interleaved per cell for the flux kernel: one load per cell for each
+    // group
+    const float2 *cA;  // (ax, ay)
+    const float4 *cD;  // (rho*rkh1, 1/rho, rho*rkver, rho*rkhor)
     const ZLev *zl;    // per level 0..nz+1
     const float *base;  // the buffer all FP32 fields live in, and its size
     size_t bytes;

@@ -50,14 +50,16 @@ struct FluxParams {
     float *cout;
     double *cdry;
     // all FP32 fields live in one buffer; byte offsets of the groups in it:
-    // cA (ax, fx, ay, fy), cD (dfx, 1/rho, rho*rkver: 12-byte records), dfy,
-    // cE (rho*rkh1, rho*rkhor: raw products for line ends), wz (vz+vset,
-    // class 0), bfac and tfac (class 0).  The kernel reads each record whole,
-    // once per level: a partial read of a cache line would fetch the line
-    // again for the rest.
+    // cA (ax, ay: 8-byte records), cD (rho*rkh1, 1/rho, rho*rkver,
+    // rho*rkhor), wz (vz+vset, class 0), bfac and tfac (class 0).  The face
+    // coefficients (limiter factors, face-averaged diffusivities) are formed
+    // from the two cells on either side of the face, which the kernel holds
+    // anyway, so no per-face field is read.  The kernel reads each record
+    // whole, once per level: a partial read of a cache line would fetch the
+    // line again for the rest.
     const float *fbase;
     unsigned fbytes;
-    unsigned oA, oD, oY, oE, oW, oBF, oTF;
+    unsigned oA, oD, oW, oBF, oTF;
     const double *vsum1;
     const ZLev *zl;
     double dt64;
@@ -80,6 +82,17 @@ __device__ __forceinline__ float from_right(float x)
 }
 
 __device__ __forceinline__ float minmod(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 0.f); }
+
+// Limiter factor of the face between two cells with Courant numbers A and
+// Am = dt/h*v (advctx.f90:33-43): with a = 0.5*|A + Am| the Courant number
+// of the face velocity, the second-order flux is 0.5*a*(1-a) times the
+// minmod of the two differences.  At the line start the reference takes
+// Am = A (v(max(1,i-1))), which the clamped column of the ghost supplies.
+__device__ __forceinline__ float face_factor(float A, float Am)
+{
+    const float a = 0.5f * fabsf(A + Am);
+    return 0.5f * a * (1.f - a);
+}
 
 // advctx on one region row.  EX: lo/hi mark the first/last domain column,
 // where the reference clamps W2 to the line (advctx.f90:50-52) and A to
@@ -160,10 +173,15 @@ __device__ __forceinline__ float4 bld4(rsrc_t r, unsigned voff, unsigned soff)
     return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
                        __uint_as_float(v.w));
 }
-// c_out is written once per step and read only by the next one: non-temporal
-__device__ __forceinline__ void bst(float x, rsrc_t r, unsigned voff, unsigned soff)
+// c_out is written once per step and read only by the next one: non-temporal.
+// The whole offset is in the VGPR: a lane that must not write carries
+// kNoStore, which stays past the buffer's range whatever level offset is
+// added (slabs are under 2 GiB), so the range check drops the store and no
+// branch is needed around it.
+constexpr unsigned kNoStore = 0x80000000u;
+__device__ __forceinline__ void bst(float x, rsrc_t r, unsigned voff)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, (int)voff, (int)soff, kNonTemporal);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, (int)voff, 0, kNonTemporal);
 }
 
 // per-row flags of edge tiles
@@ -174,6 +192,10 @@ enum : unsigned {
     F_GX = 64u,     // ... on an x face (else a y face)
     F_GLO = 128u,   // ... on the low face (i = 0 or j = 0)
 };
+
+// LDS planes of one tile: SP, SF, SQ (one per slab), SA, SY and, on odd
+// steps, the two SC planes
+constexpr int flux_planes(int mc, bool odd) { return 3 * mc + 2 + (odd ? 2 : 0); }
 
 template <int RY, int NW, int MC, bool ODD, bool EX, bool EY>
 struct FluxTile {
@@ -189,9 +211,15 @@ struct FluxTile {
     int nz, li, w;
     unsigned NXY, SZ;
     float *SP, *SF, *SQ;  // MC planes each
-    // odd steps: (dfx, 1/rho) of the level awaiting its last sweep, own rows
-    // only (kept in LDS rather than registers, which would keep the loaded
-    // record live into a third level)
+    // ay of each band's top row (the face below the next band's first row),
+    // and rho*rkhor of the level awaiting its last sweep (every row: held in
+    // a register, the lane would outlive the rest of its record and be
+    // copied out of the next load into the same registers, which waits for
+    // that load)
+    float *SA, *SY;
+    // odd steps: (rho*rkh1, 1/rho) of the level awaiting its last sweep, own
+    // rows only (kept in LDS rather than registers, which would keep the
+    // loaded record live into a third level)
     float *SC;
     rsrc_t rf;            // the field buffer
     const ZLev *zt;       // the z-level table, staged in LDS
@@ -206,26 +234,18 @@ struct FluxTile {
 
     // per-row geometry: byte offsets of the (clamped) column in c, in the
     // meteo arrays and of the rows above/below; edge flags
-    unsigned co4[NS], m4[NS], m12[NS], m16[NS], mup4[NS], mdn8[NS], mup8[NS], fl[NS];
+    unsigned co4[NS], so4[NS], m4[NS], m8[NS], m16[NS], fl[NS];
     int lj[NS];
-    // Class-independent coefficients: (ax, fx, ay, fy) of the current and
-    // the next level, (dfx, lr, pz) of the current level and the one below.
+    // Class-independent coefficients: (ax, ay) of the current and the next
+    // level, (pkx, lr, pz, pky) of the current level and the one below.
     // Every load fills a register after the last use of the value it
     // replaces, every loaded lane is used (a dead lane is written back late
     // and makes its register's next use wait), and each value is loaded a
     // level before it is needed.  gv: wind at the face cell of a ghost (edge
     // tiles), one level ahead.
-    float4 CA[NS], CAn[NS];
-    float3 Dn[NS], D0[NS];
+    float2 CA[NS], CAn[NS];
+    float4 Dn[NS], D0[NS];
     float gv[NS];
-    // dfy of the level in the last sweep (own rows, and the row above the
-    // band) and of the level after it
-    float dyl[NS], dyu[NS], dyl1[NS], dyu1[NS];
-    // carried between phase 3 and the next phase 2 (class-independent)
-    float pyo[NS], pym[NS], pyp[NS];
-    // x-side tiles: rho*rkh1 of the level awaiting its diffusion, loaded a
-    // level ahead (the line-end diffx fluxes)
-    float pkk[NS];
     // per class: advctzc window c(s), c(s+1), w(s), U(s), W2(s), delta(s) (after
     // level nz, c0 holds the top ghost c(nz+1)), prefetched c(s+2), w(s+1);
     // diffz window b(k), q(k) = b/rho and the face flux it leaves for the
@@ -256,7 +276,9 @@ struct FluxTile {
         SP = sm;
         SF = sm + MC * PL;
         SQ = sm + 2 * MC * PL;
-        SC = sm + 3 * MC * PL;
+        SA = sm + 3 * MC * PL;
+        SY = SA + PL;
+        SC = SY + PL;
         const size_t G = (size_t)p.g.G, N = (size_t)p.g.N;
         rf = mkbuf(p.fbase, p.fbytes);
         const int nq = nc * p.g.nm;
@@ -267,7 +289,8 @@ struct FluxTile {
             icv[c] = q % nc;
             memv[c] = q / nc;
             rin[c] = mkbuf(p.cin + (size_t)q * G, 4 * G);
-            rout[c] = mkbuf(p.cout + (size_t)q * G, 4 * G);
+            // the duplicate slab of an odd class count writes nothing
+            rout[c] = mkbuf(p.cout + (size_t)q * G, live[c] ? 4 * G : 0);
             oW[c] = p.oW + 4u * (unsigned)(icv[c] * N);
             oBF[c] = p.oBF + 4u * (unsigned)icv[c] * NXY;
             oTF[c] = p.oTF + 4u * (unsigned)icv[c] * NXY;
@@ -282,13 +305,8 @@ struct FluxTile {
             const unsigned mo = (unsigned)((ci - 1) + nx * (cj - 1));
             co4[sl] = 4u * (unsigned)cidx(nx, ny, ci, cj, 0);
             m4[sl] = 4u * mo;
-            m12[sl] = 12u * mo;
+            m8[sl] = 8u * mo;
             m16[sl] = 16u * mo;
-            const unsigned mu = (unsigned)((ci - 1) + nx * (min(cj + 1, ny) - 1));
-            const unsigned md = (unsigned)((ci - 1) + nx * (max(cj - 1, 1) - 1));
-            mup4[sl] = 4u * mu;
-            mup8[sl] = 8u * mu;
-            mdn8[sl] = 8u * md;
             unsigned f = 0;
             const bool inx = !EX || (gi >= 1 && gi <= nx);
             const bool iny = !EY || (gj >= 1 && gj <= ny);
@@ -302,6 +320,8 @@ struct FluxTile {
             if (EX && iny && (gi == 0 || gi == nx + 1)) f |= F_GHOST | F_GX | (gi == 0 ? F_GLO : 0u);
             if (EY && inx && (gj == 0 || gj == ny + 1)) f |= F_GHOST | (gj == 0 ? F_GLO : 0u);
             fl[sl] = f;
+            // output cells: the tile's columns, inside the domain
+            so4[sl] = (li >= 3 && li < RX - 3 && (f & F_INNER)) ? co4[sl] : kNoStore;
         }
     }
 
@@ -319,14 +339,11 @@ struct FluxTile {
         dtop = p.dt * zt[nz].ldz;
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
-            CA[sl] = bld4(rf, m16[sl], p.oA);  // level 1
-            CAn[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
-            Dn[sl] = bld3(rf, m12[sl], p.oD);
-            D0[sl] = make_float3(0.f, 0.f, 0.f);
-            dyl1[sl] = dyu1[sl] = 0.f;
-            gv[sl] = EDGE ? bld(rf, m16[sl] + (has(sl, F_GX) ? 0u : 8u), p.oA) : 0.f;
-            dyl[sl] = dyu[sl] = pyo[sl] = pym[sl] = pyp[sl] = 0.f;
-            pkk[sl] = 0.f;
+            CA[sl] = bld2(rf, m8[sl], p.oA);  // level 1
+            CAn[sl] = make_float2(0.f, 0.f);
+            Dn[sl] = bld4(rf, m16[sl], p.oD);
+            D0[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
+            gv[sl] = EDGE ? bld(rf, m8[sl] + (has(sl, F_GX) ? 0u : 4u), p.oA) : 0.f;
 #pragma unroll
             for (int c = 0; c < MC; ++c) {
                 const float c1 = bld(rin[c], co4[sl], 4 * SZ);
@@ -379,6 +396,8 @@ struct FluxTile {
             }
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
+                const float Ax = CA[sl].x;
+                const float Fx = ODD ? 0.f : face_factor(Ax, from_left(Ax));
                 bool zero = false;
                 if (EDGE) {
                     // ghost faces: zero when the wind at the face cell blows
@@ -411,7 +430,7 @@ struct FluxTile {
                     Uc[c][sl] = Un;
                     W2k[c][sl] = W2n;
                     dprev[c][sl] = dn;
-                    if (!ODD) a = adv_x<EX>(a, CA[sl].x, CA[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
+                    if (!ODD) a = adv_x<EX>(a, Ax, Fx, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
                     P[c][sl] = a;
                 }
             }
@@ -420,6 +439,7 @@ struct FluxTile {
                 if (sl == 0 || sl == NS - 1)
 #pragma unroll
                     for (int c = 0; c < MC; ++c) *row(SP, c, lj[sl]) = P[c][sl];
+            *row(SA, 0, lj[NS - 1]) = CA[NS - 1].y;
             // Reloads go after the last use of the registers they refill: a
             // load issued while the old value is still live lands in another
             // register and is copied back, and the copy waits for the load.
@@ -427,7 +447,7 @@ struct FluxTile {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 if (EDGE && (STEADY || s + 1 <= nzl))
-                    gv[sl] = bld(rf, m16[sl] + (has(sl, F_GX) ? 0u : 8u), p.oA + 4u * lz);
+                    gv[sl] = bld(rf, m8[sl] + (has(sl, F_GX) ? 0u : 4u), p.oA + 2u * lz);
 #pragma unroll
                 for (int c = 0; c < MC; ++c) {
                     if (PIPE)
@@ -436,7 +456,7 @@ struct FluxTile {
                         nc2[c][sl] = bldnt(rin[c], co4[sl], 4u * SZ * (unsigned)(s + 3));
                     if (STEADY || s + 2 <= nzl) nwz[c][sl] = bld(rf, m4[sl], oW[c] + lz + 4u * NXY);
                 }
-                if (STEADY || s + 1 <= nzl) CAn[sl] = bld4(rf, m16[sl], p.oA + 4u * lz);
+                if (STEADY || s + 1 <= nzl) CAn[sl] = bld2(rf, m8[sl], p.oA + 2u * lz);
             }
         }
         __syncthreads();
@@ -446,13 +466,12 @@ struct FluxTile {
         // Every row is computed (no wave-uniform skips, so no loop-carried
         // value is assigned under a branch); the region's first and last
         // rows read a clamped neighbour and their results are never used.
-        float ePm[MC], ePp[MC], eqm[MC], eqp[MC], edx[NS], elr[NS], epk[NS];
+        float ePm[MC], ePp[MC], eqm[MC], eqp[MC], elr[NS], epk[NS];
         if (ODD) {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
-                edx[sl] = *row(SC, 0, lj[sl]);
+                epk[sl] = *row(SC, 0, lj[sl]);
                 elr[sl] = *row(SC, 1, lj[sl]);
-                epk[sl] = EX ? *row(SC, 2, lj[sl]) : 0.f;
             }
         }
 #pragma unroll
@@ -462,20 +481,32 @@ struct FluxTile {
             eqm[c] = *row(SQ, c, max(lj[0] - 1, 0));
             eqp[c] = *row(SQ, c, min(lj[NS - 1] + 1, RY - 1));
         }
+        // ay of the row below the band (the region's first row has none and
+        // pairs with itself: its fluxes are never used), rho*rkhor of the
+        // rows below and above the band
+        const float eay = *row(SA, 0, max(lj[0] - 1, 0));
+        const float eym = *row(SY, 0, max(lj[0] - 1, 0));
+        const float eyp = *row(SY, 0, min(lj[NS - 1] + 1, RY - 1));
+        float pyk[NS];
+#pragma unroll
+        for (int sl = 0; sl < NS; ++sl) pyk[sl] = *row(SY, 0, lj[sl]);
         if (STEADY || s <= nzl) {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
+                const float ay = CA[sl].y;
+                const float aym = (sl > 0) ? CA[sl - 1].y : (lj[0] > 0 ? eay : ay);
+                const float Fy = face_factor(ay, aym);
 #pragma unroll
                 for (int c = 0; c < MC; ++c) {
                     const float Pm = (sl > 0) ? P[c][sl - 1] : ePm[c];
                     const float Pp = (sl < NS - 1) ? P[c][sl + 1] : ePp[c];
-                    const float w2 = CA[sl].w * minmod(Pp - P[c][sl], P[c][sl] - Pm);
-                    phi[c][sl] = fmaf(CA[sl].z, P[c][sl], w2);
+                    const float w2 = Fy * minmod(Pp - P[c][sl], P[c][sl] - Pm);
+                    phi[c][sl] = fmaf(ay, P[c][sl], w2);
                     if (EY) {
                         // advcty.f90:49-51: W2 clamps to the line at j = 1 / ny,
                         // A at the ghost row is v(1) / v(ny)
-                        apl[c][sl] = fmaf(CA[sl].z, Pm, w2);
-                        apr[c][sl] = fmaf(CA[sl].z, Pp, w2);
+                        apl[c][sl] = fmaf(ay, Pm, w2);
+                        apr[c][sl] = fmaf(ay, Pp, w2);
                     }
                     if (sl == 0 || sl == NS - 1) *row(SF, c, lj[sl]) = phi[c][sl];
                 }
@@ -483,34 +514,39 @@ struct FluxTile {
         }
         if (STEADY || s >= 3) {
             const int k = s - 2;
+            const float d2 = p.dtdy2, hd2 = 0.5f * p.dtdy2;
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 if (lj[sl] < 3 || lj[sl] >= RY - 3) continue;  // wave-uniform
-                const float Du = (sl < NS - 1) ? dyl[sl + 1] : dyu[sl];
+                // face-averaged rho*rkhor times dt/dy^2 below and above the row
+                const float pm = (sl > 0) ? pyk[sl - 1] : eym;
+                const float pp = (sl < NS - 1) ? pyk[sl + 1] : eyp;
+                const float Dl = hd2 * (pyk[sl] + pm);
+                const float Du = hd2 * (pp + pyk[sl]);
+                const float Dx = ODD ? 0.5f * p.dtdx2 * (epk[sl] + from_left(epk[sl])) : 0.f;
 #pragma unroll
                 for (int c = 0; c < MC; ++c) {
                     // diffy (diffy.f90:30-44; line ends 20-29, 45-53)
                     const float qm = (sl > 0) ? qc[c][sl - 1] : eqm[c];
                     const float qp = (sl < NS - 1) ? qc[c][sl + 1] : eqp[c];
                     const float q = qc[c][sl];
-                    float glo = dyl[sl] * (q - qm);
+                    float glo = Dl * (q - qm);
                     float ghi = Du * (qp - q);
                     if (EY) {
-                        const float d2 = p.dtdy2;
                         const bool lo = has(sl, F_YLO), hi = has(sl, F_YHI);
-                        glo = lo ? fmaxf(0.f, d2 * pyo[sl] * (q - qm)) : (hi ? d2 * pym[sl] * (q - qm) : glo);
-                        ghi = lo ? d2 * pyp[sl] * (qp - q) : (hi ? fminf(0.f, d2 * pyo[sl] * (qp - q)) : ghi);
+                        glo = lo ? fmaxf(0.f, d2 * pyk[sl] * (q - qm)) : (hi ? d2 * pm * (q - qm) : glo);
+                        ghi = lo ? d2 * pp * (qp - q) : (hi ? fminf(0.f, d2 * pyk[sl] * (qp - q)) : ghi);
                     }
                     float out = dc[c][sl] + (ghi - glo);
                     if (EDGE) out = inner(sl) ? out : dc[c][sl];
                     if (ODD)
-                        out = diff_x<EX>(out, elr[sl], edx[sl], epk[sl], p.dtdx2, inner(sl),
+                        out = diff_x<EX>(out, elr[sl], Dx, epk[sl], p.dtdx2, inner(sl),
                                          has(sl, F_XLO), has(sl, F_XHI));
                     // sizloop.f90:77-78; interior tiles store in flush()
                     if (PIPE)
                         sto[c][sl] = fmaxf(0.f, out);
-                    else if (stored(sl, c))
-                        bst(fmaxf(0.f, out), rout[c], co4[sl], 4u * SZ * (unsigned)k);
+                    else
+                        bst(fmaxf(0.f, out), rout[c], so4[sl] + 4u * SZ * (unsigned)k);
                     if (!STEADY && k == 1 && stored(sl, c)) {
                         // accum.f90:22: deposit from c(:,:,1) before the clip, FP64
                         const size_t t2 = m4[sl] / 4u;
@@ -518,19 +554,6 @@ struct FluxTile {
                         *cd = *cd - p.dt64 * (double)out * p.vsum1[(size_t)icv[c] * NXY + t2];
                     }
                 }
-            }
-        }
-        // dfy: the level just swept is done with; level s is loaded for the
-        // sweep two levels on
-        if (STEADY) __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int sl = 0; sl < NS; ++sl) {
-            dyl[sl] = dyl1[sl];
-            if (sl == NS - 1) dyu[sl] = dyu1[sl];
-            if (STEADY || s <= nzl) {
-                const unsigned ly = p.oY + 4u * NXY * (unsigned)(s - 1);
-                dyl1[sl] = bld(rf, m4[sl], ly);
-                if (sl == NS - 1) dyu1[sl] = bld(rf, mup4[sl], ly);
             }
         }
         __syncthreads();
@@ -543,7 +566,6 @@ struct FluxTile {
         }
         const int k = s - 1;
         const ZLev zk = zt[k > 0 ? k : 0];
-        const unsigned lk = 4u * NXY * (unsigned)(k > 0 ? k - 1 : 0);  // level k in a 3-D field
         // band-edge fluxes first; every row is computed, as in phase 2
         float epl[MC], epr[MC];
 #pragma unroll
@@ -554,9 +576,11 @@ struct FluxTile {
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
             const float lrn = Dn[sl].y, pzn = Dn[sl].z;  // level s (unused at s = nz+1)
-            const float3 Dk = D0[sl];                    // level k (unused at s = 1)
+            const float4 Dk = D0[sl];                    // level k (unused at s = 1)
             const float lr0 = Dk.y, pz0 = Dk.z;
-            const float ay = CA[sl].z;
+            const float ay = CA[sl].y;
+            const float Ax = CA[sl].x;
+            const float Fx = ODD ? face_factor(Ax, from_left(Ax)) : 0.f;
 #pragma unroll
             for (int c = 0; c < MC; ++c) {
                 float bnew;
@@ -571,7 +595,7 @@ struct FluxTile {
                     bnew = (ay > 0.f) ? Pc + (pl - ph) : Pc + (ph - pr);
                     if (EDGE) bnew = inner(sl) ? bnew : Pc;
                     if (ODD)
-                        bnew = adv_x<EX>(bnew, CA[sl].x, CA[sl].y, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
+                        bnew = adv_x<EX>(bnew, Ax, Fx, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
                 } else {
                     bnew = c0[c][sl];  // b(nz+1): the top ghost of c_in
                 }
@@ -605,43 +629,34 @@ struct FluxTile {
             }
             if (!STEADY && s == 1) {
                 D0[sl] = Dn[sl];
-                Dn[sl] = bld3(rf, m12[sl], p.oD + 12u * NXY);  // level 2 (nz >= 2)
+                Dn[sl] = bld4(rf, m16[sl], p.oD + 16u * NXY);  // level 2 (nz >= 2)
                 continue;
             }
             D0[sl] = Dn[sl];
-            // first horizontal diffusion at level k
+            // first horizontal diffusion at level k: the face-averaged
+            // rho*rkh1 times dt/dx^2 (diffx.f90:35-36); x-side tiles also
+            // take the raw product at the line ends
             if (!ODD) {
-                const float pk = EX ? pkk[sl] : 0.f;
+                const float Dx = 0.5f * p.dtdx2 * (Dk.x + from_left(Dk.x));
 #pragma unroll
                 for (int c = 0; c < MC; ++c)
-                    dc[c][sl] = diff_x<EX>(dc[c][sl], lr0, Dk.x, pk, p.dtdx2, inner(sl), has(sl, F_XLO),
+                    dc[c][sl] = diff_x<EX>(dc[c][sl], lr0, Dx, Dk.x, p.dtdx2, inner(sl), has(sl, F_XLO),
                                            has(sl, F_XHI));
             } else {
                 *row(SC, 0, lj[sl]) = Dk.x;
                 *row(SC, 1, lj[sl]) = lr0;
-                if (EX) *row(SC, 2, lj[sl]) = pkk[sl];
             }
 #pragma unroll
             for (int c = 0; c < MC; ++c) qc[c][sl] = dc[c][sl] * lr0;
-            if (EY) {
-                pyo[sl] = bld(rf, m4[sl] * 2u + 4u, p.oE + 2u * lk);
-                pym[sl] = bld(rf, mdn8[sl] + 4u, p.oE + 2u * lk);
-                pyp[sl] = bld(rf, mup8[sl] + 4u, p.oE + 2u * lk);
-            }
+            *row(SY, 0, lj[sl]) = Dk.w;
         }
-        // after their last use: (dfx, lr, pz) of level s+1; its (ax, fx, ay,
-        // fy) arrived during phase 1; on x-side tiles rho*rkh1 of level s,
-        // which the next level diffuses
+        // after their last use: (pkx, lr, pz, pky) of level s+1; its (ax, ay)
+        // arrived during phase 1
         if (STEADY) __builtin_amdgcn_sched_barrier(0);
-        if (EX && (STEADY || s <= nzl)) {
-#pragma unroll
-            for (int sl = 0; sl < NS; ++sl)
-                pkk[sl] = bld(rf, m4[sl] * 2u, p.oE + 8u * NXY * (unsigned)(s - 1));
-        }
         if (STEADY || s + 1 <= nzl) {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
-                if (STEADY || s >= 2) Dn[sl] = bld3(rf, m12[sl], p.oD + 12u * NXY * (unsigned)s);
+                if (STEADY || s >= 2) Dn[sl] = bld4(rf, m16[sl], p.oD + 16u * NXY * (unsigned)s);
                 CA[sl] = CAn[sl];
             }
         }
@@ -671,7 +686,7 @@ struct FluxTile {
             if (lj[sl] < 3 || lj[sl] >= RY - 3) continue;  // wave-uniform
 #pragma unroll
             for (int c = 0; c < MC; ++c)
-                if (stored(sl, c)) bst(sto[c][sl], rout[c], co4[sl], ko);
+                bst(sto[c][sl], rout[c], so4[sl] + ko);
         }
     }
 
@@ -771,7 +786,7 @@ __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
         }
     }
     const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * (RY - 6) - 3;
-    __shared__ float sm[(3 * MC + (ODD ? 3 : 0)) * RX * RY];
+    __shared__ float sm[flux_planes(MC, ODD) * RX * RY];
     __shared__ ZLev zt[kMaxLevels + 2];
     for (int k = threadIdx.x; k <= p.g.nz + 1; k += NW * 64) zt[k] = p.zl[k];
     __syncthreads();
@@ -872,8 +887,8 @@ static int make_params(const StepArgs &a, const Fields &f, FluxParams &p)
     };
     // every byte offset the kernel forms (field buffer, class slab of c) is
     // 32-bit
-    if (f.bytes >= (size_t)1 << 32 || (size_t)a.g.G * 4 >= (size_t)1 << 32) {
-        set_error("flux kernel: fields (%zu B) or a class slab exceed 4 GiB", f.bytes);
+    if (f.bytes >= (size_t)1 << 32 || (size_t)a.g.G * 4 >= (size_t)1 << 31) {
+        set_error("flux kernel: fields (%zu B) exceed 4 GiB or a class slab 2 GiB", f.bytes);
         return F3D_EINVAL;
     }
     if (a.g.nz > kMaxLevels) {
@@ -884,8 +899,6 @@ static int make_params(const StepArgs &a, const Fields &f, FluxParams &p)
     p.fbytes = (unsigned)f.bytes;
     p.oA = off(f.cA);
     p.oD = off(f.cD);
-    p.oY = off(f.dfy);
-    p.oE = off(f.cE);
     p.oW = off(f.wz);
     p.oBF = off(f.bfac);
     p.oTF = off(f.tfac);
@@ -933,7 +946,7 @@ __global__ void __launch_bounds__(NW * 64)
     constexpr int PW = sizeof(FluxParams) / sizeof(int);
     static_assert(sizeof(FluxParams) % sizeof(int) == 0, "parameter block of whole dwords");
     static_assert(PW <= NW * 64, "one thread per parameter dword");
-    __shared__ float sm[(3 * MC + (ODD ? 3 : 0)) * RX * RY];
+    __shared__ float sm[flux_planes(MC, ODD) * RX * RY];
     __shared__ ZLev zt[kMaxLevels + 2];
     __shared__ int sp[PW];
     __shared__ int sitem[4];

@@ -625,45 +625,27 @@ __global__ void k_fields3d(long N, const double *vx, const double *vy, const dou
     o[6 * N + t] = (float)(r * rkver[t]);
 }
 
-// Class-independent face coefficients of the horizontal sweeps, formed once
-// per meteo read (they depend on the wind, the diffusivities and dt only):
-//   ax = dt/dx * vx                                   (advctx.f90:36)
-//   fx = 0.5*dt/dx * (1 - dt/dx*|v12|) * |v12|        at face i-1/2 (:42-43),
-//        v12 = 0.5*(vx(i) + vx(max(1,i-1)))
-//   dfx = dt/dx^2 * 0.5*(rho*rkh1)(i) + (rho*rkh1)(i-1)) at face i-1/2 (diffx.f90:35-36)
-// and the same along y with vy, rho*rkhor (advcty.f90, diffy.f90).
-__global__ void k_fields_face(Grid g, const double *vx, const double *vy, const double *rho,
-                              const double *rkh1, const double *rkhor, double dt, float *ax,
-                              float *fx, float *ay, float *fy, float *dfx, float *dfy)
+// Courant numbers of the horizontal sweeps, formed once per meteo read
+// (they depend on the wind and dt only): ax = dt/dx * vx, ay = dt/dy * vy
+// (advctx.f90:24-36, advcty.f90).  The flux kernel forms the limiter factor
+// of each face from the two cells beside it, and the face-averaged
+// diffusivities from the products rho*rkh1 and rho*rkhor.
+__global__ void k_fields_face(Grid g, const double *vx, const double *vy, double dt, float *ax,
+                              float *ay)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.N) return;
-    const int nx = g.nx;
-    const int i = (int)(t % nx) + 1;
-    const int j = (int)((t / nx) % g.ny) + 1;
-    const long tx = (i > 1) ? t - 1 : t;
-    const long ty = (j > 1) ? t - nx : t;
-    const double auxx = dt / g.dx, auxy = dt / g.dy;
-    const double v12x = fabs(0.5 * (vx[t] + vx[tx]));
-    const double v12y = fabs(0.5 * (vy[t] + vy[ty]));
-    ax[t] = (float)(auxx * vx[t]);
-    fx[t] = (float)(0.5 * auxx * (1.0 - auxx * v12x) * v12x);
-    ay[t] = (float)(auxy * vy[t]);
-    fy[t] = (float)(0.5 * auxy * (1.0 - auxy * v12y) * v12y);
-    dfx[t] = (float)(dt / (g.dx * g.dx) * 0.5 * (rho[t] * rkh1[t] + rho[tx] * rkh1[tx]));
-    dfy[t] = (float)(dt / (g.dy * g.dy) * 0.5 * (rho[t] * rkhor[t] + rho[ty] * rkhor[ty]));
+    ax[t] = (float)(dt / g.dx * vx[t]);
+    ay[t] = (float)(dt / g.dy * vy[t]);
 }
 
 // interleave the per-cell groups read together by the flux kernel
-__global__ void k_fields_pack(long n, Fields f, float4 *cA, float *cD, float2 *cE)
+__global__ void k_fields_pack(long n, Fields f, float2 *cA, float4 *cD)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    cA[t] = make_float4(f.ax[t], f.fx[t], f.ay[t], f.fy[t]);
-    cD[3 * t] = f.dfx[t];
-    cD[3 * t + 1] = f.lr[t];
-    cD[3 * t + 2] = f.pkz[t];
-    cE[t] = make_float2(f.pkx[t], f.pky[t]);
+    cA[t] = make_float2(f.ax[t], f.ay[t]);
+    cD[t] = make_float4(f.pkx[t], f.lr[t], f.pkz[t], f.pky[t]);
 }
 
 // per-level constants of advctzc (advctzc.f90:153-181) and diffz
@@ -798,22 +780,21 @@ int launch_fused(hipStream_t s, const Params &p0)
 static size_t separate_offset(const Grid &g)
 {
     const size_t zl = (size_t)(g.nz + 2) * (sizeof(ZLev) / sizeof(float));
-    return ((size_t)10 * g.N + zl + 3) & ~(size_t)3;
+    return ((size_t)6 * g.N + zl + 3) & ~(size_t)3;
 }
 
 size_t fields_floats(const Grid &g)
 {
-    return separate_offset(g) + (size_t)13 * g.N + (size_t)g.nc * g.N +
+    return separate_offset(g) + (size_t)9 * g.N + (size_t)g.nc * g.N +
            (size_t)2 * g.nc * g.NXY;
 }
 
 Fields fields_view(const Grid &g, const float *b0, const double *vsum1)
 {
     Fields f;
-    f.cA = reinterpret_cast<const float4 *>(b0);
-    f.cE = reinterpret_cast<const float2 *>(b0 + 4 * g.N);
-    f.cD = b0 + 6 * g.N;
-    f.zl = reinterpret_cast<const ZLev *>(b0 + 10 * g.N);
+    f.cD = reinterpret_cast<const float4 *>(b0);
+    f.cA = reinterpret_cast<const float2 *>(b0 + 4 * g.N);
+    f.zl = reinterpret_cast<const ZLev *>(b0 + 6 * g.N);
     f.base = b0;
     f.bytes = sizeof(float) * fields_floats(g);
     const float *b = b0 + separate_offset(g);
@@ -829,11 +810,7 @@ Fields fields_view(const Grid &g, const float *b0, const double *vsum1)
     f.tfac = f.bfac + (long)g.nc * g.NXY;
     const float *face = f.tfac + (long)g.nc * g.NXY;
     f.ax = face;
-    f.fx = face + g.N;
-    f.ay = face + 2 * g.N;
-    f.fy = face + 3 * g.N;
-    f.dfx = face + 4 * g.N;
-    f.dfy = face + 5 * g.N;
+    f.ay = face + g.N;
     f.vsum1 = vsum1;
     return f;
 }
@@ -853,14 +830,10 @@ int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *v
                        const_cast<float *>(f.bfac), const_cast<float *>(f.tfac), vsum1);
     F3D_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_fields_face, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g, vx,
-                       vy, rho, rkh1, rkhor, dt, const_cast<float *>(f.ax),
-                       const_cast<float *>(f.fx), const_cast<float *>(f.ay),
-                       const_cast<float *>(f.fy), const_cast<float *>(f.dfx),
-                       const_cast<float *>(f.dfy));
+                       vy, dt, const_cast<float *>(f.ax), const_cast<float *>(f.ay));
     F3D_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_fields_pack, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g.N, f,
-                       const_cast<float4 *>(f.cA), const_cast<float *>(f.cD),
-                       const_cast<float2 *>(f.cE));
+                       const_cast<float2 *>(f.cA), const_cast<float4 *>(f.cD));
     F3D_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_zlev, dim3(1), dim3(64), 0, s, g.nz, zc, dt, const_cast<ZLev *>(f.zl));
     F3D_CHECK_LAUNCH();
