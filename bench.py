@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--tile", default="58x18")
+    ap.add_argument("--waves", type=int, default=0,
+                    help="waves per workgroup of the flux kernel (0: two region rows per wave)")
     ap.add_argument("--skip-ensemble", action="store_true")
     ap.add_argument("--ensemble-members", type=int, default=64)
     ap.add_argument("--skip-uneven", action="store_true")
@@ -91,7 +93,7 @@ def main():
     # ---------------- FP32 fused path ----------------
     r = F.Run(sc, precision=F.FP32)
     tx, ty = (int(v) for v in args.tile.split("x"))
-    r.ctx.set_tile(tx, ty, 0, args.slabs)
+    r.ctx.set_tile(tx, ty, args.waves, args.slabs)
     t0 = time.time()
     r.meteo_update()
     r.ctx.sync()

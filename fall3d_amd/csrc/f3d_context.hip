@@ -192,7 +192,8 @@ int rebuild_fields(f3d_ctx *c)
     TRY(f32::build_fields(c->stream, g, c->d_vx.as<double>(), c->d_vy.as<double>(),
                           c->d_vz.as<double>(), c->d_rho.as<double>(), c->d_rkh1.as<double>(),
                           c->d_rkhor.as<double>(), c->d_rkver.as<double>(), c->d_vset.as<double>(),
-                          c->d_vdep.as<double>(), c->d_dz.as<double>(), c->d_zc.as<ZCoef>(),
+                          c->d_vdep.as<double>(), c->d_dz.as<double>(), c->d_hm1.as<double>(),
+                          c->d_zc.as<ZCoef>(),
                           c->dt, c->d_f32.as<float>(),
                           c->d_vsum1.as<double>()));
     F3D_HIP(hipStreamSynchronize(c->stream));

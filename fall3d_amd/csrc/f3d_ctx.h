@@ -64,7 +64,8 @@ struct Fields {
     // interleaved per cell for the flux kernel: one load per cell for each
     // group
     const float2 *cA;  // (ax, ay)
-    const float4 *cD;  // (rho*rkh1, 1/rho, rho*rkver, rho*rkhor)
+    const float *cD;   // (1/rho, rho*rkver, rho*rkhor), 12-byte records
+    const float *h2;   // 1/Hm1^2 per column (nx,ny): rho*rkh1 = rho*rkhor*h2
     const ZLev *zl;    // per level 0..nz+1
     const float *base;  // the buffer all FP32 fields live in, and its size
     size_t bytes;
@@ -74,7 +75,7 @@ struct Fields {
 int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *vy,
                  const double *vz, const double *rho, const double *rkh1, const double *rkhor,
                  const double *rkver, const double *vset, const double *vdep, const double *dz,
-                 const ZCoef *zc, double dt, float *f32buf, double *vsum1);
+                 const double *hm1, const ZCoef *zc, double dt, float *f32buf, double *vsum1);
 size_t fields_floats(const Grid &g);
 Fields fields_view(const Grid &g, const float *f32buf, const double *vsum1);
 

@@ -50,8 +50,10 @@ struct FluxParams {
     float *cout;
     double *cdry;
     // all FP32 fields live in one buffer; byte offsets of the groups in it:
-    // cA (ax, ay: 8-byte records), cD (rho*rkh1, 1/rho, rho*rkver,
-    // rho*rkhor), wz (vz+vset, class 0), bfac and tfac (class 0).  The face
+    // cA (ax, ay: 8-byte records), cD (1/rho, rho*rkver, rho*rkhor:
+    // 12-byte records), wz (vz+vset, class 0), bfac and tfac (class 0), h2
+    // (1/Hm1^2 per column: rho*rkh1 = rho*rkhor/Hm1^2, meteo.f90:108-112,
+    // so no rho*rkh1 field is read; 1 on UTM grids).  The face
     // coefficients (limiter factors, face-averaged diffusivities) are formed
     // from the two cells on either side of the face, which the kernel holds
     // anyway, so no per-face field is read.  The kernel reads each record
@@ -59,7 +61,7 @@ struct FluxParams {
     // line again for the rest.
     const float *fbase;
     unsigned fbytes;
-    unsigned oA, oD, oW, oBF, oTF;
+    unsigned oA, oD, oW, oBF, oTF, oH;
     const double *vsum1;
     const ZLev *zl;
     double dt64;
@@ -234,17 +236,19 @@ struct FluxTile {
 
     // per-row geometry: byte offsets of the (clamped) column in c, in the
     // meteo arrays and of the rows above/below; edge flags
-    unsigned co4[NS], so4[NS], m4[NS], m8[NS], m16[NS], fl[NS];
+    unsigned co4[NS], so4[NS], m4[NS], m8[NS], m12[NS], fl[NS];
     int lj[NS];
     // Class-independent coefficients: (ax, ay) of the current and the next
-    // level, (pkx, lr, pz, pky) of the current level and the one below.
+    // level, (lr, pz, pky) of the current level and the one below, and the
+    // column's 1/Hm1^2, which turns pky = rho*rkhor into pkx = rho*rkh1.
     // Every load fills a register after the last use of the value it
     // replaces, every loaded lane is used (a dead lane is written back late
     // and makes its register's next use wait), and each value is loaded a
     // level before it is needed.  gv: wind at the face cell of a ghost (edge
     // tiles), one level ahead.
     float2 CA[NS], CAn[NS];
-    float4 Dn[NS], D0[NS];
+    float3 Dn[NS], D0[NS];
+    float h2[NS];
     float gv[NS];
     // per class: advctzc window c(s), c(s+1), w(s), U(s), W2(s), delta(s) (after
     // level nz, c0 holds the top ghost c(nz+1)), prefetched c(s+2), w(s+1);
@@ -306,7 +310,7 @@ struct FluxTile {
             co4[sl] = 4u * (unsigned)cidx(nx, ny, ci, cj, 0);
             m4[sl] = 4u * mo;
             m8[sl] = 8u * mo;
-            m16[sl] = 16u * mo;
+            m12[sl] = 12u * mo;
             unsigned f = 0;
             const bool inx = !EX || (gi >= 1 && gi <= nx);
             const bool iny = !EY || (gj >= 1 && gj <= ny);
@@ -341,8 +345,9 @@ struct FluxTile {
         for (int sl = 0; sl < NS; ++sl) {
             CA[sl] = bld2(rf, m8[sl], p.oA);  // level 1
             CAn[sl] = make_float2(0.f, 0.f);
-            Dn[sl] = bld4(rf, m16[sl], p.oD);
-            D0[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
+            Dn[sl] = bld3(rf, m12[sl], p.oD);
+            D0[sl] = make_float3(0.f, 0.f, 0.f);
+            h2[sl] = bld(rf, m4[sl], p.oH);
             gv[sl] = EDGE ? bld(rf, m8[sl] + (has(sl, F_GX) ? 0u : 4u), p.oA) : 0.f;
 #pragma unroll
             for (int c = 0; c < MC; ++c) {
@@ -575,9 +580,9 @@ struct FluxTile {
         }
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
-            const float lrn = Dn[sl].y, pzn = Dn[sl].z;  // level s (unused at s = nz+1)
-            const float4 Dk = D0[sl];                    // level k (unused at s = 1)
-            const float lr0 = Dk.y, pz0 = Dk.z;
+            const float lrn = Dn[sl].x, pzn = Dn[sl].y;  // level s (unused at s = nz+1)
+            const float3 Dk = D0[sl];                    // level k (unused at s = 1)
+            const float lr0 = Dk.x, pz0 = Dk.y;
             const float ay = CA[sl].y;
             const float Ax = CA[sl].x;
             const float Fx = ODD ? face_factor(Ax, from_left(Ax)) : 0.f;
@@ -629,34 +634,35 @@ struct FluxTile {
             }
             if (!STEADY && s == 1) {
                 D0[sl] = Dn[sl];
-                Dn[sl] = bld4(rf, m16[sl], p.oD + 16u * NXY);  // level 2 (nz >= 2)
+                Dn[sl] = bld3(rf, m12[sl], p.oD + 12u * NXY);  // level 2 (nz >= 2)
                 continue;
             }
             D0[sl] = Dn[sl];
             // first horizontal diffusion at level k: the face-averaged
             // rho*rkh1 times dt/dx^2 (diffx.f90:35-36); x-side tiles also
             // take the raw product at the line ends
+            const float pkx = Dk.z * h2[sl];
             if (!ODD) {
-                const float Dx = 0.5f * p.dtdx2 * (Dk.x + from_left(Dk.x));
+                const float Dx = 0.5f * p.dtdx2 * (pkx + from_left(pkx));
 #pragma unroll
                 for (int c = 0; c < MC; ++c)
-                    dc[c][sl] = diff_x<EX>(dc[c][sl], lr0, Dx, Dk.x, p.dtdx2, inner(sl), has(sl, F_XLO),
+                    dc[c][sl] = diff_x<EX>(dc[c][sl], lr0, Dx, pkx, p.dtdx2, inner(sl), has(sl, F_XLO),
                                            has(sl, F_XHI));
             } else {
-                *row(SC, 0, lj[sl]) = Dk.x;
+                *row(SC, 0, lj[sl]) = pkx;
                 *row(SC, 1, lj[sl]) = lr0;
             }
 #pragma unroll
             for (int c = 0; c < MC; ++c) qc[c][sl] = dc[c][sl] * lr0;
-            *row(SY, 0, lj[sl]) = Dk.w;
+            *row(SY, 0, lj[sl]) = Dk.z;
         }
-        // after their last use: (pkx, lr, pz, pky) of level s+1; its (ax, ay)
+        // after their last use: (lr, pz, pky) of level s+1; its (ax, ay)
         // arrived during phase 1
         if (STEADY) __builtin_amdgcn_sched_barrier(0);
         if (STEADY || s + 1 <= nzl) {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
-                if (STEADY || s >= 2) Dn[sl] = bld4(rf, m16[sl], p.oD + 16u * NXY * (unsigned)s);
+                if (STEADY || s >= 2) Dn[sl] = bld3(rf, m12[sl], p.oD + 12u * NXY * (unsigned)s);
                 CA[sl] = CAn[sl];
             }
         }
@@ -863,12 +869,14 @@ int launch_flux(hipStream_t s, const StepArgs &a, const FluxParams &p0, bool odd
 }  // namespace
 
 // Configurations of the flux kernel: ty output rows (region rows RY = ty + 6,
-// two per wavefront: 8, 12 or 16 waves) and mc = 1 or 2 member*class slabs
-// per workgroup.
+// two per wavefront: 8, 12 or 16 waves; or, for ty = 18, three per
+// wavefront: 8 waves of up to 256 registers) and mc = 1 or 2 member*class
+// slabs per workgroup.
 bool flux_supported(TileShape ts)
 {
     if (ts.tx != TXR) return false;
-    if (ts.nw != 0 && ts.nw != (ts.ty + 6) / 2) return false;
+    const bool three = ts.ty == 18 && ts.nw == 8;
+    if (ts.nw != 0 && ts.nw != (ts.ty + 6) / 2 && !three) return false;
     return (ts.ty == 10 || ts.ty == 18 || ts.ty == 26) && (ts.mc == 0 || ts.mc == 1 || ts.mc == 2);
 }
 
@@ -899,6 +907,7 @@ static int make_params(const StepArgs &a, const Fields &f, FluxParams &p)
     p.fbytes = (unsigned)f.bytes;
     p.oA = off(f.cA);
     p.oD = off(f.cD);
+    p.oH = off(f.h2);
     p.oW = off(f.wz);
     p.oBF = off(f.bfac);
     p.oTF = off(f.tfac);
@@ -919,6 +928,10 @@ int flux_step(hipStream_t s, const StepArgs &a, const Fields &f, TileShape ts)
     const int mc = ts.mc ? ts.mc : 1;
     if (ts.ty == 10 && mc == 1) return launch_flux<16, 8, 1>(s, a, p, odd);
     if (ts.ty == 10 && mc == 2) return launch_flux<16, 8, 2>(s, a, p, odd);
+    if (ts.ty == 18 && ts.nw == 8) {
+        if (mc == 1) return launch_flux<24, 8, 1>(s, a, p, odd);
+        return launch_flux<24, 8, 2>(s, a, p, odd);
+    }
     if (ts.ty == 18 && mc == 1) return launch_flux<24, 12, 1>(s, a, p, odd);
     if (ts.ty == 18 && mc == 2) return launch_flux<24, 12, 2>(s, a, p, odd);
     if (ts.ty == 26 && mc == 1) return launch_flux<32, 16, 1>(s, a, p, odd);

@@ -640,12 +640,24 @@ __global__ void k_fields_face(Grid g, const double *vx, const double *vy, double
 }
 
 // interleave the per-cell groups read together by the flux kernel
-__global__ void k_fields_pack(long n, Fields f, float2 *cA, float4 *cD)
+__global__ void k_fields_pack(long n, Fields f, float2 *cA, float *cD)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     cA[t] = make_float2(f.ax[t], f.ay[t]);
-    cD[t] = make_float4(f.pkx[t], f.lr[t], f.pkz[t], f.pky[t]);
+    cD[3 * t] = f.lr[t];
+    cD[3 * t + 1] = f.pkz[t];
+    cD[3 * t + 2] = f.pky[t];
+}
+
+// 1/Hm1^2 per column: the flux kernel forms rho*rkh1 = rho*rkhor/Hm1^2
+// (meteo.f90:108-112) from rho*rkhor instead of reading a field for it
+__global__ void k_fields_h2(long nxy, const double *hm1, float *h2)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nxy) return;
+    const double h = hm1[t];
+    h2[t] = (float)(1.0 / (h * h));
 }
 
 // per-level constants of advctzc (advctzc.f90:153-181) and diffz
@@ -775,26 +787,27 @@ int launch_fused(hipStream_t s, const Params &p0)
 
 }  // namespace
 
-// the interleaved flux-kernel groups lead the buffer (16-byte aligned),
-// followed by the z-level table and the separate fields
+// the interleaved flux-kernel groups lead the buffer ((ax, ay) 8-byte
+// aligned, then the 12-byte records), followed by the z-level table and the
+// separate fields
 static size_t separate_offset(const Grid &g)
 {
     const size_t zl = (size_t)(g.nz + 2) * (sizeof(ZLev) / sizeof(float));
-    return ((size_t)6 * g.N + zl + 3) & ~(size_t)3;
+    return ((size_t)5 * g.N + zl + 3) & ~(size_t)3;
 }
 
 size_t fields_floats(const Grid &g)
 {
     return separate_offset(g) + (size_t)9 * g.N + (size_t)g.nc * g.N +
-           (size_t)2 * g.nc * g.NXY;
+           (size_t)2 * g.nc * g.NXY + (size_t)g.NXY;
 }
 
 Fields fields_view(const Grid &g, const float *b0, const double *vsum1)
 {
     Fields f;
-    f.cD = reinterpret_cast<const float4 *>(b0);
-    f.cA = reinterpret_cast<const float2 *>(b0 + 4 * g.N);
-    f.zl = reinterpret_cast<const ZLev *>(b0 + 6 * g.N);
+    f.cA = reinterpret_cast<const float2 *>(b0);
+    f.cD = b0 + 2 * g.N;
+    f.zl = reinterpret_cast<const ZLev *>(b0 + 5 * g.N);
     f.base = b0;
     f.bytes = sizeof(float) * fields_floats(g);
     const float *b = b0 + separate_offset(g);
@@ -811,6 +824,7 @@ Fields fields_view(const Grid &g, const float *b0, const double *vsum1)
     const float *face = f.tfac + (long)g.nc * g.NXY;
     f.ax = face;
     f.ay = face + g.N;
+    f.h2 = face + 2 * g.N;
     f.vsum1 = vsum1;
     return f;
 }
@@ -818,7 +832,7 @@ Fields fields_view(const Grid &g, const float *b0, const double *vsum1)
 int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *vy,
                  const double *vz, const double *rho, const double *rkh1, const double *rkhor,
                  const double *rkver, const double *vset, const double *vdep, const double *dz,
-                 const ZCoef *zc, double dt, float *f32buf, double *vsum1)
+                 const double *hm1, const ZCoef *zc, double dt, float *f32buf, double *vsum1)
 {
     Fields f = fields_view(g, f32buf, vsum1);
     // vx .. pkz are contiguous from f.vx
@@ -833,7 +847,10 @@ int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *v
                        vy, dt, const_cast<float *>(f.ax), const_cast<float *>(f.ay));
     F3D_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_fields_pack, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g.N, f,
-                       const_cast<float2 *>(f.cA), const_cast<float4 *>(f.cD));
+                       const_cast<float2 *>(f.cA), const_cast<float *>(f.cD));
+    F3D_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_fields_h2, dim3((unsigned)((g.NXY + 255) / 256)), dim3(256), 0, s,
+                       (long)g.NXY, hm1, const_cast<float *>(f.h2));
     F3D_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_zlev, dim3(1), dim3(64), 0, s, g.nz, zc, dt, const_cast<ZLev *>(f.zl));
     F3D_CHECK_LAUNCH();

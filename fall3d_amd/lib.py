@@ -331,7 +331,8 @@ class Run:
 
     def __init__(self, sc, precision=FP32, cd_state=1.0, tile=None):
         self.sc = sc
-        self.ctx = Context(sc.nx, sc.ny, sc.nz, sc.nc, sc.dx, sc.dy, sc.zlayer, precision)
+        self.ctx = Context(sc.nx, sc.ny, sc.nz, sc.nc, sc.dx, sc.dy, sc.zlayer, precision,
+                           hm1=getattr(sc, "hm1_cols", None))
         if tile is not None:
             self.ctx.set_tile(*tile)
         self.ctx.set_classes(sc.diam, sc.rhop, sc.sphe, sc.modv)

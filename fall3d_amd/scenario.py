@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+from typing import Optional
 
 import numpy as np
 
@@ -73,6 +74,9 @@ class Scenario:
     rkh0: float = 5000.0
     csh: float = 0.2275
     end_time: float = 6 * 3600.0
+    # Hm1 per column (nx,ny) for LON-LAT grids (sin of the colatitude,
+    # reagrd.f90:247-270); None: UTM, Hm1 = 1
+    hm1_cols: Optional[np.ndarray] = None
 
     @property
     def dz(self) -> np.ndarray:
@@ -86,6 +90,8 @@ class Scenario:
 
     @property
     def hm1(self) -> np.ndarray:
+        if self.hm1_cols is not None:
+            return np.asfortranarray(self.hm1_cols, dtype=np.float64)
         return np.ones((self.nx, self.ny), order="F")
 
     @property

@@ -172,7 +172,8 @@ int f3d_sync(f3d_ctx *ctx);
  * member*class slabs per workgroup (f3d_set_tile_config). */
 int f3d_set_tile(f3d_ctx *ctx, int tx, int ty);
 /* The same with the 58-column kernel's options: waves per workgroup (0 or
- * (ty+6)/2) and member*class slabs per workgroup (0 or 1: one; 2: two slabs
+ * (ty+6)/2: two region rows per wave; or 8 with ty = 18: three rows per
+ * wave) and member*class slabs per workgroup (0 or 1: one; 2: two slabs
  * share the coefficient loads and barriers).  No counterpart in the
  * reference. */
 int f3d_set_tile_config(f3d_ctx *ctx, int tx, int ty, int waves, int slabs);

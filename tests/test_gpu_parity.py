@@ -11,6 +11,7 @@ Tolerances:
   <= 1e-4, mass budget terms <= 1e-4 relative.
 """
 import ctypes
+import dataclasses
 import os
 
 import numpy as np
@@ -157,11 +158,12 @@ def test_fp32_context_vs_golden(path):
 
 
 TILES = [(32, 8), (16, 16), (64, 4), (58, 10), (58, 18), (58, 26), (58, 10, 0, 2), (58, 18, 0, 2),
-         (58, 26, 0, 2)]
+         (58, 26, 0, 2), (58, 18, 8, 1), (58, 18, 8, 2)]
 
 
 def _tile_id(t):
-    return f"{t[0]}x{t[1]}" + (f"m{t[3]}" if len(t) > 3 else "")
+    return (f"{t[0]}x{t[1]}" + (f"w{t[2]}" if len(t) > 3 and t[2] else "")
+            + (f"m{t[3]}" if len(t) > 3 else ""))
 
 
 def _compare_fp32_with_oracle(sc, nsteps, tile=None):
@@ -194,6 +196,41 @@ def test_fp32_vs_oracle_moderate_grid(parity_steps):
     assert m["rho_m"] <= 1e-5, m
 
 
+def _lonlat_band(sc):
+    # LON-LAT rows between colatitudes 35 and 50 degrees: Hm1 = sin(colat)
+    # per row (reagrd.f90:247-270), so the scaling of meteo.f90:108-112
+    # (vx/Hm1, rho*Hm1, rkh1 = rkhor/Hm1^2) and the setsrc volume differ
+    # from the UTM case
+    colat = np.radians(np.linspace(35.0, 50.0, sc.ny))
+    h = np.repeat(np.sin(colat)[None, :], sc.nx, axis=0)
+    return dataclasses.replace(sc, hm1_cols=np.asfortranarray(h))
+
+
+def test_fp32_vs_oracle_lonlat_hm1():
+    sc = _lonlat_band(S.config_small(nx=130, ny=47, nz=9, nc=3, seed=33, mean_u=4.0, mean_v=-3.0))
+    m = _compare_fp32_with_oracle(sc, 10)
+    assert m["c_l1"] <= 1e-4 and m["load_l1"] <= 1e-4, m
+    assert m["rho_m"] <= 1e-5, m
+
+
+def test_fp64_vs_oracle_lonlat_hm1():
+    sc = _lonlat_band(S.config_small(nx=21, ny=17, nz=7, nc=3, seed=34, mean_u=-5.0, mean_v=2.0))
+    o = OracleRun(sc)
+    o.meteo_update()
+    r = F.Run(sc, precision=F.FP64)
+    try:
+        r.meteo_update()
+        assert abs(r.dt - o.dt) <= 1e-11 * o.dt
+        for _ in range(8):
+            o.step()
+        r.step(8)
+        c, cdry = r.state()
+        assert rel_linf(interior(c), interior(o.c)) <= 1e-11
+        assert rel_linf(cdry[:, :, 1:], o.cdry[:, :, 1:]) <= 1e-11
+    finally:
+        r.close()
+
+
 @pytest.mark.parametrize("tile", TILES, ids=_tile_id)
 def test_fp32_every_tile_shape(tile):
     # a grid larger than every tile in both directions (interior and edge
@@ -222,8 +259,8 @@ def test_fp32_tile_kinds_one_axis_without_interior_tiles(shape):
     assert m["c_l1"] <= 1e-4 and m["load_l1"] <= 1e-4, m
 
 
-@pytest.mark.parametrize("tile", [(58, 18, 0, 2), (58, 10, 0, 2), (58, 26, 0, 1)],
-                         ids=lambda t: f"{t[0]}x{t[1]}m{t[3]}")
+@pytest.mark.parametrize("tile", [(58, 18, 0, 2), (58, 10, 0, 2), (58, 26, 0, 1), (58, 18, 8, 2)],
+                         ids=_tile_id)
 def test_fp32_deterministic(tile):
     # grids where every tile is an edge tile in x and y, many steps: repeated
     # runs are bit-identical (regression test for a cross-phase LDS hazard
