@@ -61,15 +61,26 @@ struct Fields {
     const double *vsum1;        // min(0, vset(:,:,1)+vdep) (nx,ny,nc), FP64 deposit factor
     // Courant numbers dt/dx*vx, dt/dy*vy (dt-dependent), see k_fields_face
     const float *ax, *ay;
-    // interleaved per cell for the flux kernel: one load per cell for each
-    // group
+    // Read by the flux kernel, in its tile-major layout: for each level,
+    // row j and tile column bx (58 columns, kFluxTileCols), the 64 columns
+    // of the tile's region 58*bx-2 .. 58*bx+61 (clamped to 1..nx) are 64
+    // consecutive records, so one region row is one aligned, whole-line
+    // load.  Record index ((k-1)*ny + j-1)*ftx*64 + bx*64 + lane; ftl =
+    // ny*ftx*64 records per level.
     const float2 *cA;  // (ax, ay)
     const float *cD;   // (1/rho, rho*rkver, rho*rkhor), 12-byte records
+    const float *wzt;  // vz+vset per class, nz*ftl records per class
+    int ftx;           // tile columns of the layout
+    long ftl;          // records per level
     const float *h2;   // 1/Hm1^2 per column (nx,ny): rho*rkh1 = rho*rkhor*h2
     const ZLev *zl;    // per level 0..nz+1
     const float *base;  // the buffer all FP32 fields live in, and its size
     size_t bytes;
 };
+
+// output columns of one tile of the flux kernel (a 64-lane region row
+// minus the halo of 3 on either side)
+constexpr int kFluxTileCols = 58;
 
 // Build Fields from the FP64 coefficient arrays (meteo.f90:108-112 applied).
 int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *vy,

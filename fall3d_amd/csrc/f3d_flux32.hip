@@ -38,6 +38,7 @@ namespace {
 constexpr int kXCD = 8;
 constexpr int RX = 64;       // region row = one wavefront
 constexpr int TXR = RX - 6;  // output columns per tile
+static_assert(TXR == kFluxTileCols, "the field layout's tile columns are the kernel's");
 constexpr int kMaxLevels = 64;
 // s_waitcnt immediate of gfx9: vmcnt(0), expcnt and lgkmcnt left open
 constexpr int kWaitVm0 = 0x0F70;
@@ -51,7 +52,9 @@ struct FluxParams {
     double *cdry;
     // all FP32 fields live in one buffer; byte offsets of the groups in it:
     // cA (ax, ay: 8-byte records), cD (1/rho, rho*rkver, rho*rkhor:
-    // 12-byte records), wz (vz+vset, class 0), bfac and tfac (class 0), h2
+    // 12-byte records) and wz (vz+vset, class 0), all three in the
+    // tile-major layout (one region row = 64 consecutive, aligned records;
+    // ftl records per level); bfac and tfac (class 0) and h2
     // (1/Hm1^2 per column: rho*rkh1 = rho*rkhor/Hm1^2, meteo.f90:108-112,
     // so no rho*rkh1 field is read; 1 on UTM grids).  The face
     // coefficients (limiter factors, face-averaged diffusivities) are formed
@@ -62,6 +65,7 @@ struct FluxParams {
     const float *fbase;
     unsigned fbytes;
     unsigned oA, oD, oW, oBF, oTF, oH;
+    unsigned ftx, ftl;  // tile columns and records per level of the tile-major layout
     const double *vsum1;
     const ZLev *zl;
     double dt64;
@@ -236,7 +240,10 @@ struct FluxTile {
 
     // per-row geometry: byte offsets of the (clamped) column in c, in the
     // meteo arrays and of the rows above/below; edge flags
-    unsigned co4[NS], so4[NS], m4[NS], m8[NS], m12[NS], fl[NS];
+    // m4: the column in 2-D fields; mt4/mt8/mt12: the lane's record in the
+    // tile-major layout, times the record size
+    unsigned co4[NS], so4[NS], m4[NS], mt4[NS], mt8[NS], mt12[NS], fl[NS];
+    unsigned FTL;  // records per level of the tile-major layout
     int lj[NS];
     // Class-independent coefficients: (ax, ay) of the current and the next
     // level, (lr, pz, pky) of the current level and the one below, and the
@@ -295,11 +302,13 @@ struct FluxTile {
             rin[c] = mkbuf(p.cin + (size_t)q * G, 4 * G);
             // the duplicate slab of an odd class count writes nothing
             rout[c] = mkbuf(p.cout + (size_t)q * G, live[c] ? 4 * G : 0);
-            oW[c] = p.oW + 4u * (unsigned)(icv[c] * N);
+            oW[c] = p.oW + 4u * (unsigned)icv[c] * (unsigned)nz * p.ftl;
             oBF[c] = p.oBF + 4u * (unsigned)icv[c] * NXY;
             oTF[c] = p.oTF + 4u * (unsigned)icv[c] * NXY;
         }
         const int gi = gi0 + li;
+        FTL = p.ftl;
+        const unsigned btx = (unsigned)((gi0 + 2) / TXR);  // the tile's column
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
             lj[sl] = w * NS + sl;
@@ -309,8 +318,10 @@ struct FluxTile {
             const unsigned mo = (unsigned)((ci - 1) + nx * (cj - 1));
             co4[sl] = 4u * (unsigned)cidx(nx, ny, ci, cj, 0);
             m4[sl] = 4u * mo;
-            m8[sl] = 8u * mo;
-            m12[sl] = 12u * mo;
+            const unsigned mt = ((unsigned)(cj - 1) * p.ftx + btx) * 64u + (unsigned)li;
+            mt4[sl] = 4u * mt;
+            mt8[sl] = 8u * mt;
+            mt12[sl] = 12u * mt;
             unsigned f = 0;
             const bool inx = !EX || (gi >= 1 && gi <= nx);
             const bool iny = !EY || (gj >= 1 && gj <= ny);
@@ -343,18 +354,18 @@ struct FluxTile {
         dtop = p.dt * zt[nz].ldz;
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
-            CA[sl] = bld2(rf, m8[sl], p.oA);  // level 1
+            CA[sl] = bld2(rf, mt8[sl], p.oA);  // level 1
             CAn[sl] = make_float2(0.f, 0.f);
-            Dn[sl] = bld3(rf, m12[sl], p.oD);
+            Dn[sl] = bld3(rf, mt12[sl], p.oD);
             D0[sl] = make_float3(0.f, 0.f, 0.f);
             h2[sl] = bld(rf, m4[sl], p.oH);
-            gv[sl] = EDGE ? bld(rf, m8[sl] + (has(sl, F_GX) ? 0u : 4u), p.oA) : 0.f;
+            gv[sl] = EDGE ? bld(rf, mt8[sl] + (has(sl, F_GX) ? 0u : 4u), p.oA) : 0.f;
 #pragma unroll
             for (int c = 0; c < MC; ++c) {
                 const float c1 = bld(rin[c], co4[sl], 4 * SZ);
                 const float c2 = bld(rin[c], co4[sl], 8 * SZ);
                 const float gb = c1 * bld(rf, m4[sl], oBF[c]);  // c(0), setbcc.f90:108-113
-                const float v1 = bld(rf, m4[sl], oW[c]);
+                const float v1 = bld(rf, mt4[sl], oW[c]);
                 c0[c][sl] = c1;
                 cp1[c][sl] = c2;
                 vk[c][sl] = v1;
@@ -366,7 +377,7 @@ struct FluxTile {
                 dprev[c][sl] = d1;
                 gprev[c][sl] = gb;
                 nc2[c][sl] = (3 <= nz) ? bld(rin[c], co4[sl], 12 * SZ) : 0.f;
-                nwz[c][sl] = bld(rf, m4[sl], oW[c] + 4 * NXY);
+                nwz[c][sl] = bld(rf, mt4[sl], oW[c] + 4u * FTL);
                 b0[c][sl] = q0[c][sl] = 0.f;
                 dc[c][sl] = qc[c][sl] = 0.f;
                 P[c][sl] = phi[c][sl] = 0.f;
@@ -386,7 +397,7 @@ struct FluxTile {
         // ======== phase 1: advctzc at level s (and advctx on even steps) ========
         if (STEADY || s <= nzl) {
             const ZLev zn = zt[s + 1];
-            const unsigned lz = 4u * NXY * (unsigned)s;  // byte offset of level s+1 in a 3-D field
+            const unsigned lz = 4u * FTL * (unsigned)s;  // byte offset of level s+1 in a tiled field
             // c(s+3) first: carrying it into the window copies the load, so
             // it is issued a whole level before that copy
             float nc3[MC][NS];
@@ -452,16 +463,16 @@ struct FluxTile {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 if (EDGE && (STEADY || s + 1 <= nzl))
-                    gv[sl] = bld(rf, m8[sl] + (has(sl, F_GX) ? 0u : 4u), p.oA + 2u * lz);
+                    gv[sl] = bld(rf, mt8[sl] + (has(sl, F_GX) ? 0u : 4u), p.oA + 2u * lz);
 #pragma unroll
                 for (int c = 0; c < MC; ++c) {
                     if (PIPE)
                         nc2[c][sl] = nc3[c][sl];
                     else if (STEADY || s + 3 <= nzl)
                         nc2[c][sl] = bldnt(rin[c], co4[sl], 4u * SZ * (unsigned)(s + 3));
-                    if (STEADY || s + 2 <= nzl) nwz[c][sl] = bld(rf, m4[sl], oW[c] + lz + 4u * NXY);
+                    if (STEADY || s + 2 <= nzl) nwz[c][sl] = bld(rf, mt4[sl], oW[c] + lz + 4u * FTL);
                 }
-                if (STEADY || s + 1 <= nzl) CAn[sl] = bld2(rf, m8[sl], p.oA + 2u * lz);
+                if (STEADY || s + 1 <= nzl) CAn[sl] = bld2(rf, mt8[sl], p.oA + 2u * lz);
             }
         }
         __syncthreads();
@@ -634,7 +645,7 @@ struct FluxTile {
             }
             if (!STEADY && s == 1) {
                 D0[sl] = Dn[sl];
-                Dn[sl] = bld3(rf, m12[sl], p.oD + 12u * NXY);  // level 2 (nz >= 2)
+                Dn[sl] = bld3(rf, mt12[sl], p.oD + 12u * FTL);  // level 2 (nz >= 2)
                 continue;
             }
             D0[sl] = Dn[sl];
@@ -662,7 +673,7 @@ struct FluxTile {
         if (STEADY || s + 1 <= nzl) {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
-                if (STEADY || s >= 2) Dn[sl] = bld3(rf, m12[sl], p.oD + 12u * NXY * (unsigned)s);
+                if (STEADY || s >= 2) Dn[sl] = bld3(rf, mt12[sl], p.oD + 12u * FTL * (unsigned)s);
                 CA[sl] = CAn[sl];
             }
         }
@@ -908,7 +919,14 @@ static int make_params(const StepArgs &a, const Fields &f, FluxParams &p)
     p.oA = off(f.cA);
     p.oD = off(f.cD);
     p.oH = off(f.h2);
-    p.oW = off(f.wz);
+    p.oW = off(f.wzt);
+    p.ftx = (unsigned)f.ftx;
+    p.ftl = (unsigned)f.ftl;
+    if (f.ftx != (a.g.nx + TXR - 1) / TXR) {
+        set_error("flux kernel: field layout of %d tile columns, grid needs %d", f.ftx,
+                  (a.g.nx + TXR - 1) / TXR);
+        return F3D_EINVAL;
+    }
     p.oBF = off(f.bfac);
     p.oTF = off(f.tfac);
     p.vsum1 = f.vsum1;

@@ -639,15 +639,25 @@ __global__ void k_fields_face(Grid g, const double *vx, const double *vy, double
     ay[t] = (float)(dt / g.dy * vy[t]);
 }
 
-// interleave the per-cell groups read together by the flux kernel
-__global__ void k_fields_pack(long n, Fields f, float2 *cA, float *cD)
+// the flux kernel's records in its tile-major layout (Fields::cA): record r
+// of a level is lane r%64 of tile column (r/64)%ftx in row r/(64*ftx); the
+// lane's column is clamped to the domain, as the kernel clamps ghost columns
+__global__ void k_fields_pack(Grid g, Fields f, float2 *cA, float *cD, float *wzt)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    cA[t] = make_float2(f.ax[t], f.ay[t]);
-    cD[3 * t] = f.lr[t];
-    cD[3 * t + 1] = f.pkz[t];
-    cD[3 * t + 2] = f.pky[t];
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = f.ftl * g.nz;
+    if (r >= n) return;
+    const int li = (int)(r & 63);
+    const long t = r >> 6;
+    const int bx = (int)(t % f.ftx);
+    const long jk = t / f.ftx;  // (j-1) + ny*(k-1)
+    const int ci = min(max(kFluxTileCols * bx - 2 + li, 1), g.nx);
+    const long src = (ci - 1) + (long)g.nx * jk;
+    cA[r] = make_float2(f.ax[src], f.ay[src]);
+    cD[3 * r] = f.lr[src];
+    cD[3 * r + 1] = f.pkz[src];
+    cD[3 * r + 2] = f.pky[src];
+    for (int ic = 0; ic < g.nc; ++ic) wzt[ic * n + r] = f.wz[ic * g.N + src];
 }
 
 // 1/Hm1^2 per column: the flux kernel forms rho*rkh1 = rho*rkhor/Hm1^2
@@ -787,13 +797,21 @@ int launch_fused(hipStream_t s, const Params &p0)
 
 }  // namespace
 
-// the interleaved flux-kernel groups lead the buffer ((ax, ay) 8-byte
-// aligned, then the 12-byte records), followed by the z-level table and the
-// separate fields
+// records per level of the flux kernel's tile-major layout
+static size_t tiled_level(const Grid &g)
+{
+    return (size_t)g.ny * (size_t)((g.nx + kFluxTileCols - 1) / kFluxTileCols) * 64;
+}
+
+// the flux kernel's tile-major groups lead the buffer ((ax, ay), the 12-byte
+// records, vz+vset per class: each starts on a multiple of 64 records, so
+// every region row is 256-byte aligned), followed by the z-level table and
+// the separate fields
 static size_t separate_offset(const Grid &g)
 {
     const size_t zl = (size_t)(g.nz + 2) * (sizeof(ZLev) / sizeof(float));
-    return ((size_t)5 * g.N + zl + 3) & ~(size_t)3;
+    const size_t t = tiled_level(g) * (size_t)g.nz;
+    return ((size_t)(5 + g.nc) * t + zl + 63) & ~(size_t)63;
 }
 
 size_t fields_floats(const Grid &g)
@@ -805,9 +823,13 @@ size_t fields_floats(const Grid &g)
 Fields fields_view(const Grid &g, const float *b0, const double *vsum1)
 {
     Fields f;
+    f.ftx = (g.nx + kFluxTileCols - 1) / kFluxTileCols;
+    f.ftl = (long)tiled_level(g);
+    const size_t t = (size_t)f.ftl * (size_t)g.nz;
     f.cA = reinterpret_cast<const float2 *>(b0);
-    f.cD = b0 + 2 * g.N;
-    f.zl = reinterpret_cast<const ZLev *>(b0 + 5 * g.N);
+    f.cD = b0 + 2 * t;
+    f.wzt = b0 + 5 * t;
+    f.zl = reinterpret_cast<const ZLev *>(b0 + (5 + g.nc) * t);
     f.base = b0;
     f.bytes = sizeof(float) * fields_floats(g);
     const float *b = b0 + separate_offset(g);
@@ -846,8 +868,10 @@ int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *v
     hipLaunchKernelGGL(k_fields_face, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g, vx,
                        vy, dt, const_cast<float *>(f.ax), const_cast<float *>(f.ay));
     F3D_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_fields_pack, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g.N, f,
-                       const_cast<float2 *>(f.cA), const_cast<float *>(f.cD));
+    const long nt = f.ftl * g.nz;
+    hipLaunchKernelGGL(k_fields_pack, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, g, f,
+                       const_cast<float2 *>(f.cA), const_cast<float *>(f.cD),
+                       const_cast<float *>(f.wzt));
     F3D_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_fields_h2, dim3((unsigned)((g.NXY + 255) / 256)), dim3(256), 0, s,
                        (long)g.NXY, hm1, const_cast<float *>(f.h2));
