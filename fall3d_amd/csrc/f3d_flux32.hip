@@ -203,16 +203,16 @@ enum : unsigned {
 // steps, the two SC planes
 constexpr int flux_planes(int mc, bool odd) { return 3 * mc + 2 + (odd ? 2 : 0); }
 
-template <int RY, int NW, int MC, bool ODD, bool EX, bool EY, bool PIPE_EDGE>
+template <int RY, int NW, int MC, bool ODD, bool EX, bool EY>
 struct FluxTile {
     static constexpr int NS = RY / NW;
     static constexpr bool EDGE = EX || EY;
     // groups of at most 768 lanes (a budget of 168 registers) run the
     // steady levels in pairs, prefetch c a level early and store each
-    // level's outputs after its loads; edge tiles only when PIPE_EDGE (the
-    // persistent batched kernel, which holds its work item and parameter
-    // block as well, would spill)
-    static constexpr bool PIPE = (!EDGE || PIPE_EDGE) && NW * 64 <= 768;
+    // level's outputs after its loads, interior and edge tiles alike; the
+    // single and the batched launch run the same tile code, so a scenario's
+    // result is the same bits in both
+    static constexpr bool PIPE = NW * 64 <= 768;
     static constexpr int PL = RX * RY;  // one LDS plane
     static_assert(RY % NW == 0, "region rows must divide evenly over the waves");
 
@@ -691,8 +691,8 @@ struct FluxTile {
     // PIPE tiles store the outputs of the last sweep (level s-2) after
     // every load of the level is issued: vector-memory counters complete in
     // order, so a store issued before a load makes every wait for that load
-    // also wait for the store's write.  (The others store at once: holding
-    // the outputs costs registers they do not have.)
+    // also wait for the store's write.  (1024-lane groups store at once:
+    // holding the outputs costs registers they do not have.)
     __device__ __forceinline__ bool stored(int sl, int c) const
     {
         return li >= 3 && li < RX - 3 && inner(sl) && (c == 0 || live[c]);
@@ -737,11 +737,11 @@ struct FluxTile {
     }
 };
 
-template <int RY, int NW, int MC, bool ODD, bool EX, bool EY, bool PIPE_EDGE = true>
+template <int RY, int NW, int MC, bool ODD, bool EX, bool EY>
 __device__ __forceinline__ void flux_tile(const FluxParams &p, const int *qv, const int gi0,
                                           const int gj0, float *sm, const ZLev *zt)
 {
-    FluxTile<RY, NW, MC, ODD, EX, EY, PIPE_EDGE> t(p, qv, gi0, gj0, sm, zt);
+    FluxTile<RY, NW, MC, ODD, EX, EY> t(p, qv, gi0, gj0, sm, zt);
     t.run();
 }
 
@@ -1014,14 +1014,14 @@ __global__ void __launch_bounds__(NW * 64)
         const bool ex = gi0 < 1 || gi0 + RX - 1 > p.g.nx;
         const bool ey = gj0 < 1 || gj0 + RY - 1 > p.g.ny;
         if (LIST == L_INTERIOR || (LIST == L_MIXED && !ex && !ey)) {
-            flux_tile<RY, NW, MC, ODD, false, false, false>(p, qv, gi0, gj0, sm, zt);
+            flux_tile<RY, NW, MC, ODD, false, false>(p, qv, gi0, gj0, sm, zt);
         } else {
             if (ex && ey)
-                flux_tile<RY, NW, MC, ODD, true, true, false>(p, qv, gi0, gj0, sm, zt);
+                flux_tile<RY, NW, MC, ODD, true, true>(p, qv, gi0, gj0, sm, zt);
             else if (ex)
-                flux_tile<RY, NW, MC, ODD, true, false, false>(p, qv, gi0, gj0, sm, zt);
+                flux_tile<RY, NW, MC, ODD, true, false>(p, qv, gi0, gj0, sm, zt);
             else
-                flux_tile<RY, NW, MC, ODD, false, true, false>(p, qv, gi0, gj0, sm, zt);
+                flux_tile<RY, NW, MC, ODD, false, true>(p, qv, gi0, gj0, sm, zt);
         }
         __syncthreads();  // LDS is reused by the next item
     }
