@@ -76,8 +76,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the path shards)", file=sys.stderr)
+        print("bench.py: measures one GPU only; --gpus above 1 is not supported (the path "
+              "would shard (member, class) slabs across GPUs, with one FP64 all-reduce of "
+              "each member's outflux)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
