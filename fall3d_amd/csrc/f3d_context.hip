@@ -446,6 +446,13 @@ static int meteo_stage(f3d_ctx *c)
     TRY(met::setvset(s, g, g.nc, c->modv, c->d_cls.as<met::ClassConst>(), c->d_tempe.as<double>(),
                      c->d_rho.as<double>(), c->d_visc.as<double>(), c->d_vset.as<double>(),
                      c->d_cd.as<double>(), c->d_status.as<int>(), 256, 64));
+    // meteo.f90:99-100 setvdrydep, on the unscaled density, when the land
+    // use is known (otherwise vdep is the caller's)
+    if (c->landuse_set)
+        TRY(met::setvdrydep(s, g, c->d_cls.as<met::ClassConst>(), c->zlayer[0],
+                            c->d_ldu.as<double>(), c->d_znt.as<double>(), c->d_rl.as<double>(),
+                            c->d_ustar.as<double>(), c->d_tempe.as<double>(),
+                            c->d_rho.as<double>(), c->d_vdep.as<double>()));
     // meteo.f90:108-112 Hm1 scaling
     TRY(met::scale(s, g, c->d_hm1.as<double>(), c->d_vx.as<double>(), c->d_rho.as<double>(),
                    c->d_rkhor.as<double>(), c->d_rkh1.as<double>()));
@@ -783,6 +790,40 @@ int f3d_set_state(f3d_ctx *c, const double *ch, const double *cdry)
         F3D_HIP(hipMemcpyAsync(c->d_cdry.p, cdry, sizeof(double) * g.NXY * (g.nc + 1),
                                hipMemcpyHostToDevice, s));
     F3D_HIP(hipStreamSynchronize(s));
+    return F3D_OK;
+}
+
+int f3d_set_landuse(f3d_ctx *c, const double *ldu, const double *znt)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (!ldu || !znt) {
+        set_error("f3d_set_landuse: null field");
+        return F3D_EINVAL;
+    }
+    const Grid &g = c->g;
+    const size_t b = sizeof(double) * g.NXY;
+    if (!c->d_ldu.p && (c->d_ldu.alloc(b, &c->bytes) || c->d_znt.alloc(b, &c->bytes))) {
+        set_error("f3d_set_landuse: out of device memory");
+        return F3D_ENOMEM;
+    }
+    TRY(upload(c->d_ldu, ldu, b, c->stream));
+    TRY(upload(c->d_znt, znt, b, c->stream));
+    F3D_HIP(hipStreamSynchronize(c->stream));
+    c->landuse_set = true;
+    return F3D_OK;
+}
+
+int f3d_get_vdep(f3d_ctx *c, double *vdep)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (!vdep) {
+        set_error("f3d_get_vdep: null output");
+        return F3D_EINVAL;
+    }
+    const Grid &g = c->g;
+    F3D_HIP(hipMemcpyAsync(vdep, c->d_vdep.p, sizeof(double) * g.NXY * g.nc,
+                           hipMemcpyDeviceToHost, c->stream));
+    F3D_HIP(hipStreamSynchronize(c->stream));
     return F3D_OK;
 }
 

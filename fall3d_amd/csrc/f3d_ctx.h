@@ -222,6 +222,10 @@ struct f3d_ctx {
     f3d::DeviceBuffer d_dz, d_ldz, d_zlayer, d_hm1, d_kmh, d_zc;
     // device: meteo and coefficients (FP64)
     f3d::DeviceBuffer d_vx, d_vy, d_vz, d_tempe, d_vptemp, d_rho, d_ustar, d_zi, d_rl, d_vdep;
+    // land use (USGS category) and roughness length per column: when set,
+    // every meteo read computes vdep on the device (meteo.f90:99-100)
+    f3d::DeviceBuffer d_ldu, d_znt;
+    bool landuse_set = false;
     f3d::DeviceBuffer d_rkhor, d_rkh1, d_rkver, d_vset, d_visc;
     f3d::DeviceBuffer d_cls, d_cd, d_status, d_dtcn;
     // device: state

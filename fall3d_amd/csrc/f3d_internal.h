@@ -172,6 +172,12 @@ struct ClassConst {
 int setvset(hipStream_t s, const Grid &g, int np, int modv, const ClassConst *cls,
             const double *tempe, const double *rho, double *visc, double *vset, double *cd_io,
             int *status, int chunk, int warm);
+// setvdrydep.f90:54-264 for every column and class (meteo.f90:99-100);
+// tempe and rho at level 1, before the Hm1 scaling; classes above 100 um
+// keep their vdep
+int setvdrydep(hipStream_t s, const Grid &g, const ClassConst *cls, double zlayer1,
+               const double *ldu, const double *znt, const double *rl, const double *ustar,
+               const double *tempe, const double *rho, double *vdep);
 int scale(hipStream_t s, const Grid &g, const double *hm1, double *vx, double *rho,
           const double *rkhor, double *rkh1);
 // dtcn = max over cells and classes of the CFL rate (finddt.f90:26-48);

@@ -295,6 +295,81 @@ __global__ void k_finddt(Grid g, const double *dz, double dX1, double dX2, const
 
 inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// setvdrydep.f90:54-264 (Feng 2008 dry deposition), one thread per column:
+// the aerodynamic resistance at the first level with the stability
+// corrections, Sutherland viscosity, the Re* term, and per class up to
+// 100 um the resistances in series.  Classes coarser than 100 um keep
+// their vdep.  Same expression order as the reference; the file is built
+// without contraction.
+__constant__ double kFengA[6][4] = {
+    {0.0048, 0.0315, 1.2891, 1.0338}, {0.0037, 0.0120, 1.3977, 1.0707},
+    {0.0042, 0.2928, 1.3970, 0.9155}, {0.0032, 0.1201, 1.1838, 1.0096},
+    {0.0043, 0.1337, 1.2834, 1.1595}, {0.0045, 0.0925, 1.2654, 1.0891}};
+__constant__ double kFengB[6][4] = {
+    {1.0, 2.7925, 2.6878, 1.2644}, {1.0, 2.2413, 2.5838, 1.3247},
+    {1.0, 3.8581, 2.5580, 1.0364}, {1.0, 3.4407, 2.8033, 1.2069},
+    {1.0, 3.5456, 2.7157, 1.4863}, {1.0, 3.2920, 2.7227, 1.4240}};
+
+// Feng mode of a diameter (setvdrydep.f90:134-152): 0 above 100 um
+__device__ __forceinline__ int feng_mode(double d)
+{
+    return d > 100e-6 ? 0 : (d > 10e-6 ? 4 : (d > 2.5e-6 ? 3 : (d > 0.1e-6 ? 2 : 1)));
+}
+
+// USGS land-use category, truncated as Fortran INT does, onto Feng's six
+// classes (setvdrydep.f90:167-216)
+__device__ __forceinline__ int feng_land(double ldu)
+{
+    const int u = (int)ldu;
+    if (u == 1) return 1;
+    if (u == 10 || u == 19) return 3;
+    if (u == 16) return 5;
+    if (u >= 20 && u <= 23) return 2;
+    if (u == 24) return 4;
+    return 6;
+}
+
+__global__ void k_setvdrydep(Grid g, int nc, const met::ClassConst *cls, double zlayer1,
+                             const double *ldu, const double *znt, const double *rl,
+                             const double *ustar, const double *tempe, const double *rho,
+                             double *vdep)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.NXY) return;
+    const double vkarm = 0.4, visc0 = 1.827e-5, Tair0 = 291.15;
+    const int land = feng_land(ldu[t]);
+    const double us = ustar[t], z0 = znt[t];
+    const double zeta = rl[t] * zlayer1, zeta0 = rl[t] * z0;
+    double ra;
+    if (us > 0.0) {
+        ra = log(zlayer1 / z0);
+        if (zeta > 0) {
+            ra = ra + 4.7 * (zeta - zeta0);
+        } else if (zeta < 0) {
+            const double e = pow(1.0 - 15.0 * zeta, 0.25);
+            const double e0 = pow(1.0 - 15.0 * zeta0, 0.25);
+            ra = ra + log((e0 * e0 + 1) * ((e0 + 1) * (e0 + 1)) / ((e * e + 1) * ((e + 1) * (e + 1))));
+            ra = ra + 2 * (atan(e) - atan(e0));
+        }
+        ra = ra / (us * vkarm);
+    } else {
+        ra = 1e20;
+    }
+    const double T = tempe[t];  // level 1
+    const double visc = visc0 * ((Tair0 + 120.0) / (T + 120.0)) * pow(T / Tair0, 1.5);
+    double Re = us * z0 * rho[t] / visc;
+    if (Re > 40300.0) Re = 40300.0;
+    const double x = (Re - 40300.0) / 15330.0;
+    const double vd1 = 0.0226 * us * exp(-(0.5 * x * x));
+    for (int ic = 0; ic < nc; ++ic) {
+        const int m = feng_mode(cls[ic].diam);
+        if (m == 0) continue;
+        double vd = vd1 + kFengA[land - 1][m - 1] * pow(us, kFengB[land - 1][m - 1]);
+        vd = ra + 1.0 / vd;
+        vdep[t + (long)g.NXY * ic] = -1.0 / vd;
+    }
+}
+
 }  // namespace
 
 namespace met {
@@ -322,6 +397,16 @@ int setvset(hipStream_t s, const Grid &g, int np, int modv, const ClassConst *cl
     const long nthreads = (P + chunk - 1) / chunk;
     hipLaunchKernelGGL(k_setvset, dim3(blocks_for(nthreads, 128)), dim3(128), 0, s, g, np, modv,
                        cls, visc, rho, vset, cd_io, status, P, chunk, warm);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+int setvdrydep(hipStream_t s, const Grid &g, const ClassConst *cls, double zlayer1,
+               const double *ldu, const double *znt, const double *rl, const double *ustar,
+               const double *tempe, const double *rho, double *vdep)
+{
+    hipLaunchKernelGGL(k_setvdrydep, dim3((unsigned)((g.NXY + 255) / 256)), dim3(256), 0, s, g,
+                       g.nc, cls, zlayer1, ldu, znt, rl, ustar, tempe, rho, vdep);
     F3D_CHECK_LAUNCH();
     return F3D_OK;
 }

@@ -70,6 +70,20 @@ module fall3d_amd
        real(c_double), intent(in) :: ustar(*), zi(*), rl(*), vdep(*)
      end function f3d_set_meteo
 
+     ! LDU and znt (Master.f90:96,108): later meteo reads run setvdrydep
+     ! (meteo.f90:99-100) on the device
+     integer(c_int) function f3d_set_landuse(ctx, ldu, znt) bind(C, name='f3d_set_landuse')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: ldu(*), znt(*)
+     end function f3d_set_landuse
+
+     integer(c_int) function f3d_get_vdep(ctx, vdep) bind(C, name='f3d_get_vdep')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: vdep(*)
+     end function f3d_get_vdep
+
      integer(c_int) function f3d_get_dt(ctx, dt, dtc) bind(C, name='f3d_get_dt')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

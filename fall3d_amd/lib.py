@@ -28,7 +28,8 @@ DROPIN_SYMBOLS = ["setbcc_", "advctzc_", "advctx_", "advcty_", "diffz_", "diffx_
 CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                "f3d_ensemble_budget", "f3d_get_member_state", "f3d_destroy", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
                "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
-               "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_get_clock",
+               "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_set_landuse",
+               "f3d_get_vdep", "f3d_get_clock",
                "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
                "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time", "f3d_device_bytes",
                "f3d_last_error", "f3d_hbm_copy", "f3d_batch_create", "f3d_batch_destroy",
@@ -79,6 +80,8 @@ def lib():
         L.f3d_get_state.argtypes = [vp, dp, dp]
         L.f3d_set_state.argtypes = [vp, dp, dp]
         L.f3d_get_coef.argtypes = [vp] + [dp] * 6
+        L.f3d_set_landuse.argtypes = [vp, dp, dp]
+        L.f3d_get_vdep.argtypes = [vp, dp]
         L.f3d_get_clock.argtypes = [vp, dp, ip, dp, dp]
         L.f3d_set_clock.argtypes = [vp, d, i, d, d]
         L.f3d_get_cd_state.argtypes = [vp, dp]
@@ -109,7 +112,8 @@ def lib():
         for name in ("f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                      "f3d_ensemble_budget", "f3d_get_member_state", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
                      "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
-                     "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_get_clock",
+                     "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_set_landuse",
+                     "f3d_get_vdep", "f3d_get_clock",
                      "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
                      "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time"):
             getattr(L, name).restype = c
@@ -278,6 +282,17 @@ class Context:
         vset = np.zeros((nx, ny, nz, nc), order="F")
         _check(lib().f3d_get_coef(self.h, *[_P(x) for x in a], _P(vset)), "f3d_get_coef")
         return dict(vx=a[0], rho=a[1], rkhor=a[2], rkh1=a[3], rkver=a[4], vset=vset)
+
+    def set_landuse(self, ldu, znt):
+        """Land use (USGS category) and roughness length per column: later
+        meteo reads compute vdep on the device (setvdrydep)."""
+        self._ldu, self._znt = _F(ldu), _F(znt)
+        _check(lib().f3d_set_landuse(self.h, _P(self._ldu), _P(self._znt)), "f3d_set_landuse")
+
+    def vdep(self):
+        out = np.zeros((self.nx, self.ny, self.nc), order="F")
+        _check(lib().f3d_get_vdep(self.h, _P(out)), "f3d_get_vdep")
+        return out
 
     def clock(self):
         t, it, em, om = ctypes.c_double(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double()

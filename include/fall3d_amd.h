@@ -157,6 +157,15 @@ int f3d_set_state(f3d_ctx *ctx, const double *c, const double *cdry);
  * vx/Hm1, rho*Hm1, rkhor, rkh1, rkver (nx,ny,nz) and vset (nx,ny,nz,nc). */
 int f3d_get_coef(f3d_ctx *ctx, double *vx, double *rho, double *rkhor, double *rkh1,
                  double *rkver, double *vset);
+/* Land use and roughness length per column (nx,ny): LDU (USGS category) and
+ * znt of Master.f90:96,108, the fields setvdrydep reads (setvdrydep.f90,
+ * called at meteo.f90:99-100).  Once set, every meteo read computes vdep on
+ * the device for the classes up to 100 um, on the density before the Hm1
+ * scaling; classes above 100 um keep the vdep f3d_set_meteo was given.
+ * Without it vdep is the caller's. */
+int f3d_set_landuse(f3d_ctx *ctx, const double *ldu, const double *znt);
+/* vdep (nx,ny,nc) as the last meteo read left it (Master vdep). */
+int f3d_get_vdep(f3d_ctx *ctx, double *vdep);
 int f3d_get_clock(f3d_ctx *ctx, double *time, int *iiter, double *erumass, double *outmass);
 int f3d_set_clock(f3d_ctx *ctx, double time, int iiter, double erumass, double outmass);
 /* The SAVEd vsettl drag coefficient carried between meteo reads

@@ -734,6 +734,92 @@ void or_meteo_scale(int nx, int ny, int nz, const double *Hm1, double *vx, doubl
             }
 }
 
+/* setvdrydep.f90:54-264: dry deposition velocity after Feng (JGR 113,
+ * 2008) for classes up to 100 um.  Mode by diameter (:134-152), the USGS
+ * land-use class folded onto Feng's six categories (:167-216), the
+ * aerodynamic resistance from the log profile with the stable / unstable
+ * corrections at the first level (:218-237), Sutherland viscosity with the
+ * FP64 reference temperature (:240-241), Re* clamped at 40300 (:245-246),
+ * the first term (:250-251), and per class the resistances in series
+ * (:256-260).  Classes coarser than 100 um keep their vdep.  tempe1 and
+ * rho1 are the first level of tempe and rho (before the Hm1 scaling).
+ * The coefficients are Feng's table as the reference sets them (:76-123). */
+static const double kFengA[6][4] = {
+    {0.0048, 0.0315, 1.2891, 1.0338}, {0.0037, 0.0120, 1.3977, 1.0707},
+    {0.0042, 0.2928, 1.3970, 0.9155}, {0.0032, 0.1201, 1.1838, 1.0096},
+    {0.0043, 0.1337, 1.2834, 1.1595}, {0.0045, 0.0925, 1.2654, 1.0891}};
+static const double kFengB[6][4] = {
+    {1.0, 2.7925, 2.6878, 1.2644}, {1.0, 2.2413, 2.5838, 1.3247},
+    {1.0, 3.8581, 2.5580, 1.0364}, {1.0, 3.4407, 2.8033, 1.2069},
+    {1.0, 3.5456, 2.7157, 1.4863}, {1.0, 3.2920, 2.7227, 1.4240}};
+
+static int feng_mode(double d)
+{
+    if (d > 100e-6) return 0;
+    if (d > 10e-6) return 4;
+    if (d > 2.5e-6) return 3;
+    if (d > 0.1e-6) return 2;
+    return 1;
+}
+
+/* USGS category (truncated like Fortran INT) -> Feng land class 1..6 */
+static int feng_land(double ldu)
+{
+    const int u = (int)ldu;
+    if (u == 1) return 1;
+    if (u == 10 || u == 19) return 3;
+    if (u == 16) return 5;
+    if (u >= 20 && u <= 23) return 2;
+    if (u == 24) return 4;
+    return 6;
+}
+
+void or_setvdrydep(int nx, int ny, int nc, double zlayer1, const double *diam,
+                   const double *ldu, const double *znt, const double *rl, const double *ustar,
+                   const double *tempe1, const double *rho1, double *vdep)
+{
+    const double vkarm = 0.4, visc0 = 1.827e-5, Tair0 = 291.15;
+    int any = 0;
+    for (int ic = 0; ic < nc; ++ic) any |= feng_mode(diam[ic]) > 0;
+    if (!any) return;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            const long t = PI2(i, j);
+            const int land = feng_land(ldu[t]);
+            const double us = ustar[t], z0 = znt[t];
+            const double zeta = rl[t] * zlayer1, zeta0 = rl[t] * z0;
+            double ra;
+            if (us > 0.0) {
+                ra = log(zlayer1 / z0);
+                if (zeta > 0) {
+                    ra = ra + 4.7 * (zeta - zeta0);
+                } else if (zeta < 0) {
+                    const double e = pow(1.0 - 15.0 * zeta, 0.25);
+                    const double e0 = pow(1.0 - 15.0 * zeta0, 0.25);
+                    ra = ra + log((e0 * e0 + 1) * ((e0 + 1) * (e0 + 1)) /
+                                  ((e * e + 1) * ((e + 1) * (e + 1))));
+                    ra = ra + 2 * (atan(e) - atan(e0));
+                }
+                ra = ra / (us * vkarm);
+            } else {
+                ra = 1e20;
+            }
+            const double T = tempe1[t];
+            const double visc = visc0 * ((Tair0 + 120.0) / (T + 120.0)) * pow(T / Tair0, 1.5);
+            double Re = us * z0 * rho1[t] / visc;
+            if (Re > 40300.0) Re = 40300.0;
+            const double x = (Re - 40300.0) / 15330.0;
+            const double vd1 = 0.0226 * us * exp(-(0.5 * x * x));
+            for (int ic = 0; ic < nc; ++ic) {
+                const int m = feng_mode(diam[ic]);
+                if (m == 0) continue;
+                double vd = vd1 + kFengA[land - 1][m - 1] * pow(us, kFengB[land - 1][m - 1]);
+                vd = ra + 1.0 / vd;
+                vdep[t + (long)nx * ny * ic] = -1.0 / vd;
+            }
+        }
+}
+
 /* setsrc.f90:20-35 (1-based source indices) */
 void or_setsrc(double *c, int nx, int ny, int nz, int nc, int nsou, const int *isrc,
                const int *jsrc, const int *ksrc, const double *tmrat, const double *dz,

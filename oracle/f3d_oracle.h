@@ -67,6 +67,12 @@ long or_setvset(int nx, int ny, int nz, int np, const double *tempe, const doubl
 void or_meteo_scale(int nx, int ny, int nz, const double *Hm1, double *vx, double *rho,
                     const double *rkhor, double *rkh1);
 
+/* setvdrydep.f90:54-264 (tempe1, rho1: first level, before the Hm1
+ * scaling; vdep(nx,ny,nc) in/out: classes above 100 um keep theirs) */
+void or_setvdrydep(int nx, int ny, int nc, double zlayer1, const double *diam,
+                   const double *ldu, const double *znt, const double *rl, const double *ustar,
+                   const double *tempe1, const double *rho1, double *vdep);
+
 void or_setsrc(double *c, int nx, int ny, int nz, int nc, int nsou, const int *isrc,
                const int *jsrc, const int *ksrc, const double *tmrat, const double *dz,
                const double *zlayer, const double *Hm1, double dX1, double dX2, double dt);

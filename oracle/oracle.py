@@ -52,6 +52,7 @@ def lib():
         L.or_get_gama.argtypes = [d, d, dp]
         L.or_ganser_consts.argtypes = [d, d, d, dp, dp]
         L.or_meteo_scale.argtypes = [i, i, i, dp, dp, dp, dp, dp]
+        L.or_setvdrydep.argtypes = [i, i, i, d, dp, dp, dp, dp, dp, dp, dp, dp]
         L.or_setsrc.argtypes = [dp, i, i, i, i, i, ip, ip, ip, dp, dp, dp, dp, d, d, d]
         L.or_sizloop.argtypes = [dp, dp, i, i, i, i, i, d, dp, dp, dp, dp, dp, dp, dp, dp, dp,
                                  dp, d, d, dp, i]
@@ -213,6 +214,7 @@ class RefRun(_RunBase):
             L.ref_set_state.argtypes = [dp, dp]
             L.ref_get_coef.argtypes = [dp] * 6
             L.ref_set_coef.argtypes = [dp] * 9
+            L.ref_setvdrydep.argtypes = [dp, dp, dp]
             L.ref_set_clock.argtypes = [d, i, d, d]
             cls._L = L
         return cls._L

@@ -10,7 +10,7 @@
 ! reference routines in the order the reference driver does:
 !   Fall3d.f90:112-118   time += dt; iiter += 1; erumass += flowrate*dt;
 !                        setsrc; sizloop
-!   meteo.f90:27,81      kappa3, setvset
+!   meteo.f90:27,81,100  kappa3, setvset, setvdrydep
 !   meteo.f90:108-112    Hm1 scaling of vx, rho and rkh1 (three array
 !                        statements; meteo itself cannot be linked because
 !                        it calls the netCDF reader reamet)
@@ -112,6 +112,20 @@ contains
     vx = vx_; vy = vy_; vz = vz_; tempe = tempe_; vptemp = vptemp_; rho = rho_
     ustar = ustar_; zi = zi_; rl = rl_; vdep = vdep_; topg = topg_
   end subroutine ref_set_meteo
+
+  ! meteo.f90:99-100: setvdrydep from the land use LDU and the roughness
+  ! length znt; its other inputs (rl, ustar, tempe, rho, zlayer, diam) are
+  ! the module state set above.  vdep_ is in/out: classes coarser than
+  ! 100 um keep their value.
+  subroutine ref_setvdrydep(ldu_, znt_, vdep_) bind(C, name='ref_setvdrydep')
+    real(c_double) :: ldu_(nx,ny), znt_(nx,ny), vdep_(nx,ny,nc)
+    if (allocated(LDU)) deallocate(LDU)
+    if (allocated(znt)) deallocate(znt)
+    allocate(LDU(nx,ny), znt(nx,ny))
+    LDU = ldu_; znt = znt_; vdep = vdep_
+    call setvdrydep
+    vdep_ = vdep
+  end subroutine ref_setvdrydep
 
   subroutine ref_kappa3() bind(C, name='ref_kappa3')
     call kappa3

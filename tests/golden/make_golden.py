@@ -142,13 +142,62 @@ def make_routines():
     np.savez_compressed(os.path.join(OUT, "routines_9x7x6.npz"), **rec)
 
 
+def make_drydep():
+    """setvdrydep (meteo.f90:99-100) on seeded first-level fields: every
+    USGS land-use category and out-of-range ones, stable, neutral and
+    unstable columns, calm columns (ustar = 0), all four Feng modes and a
+    class above 100 um (its vdep must stay as it was), at three heights of
+    the first level (zlayer(1) = 0 gives vdep = +0)."""
+    L = RefRun.L()
+    rng = np.random.default_rng(77)
+    nx, ny, nz, nc = 12, 9, 4, 6
+    diam = np.array([0.05e-6, 1.0e-6, 5.0e-6, 50.0e-6, 200.0e-6, 2.5e-6])
+    rhop = np.full(nc, 2000.0)
+    sphe = np.full(nc, 0.9)
+    L.ref_alloc(nx, ny, nz, nc, 1)
+    L.ref_set_models(S.KV_CONSTANT, S.KH_CONSTANT, S.VS_GANSER, 100.0, 1000.0)
+    L.ref_set_classes(P(F(diam)), P(F(rhop)), P(F(sphe)))
+    ldu = rng.integers(1, 25, (nx, ny)).astype(np.float64)
+    ldu[0, :4] = [-99.0, 0.0, 25.0, 16.7]
+    ldu = F(ldu)
+    znt = F(rng.uniform(0.001, 1.5, (nx, ny)))
+    rl = rng.uniform(-0.05, 0.05, (nx, ny))
+    rl[rng.random((nx, ny)) < 0.15] = 0.0
+    rl = F(rl)
+    ustar = rng.uniform(0.0, 0.8, (nx, ny))
+    ustar[rng.random((nx, ny)) < 0.1] = 0.0
+    ustar = F(ustar)
+    tempe = F(rng.uniform(250.0, 310.0, (nx, ny, nz)))
+    rho = F(rng.uniform(0.9, 1.3, (nx, ny, nz)))
+    zero3 = F(np.zeros((nx, ny, nz)))
+    zero2 = F(np.zeros((nx, ny)))
+    rec = dict(nx=nx, ny=ny, nc=nc, diam=diam, ldu=ldu, znt=znt, rl=rl, ustar=ustar,
+               tempe1=F(tempe[:, :, 0]), rho1=F(rho[:, :, 0]), vdep0=np.full((nx, ny, nc), 7.0))
+    z1s = [10.0, 40.0, 0.0]
+    rec["zlayer1"] = np.array(z1s)
+    for n, z1 in enumerate(z1s):
+        zl = F(z1 + np.arange(nz) * 500.0)
+        L.ref_set_grid(3000.0, 3000.0, P(zl))
+        vdep = F(np.full((nx, ny, nc), 7.0))
+        L.ref_set_meteo(P(zero3), P(zero3), P(zero3), P(tempe), P(tempe), P(rho), P(ustar),
+                        P(zero2), P(rl), P(vdep), P(zero2))
+        L.ref_setvdrydep(P(ldu), P(znt), P(vdep))
+        rec[f"vdep_{n}"] = vdep.copy(order="F")
+    np.savez_compressed(os.path.join(OUT, "drydep_12x9x6.npz"), **rec)
+
+
 if __name__ == "__main__":
     if not os.path.exists(REF_PATH):
         sys.exit(f"{REF_PATH} missing: build it with `make -C oracle ref`")
     if len(sys.argv) > 1:
         which = sys.argv[1]
-        make_routines() if which == "routines" else make_run(which)
+        if which == "routines":
+            make_routines()
+        elif which == "drydep":
+            make_drydep()
+        else:
+            make_run(which)
     else:
-        for name in ["routines"] + list(RUNS):
+        for name in ["routines", "drydep"] + list(RUNS):
             subprocess.run([sys.executable, os.path.abspath(__file__), name], check=True)
             print("wrote", name)

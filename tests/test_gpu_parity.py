@@ -231,6 +231,73 @@ def test_fp64_vs_oracle_lonlat_hm1():
         r.close()
 
 
+def test_setvdrydep_vs_reference():
+    # setvdrydep on the device (meteo.f90:99-100) against the reference's
+    # own vectors: every land-use category, stable / neutral / unstable /
+    # calm columns, the four Feng modes, a class above 100 um (untouched),
+    # three first-level heights.  Device log/pow/atan/exp: 1e-13 relative.
+    d = np.load(os.path.join(GOLDEN, "drydep_12x9x6.npz"))
+    nx, ny, nc, nz = int(d["nx"]), int(d["ny"]), int(d["nc"]), 4
+    tempe = np.repeat(d["tempe1"][:, :, None], nz, axis=2)
+    rho = np.repeat(d["rho1"][:, :, None], nz, axis=2)
+    zero3 = np.zeros((nx, ny, nz))
+    for n, z1 in enumerate(d["zlayer1"]):
+        zl = z1 + np.arange(nz) * 500.0
+        ctx = F.Context(nx, ny, nz, nc, 3000.0, 3000.0, zl, precision=F.FP64)
+        try:
+            ctx.set_classes(d["diam"], np.full(nc, 2000.0), np.full(nc, 0.9), S.VS_GANSER)
+            ctx.set_models(S.KV_CONSTANT, S.KH_CONSTANT, 100.0, 1000.0, 0.2275)
+            ctx.set_landuse(d["ldu"], d["znt"])
+            ctx.set_meteo(zero3, zero3, zero3, tempe, tempe, rho, d["ustar"],
+                          np.zeros((nx, ny)), d["rl"], d["vdep0"])
+            got, ref = ctx.vdep(), d[f"vdep_{n}"]
+            assert np.all(np.abs(got - ref) <= 1e-13 * np.abs(ref)), n
+            assert (got[:, :, 4] == 7.0).all()
+        finally:
+            ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_run_with_device_drydep_vs_oracle(prec):
+    # a run whose vdep comes from setvdrydep on the device, against the
+    # oracle run given the oracle's own setvdrydep: first level 25 m above
+    # the ground (vdep nonzero), mixed land use, unstable and stable columns
+    from oracle.oracle import lib as olib
+    from oracle.oracle import P as OP
+    sc = S.config_small(nx=70, ny=31, nz=8, nc=4, seed=45, mean_u=5.0, mean_v=-2.0)
+    sc.zlayer[0] = 25.0
+    rng = np.random.default_rng(46)
+    sc.rl[:] = rng.uniform(-0.02, 0.02, sc.rl.shape)
+    sc.ustar[:] = rng.uniform(0.05, 0.6, sc.ustar.shape)
+    ldu = np.asfortranarray(rng.integers(1, 25, (sc.nx, sc.ny)).astype(np.float64))
+    znt = np.asfortranarray(rng.uniform(0.01, 1.0, (sc.nx, sc.ny)))
+    vdep = np.zeros((sc.nx, sc.ny, sc.nc), order="F")
+    Fa = lambda a: np.asfortranarray(a, dtype=np.float64)  # noqa: E731
+    olib().or_setvdrydep(sc.nx, sc.ny, sc.nc, float(sc.zlayer[0]), OP(Fa(sc.diam)), OP(ldu),
+                         OP(znt), OP(Fa(sc.rl)), OP(Fa(sc.ustar)), OP(Fa(sc.tempe[:, :, 0])),
+                         OP(Fa(sc.rho[:, :, 0])), OP(vdep))
+    assert np.any(vdep[:, :, :] < 0.0)
+    o = OracleRun(dataclasses.replace(sc, vdep=vdep))
+    o.meteo_update()
+    for _ in range(9):
+        o.step()
+    r = F.Run(sc, precision=F.FP64 if prec == "fp64" else F.FP32)
+    try:
+        r.ctx.set_landuse(ldu, znt)
+        r.meteo_update()
+        assert np.all(np.abs(r.ctx.vdep() - vdep) <= 1e-13 * np.abs(vdep))
+        r.step(9)
+        c, cdry = r.state()
+        if prec == "fp64":
+            assert rel_linf(interior(c), interior(o.c)) <= 1e-11
+            assert rel_linf(cdry[:, :, 1:], o.cdry[:, :, 1:]) <= 1e-11
+        else:
+            assert rel_l1(interior(c), interior(o.c)) <= 1e-4
+            assert rel_l1(cdry[:, :, 1:].sum(-1), o.cdry[:, :, 1:].sum(-1)) <= 1e-4
+    finally:
+        r.close()
+
+
 @pytest.mark.parametrize("tile", TILES, ids=_tile_id)
 def test_fp32_every_tile_shape(tile):
     # a grid larger than every tile in both directions (interior and edge

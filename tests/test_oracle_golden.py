@@ -117,3 +117,24 @@ def test_oracle_matches_reference_random(seed):
     out = subprocess.run([sys.executable, "-c", _RANDOM_CHECK, root, str(seed)],
                          capture_output=True, text=True, check=True).stdout
     assert "EQUAL" in out
+
+
+def test_setvdrydep_bitexact_vs_reference():
+    # setvdrydep.f90 (meteo.f90:99-100) on every land-use category, stable,
+    # neutral, unstable and calm columns, the four Feng modes and a class
+    # above 100 um, at three heights of the first level; bit patterns must
+    # match (including the signed zeros of zlayer(1) = 0 and ustar = 0)
+    from oracle.oracle import lib as olib
+    from oracle.oracle import P as OP
+    d = np.load(os.path.join(GOLDEN, "drydep_12x9x6.npz"))
+    nx, ny, nc = int(d["nx"]), int(d["ny"]), int(d["nc"])
+    F_ = lambda a: np.asfortranarray(a, dtype=np.float64)  # noqa: E731
+    for n, z1 in enumerate(d["zlayer1"]):
+        vdep = F_(d["vdep0"]).copy(order="F")
+        olib().or_setvdrydep(nx, ny, nc, float(z1), OP(F_(d["diam"])), OP(F_(d["ldu"])),
+                             OP(F_(d["znt"])), OP(F_(d["rl"])), OP(F_(d["ustar"])),
+                             OP(F_(d["tempe1"])), OP(F_(d["rho1"])), OP(vdep))
+        ref = d[f"vdep_{n}"]
+        assert np.array_equal(vdep.view(np.uint64), np.asfortranarray(ref).view(np.uint64)), n
+    assert (d["vdep_0"][:, :, 4] == 7.0).all()  # the 200 um class is untouched
+
