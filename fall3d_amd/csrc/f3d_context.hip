@@ -456,6 +456,10 @@ static int meteo_stage(f3d_ctx *c)
     // meteo.f90:108-112 Hm1 scaling
     TRY(met::scale(s, g, c->d_hm1.as<double>(), c->d_vx.as<double>(), c->d_rho.as<double>(),
                    c->d_rkhor.as<double>(), c->d_rkh1.as<double>()));
+    // meteo.f90:116-124 corvver (LON-LAT grids; flat terrain leaves vz as it is)
+    if (c->lonlat && c->topg_set)
+        TRY(met::corvver(s, g, c->d_hm1.as<double>(), c->d_topg.as<double>(), g.dx, g.dy,
+                         c->d_vx.as<double>(), c->d_vy.as<double>(), c->d_vz.as<double>()));
     // meteo.f90:149 finddt
     F3D_HIP(hipMemsetAsync(c->d_dtcn.p, 0, sizeof(unsigned long long), s));
     TRY(met::finddt(s, g, c->d_dz.as<double>(), g.dx, g.dy, c->d_vx.as<double>(),
@@ -810,6 +814,24 @@ int f3d_set_landuse(f3d_ctx *c, const double *ldu, const double *znt)
     TRY(upload(c->d_znt, znt, b, c->stream));
     F3D_HIP(hipStreamSynchronize(c->stream));
     c->landuse_set = true;
+    return F3D_OK;
+}
+
+int f3d_set_topography(f3d_ctx *c, const double *topg)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (!topg) {
+        set_error("f3d_set_topography: null field");
+        return F3D_EINVAL;
+    }
+    const size_t b = sizeof(double) * c->g.NXY;
+    if (!c->d_topg.p && c->d_topg.alloc(b, &c->bytes)) {
+        set_error("f3d_set_topography: out of device memory");
+        return F3D_ENOMEM;
+    }
+    TRY(upload(c->d_topg, topg, b, c->stream));
+    F3D_HIP(hipStreamSynchronize(c->stream));
+    c->topg_set = true;
     return F3D_OK;
 }
 

@@ -226,6 +226,10 @@ struct f3d_ctx {
     // every meteo read computes vdep on the device (meteo.f90:99-100)
     f3d::DeviceBuffer d_ldu, d_znt;
     bool landuse_set = false;
+    // terrain height per column (nx,ny): on LON-LAT grids every meteo read
+    // corrects vz for the terrain slope (corvver, meteo.f90:116-124)
+    f3d::DeviceBuffer d_topg;
+    bool topg_set = false;
     f3d::DeviceBuffer d_rkhor, d_rkh1, d_rkver, d_vset, d_visc;
     f3d::DeviceBuffer d_cls, d_cd, d_status, d_dtcn;
     // device: state

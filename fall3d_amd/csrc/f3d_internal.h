@@ -180,6 +180,9 @@ int setvdrydep(hipStream_t s, const Grid &g, const ClassConst *cls, double zlaye
                const double *tempe, const double *rho, double *vdep);
 int scale(hipStream_t s, const Grid &g, const double *hm1, double *vx, double *rho,
           const double *rkhor, double *rkh1);
+// corvver.f90:16-36 on the scaled vx (LON-LAT grids, meteo.f90:116-124)
+int corvver(hipStream_t s, const Grid &g, const double *hm1, const double *topg, double dX1,
+            double dX2, const double *vx, const double *vy, double *vz);
 // dtcn = max over cells and classes of the CFL rate (finddt.f90:26-48);
 // dtcn_bits is a uint64 holding the bit pattern of a non-negative double.
 int finddt(hipStream_t s, const Grid &g, const double *dz, double dX1, double dX2,

@@ -370,6 +370,26 @@ __global__ void k_setvdrydep(Grid g, int nc, const met::ClassConst *cls, double 
     }
 }
 
+// corvver.f90:16-36 (LON-LAT grids, meteo.f90:116-124), one thread per
+// cell: vz minus the terrain-slope projection of the scaled horizontal wind,
+// forward differences of topg clamped at the last column / row
+__global__ void k_corvver(Grid g, const double *hm1, const double *topg, double dX1, double dX2,
+                          const double *vx, const double *vy, double *vz)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.N) return;
+    const long c = t % g.NXY;
+    const int i = (int)(c % g.nx) + 1, j = (int)(c / g.nx) + 1;
+    const double dyinv = 1.0 / dX2;
+    const double dxinv = 1.0 / (hm1[c] * dX1);
+    const long ir = (i < g.nx ? i : g.nx - 1) + (long)g.nx * (j - 1);  // (min(i+1,nx), j)
+    const long ju = (i - 1) + (long)g.nx * (j < g.ny ? j : g.ny - 1);  // (i, min(j+1,ny))
+    const double zx = dxinv * (topg[ir] - topg[c]);
+    const double zy = dyinv * (topg[ju] - topg[c]);
+    const double vs = zx * vx[t] + zy * vy[t];
+    vz[t] = vz[t] - vs;
+}
+
 }  // namespace
 
 namespace met {
@@ -407,6 +427,15 @@ int setvdrydep(hipStream_t s, const Grid &g, const ClassConst *cls, double zlaye
 {
     hipLaunchKernelGGL(k_setvdrydep, dim3((unsigned)((g.NXY + 255) / 256)), dim3(256), 0, s, g,
                        g.nc, cls, zlayer1, ldu, znt, rl, ustar, tempe, rho, vdep);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+int corvver(hipStream_t s, const Grid &g, const double *hm1, const double *topg, double dX1,
+            double dX2, const double *vx, const double *vy, double *vz)
+{
+    hipLaunchKernelGGL(k_corvver, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g, hm1, topg,
+                       dX1, dX2, vx, vy, vz);
     F3D_CHECK_LAUNCH();
     return F3D_OK;
 }

@@ -78,6 +78,13 @@ module fall3d_amd
        real(c_double), intent(in) :: ldu(*), znt(*)
      end function f3d_set_landuse
 
+     ! topg (Master): LON-LAT grids run corvver (meteo.f90:116-124)
+     integer(c_int) function f3d_set_topography(ctx, topg) bind(C, name='f3d_set_topography')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: topg(*)
+     end function f3d_set_topography
+
      integer(c_int) function f3d_get_vdep(ctx, vdep) bind(C, name='f3d_get_vdep')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

@@ -29,7 +29,7 @@ CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                "f3d_ensemble_budget", "f3d_get_member_state", "f3d_destroy", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
                "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_set_landuse",
-               "f3d_get_vdep", "f3d_get_clock",
+               "f3d_get_vdep", "f3d_set_topography", "f3d_get_clock",
                "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
                "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time", "f3d_device_bytes",
                "f3d_last_error", "f3d_hbm_copy", "f3d_batch_create", "f3d_batch_destroy",
@@ -82,6 +82,7 @@ def lib():
         L.f3d_get_coef.argtypes = [vp] + [dp] * 6
         L.f3d_set_landuse.argtypes = [vp, dp, dp]
         L.f3d_get_vdep.argtypes = [vp, dp]
+        L.f3d_set_topography.argtypes = [vp, dp]
         L.f3d_get_clock.argtypes = [vp, dp, ip, dp, dp]
         L.f3d_set_clock.argtypes = [vp, d, i, d, d]
         L.f3d_get_cd_state.argtypes = [vp, dp]
@@ -113,7 +114,7 @@ def lib():
                      "f3d_ensemble_budget", "f3d_get_member_state", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
                      "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                      "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_set_landuse",
-                     "f3d_get_vdep", "f3d_get_clock",
+                     "f3d_get_vdep", "f3d_set_topography", "f3d_get_clock",
                      "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
                      "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time"):
             getattr(L, name).restype = c
@@ -289,6 +290,12 @@ class Context:
         self._ldu, self._znt = _F(ldu), _F(znt)
         _check(lib().f3d_set_landuse(self.h, _P(self._ldu), _P(self._znt)), "f3d_set_landuse")
 
+    def set_topography(self, topg):
+        """Terrain height per column: LON-LAT grids correct vz for its slope
+        at every later meteo read (corvver)."""
+        self._topg = _F(topg)
+        _check(lib().f3d_set_topography(self.h, _P(self._topg)), "f3d_set_topography")
+
     def vdep(self):
         out = np.zeros((self.nx, self.ny, self.nc), order="F")
         _check(lib().f3d_get_vdep(self.h, _P(out)), "f3d_get_vdep")
@@ -346,8 +353,11 @@ class Run:
 
     def __init__(self, sc, precision=FP32, cd_state=1.0, tile=None):
         self.sc = sc
+        lonlat = int(getattr(sc, "lonlat", 0))
         self.ctx = Context(sc.nx, sc.ny, sc.nz, sc.nc, sc.dx, sc.dy, sc.zlayer, precision,
-                           hm1=getattr(sc, "hm1_cols", None))
+                           hm1=getattr(sc, "hm1_cols", None), lonlat=lonlat)
+        if lonlat:
+            self.ctx.set_topography(sc.topg)
         if tile is not None:
             self.ctx.set_tile(*tile)
         self.ctx.set_classes(sc.diam, sc.rhop, sc.sphe, sc.modv)

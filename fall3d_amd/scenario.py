@@ -77,6 +77,9 @@ class Scenario:
     # Hm1 per column (nx,ny) for LON-LAT grids (sin of the colatitude,
     # reagrd.f90:247-270); None: UTM, Hm1 = 1
     hm1_cols: Optional[np.ndarray] = None
+    # LON-LAT coordinates: kappa3's CMAQ constant and the terrain-slope
+    # correction of vz (corvver) follow them
+    lonlat: int = 0
 
     @property
     def dz(self) -> np.ndarray:

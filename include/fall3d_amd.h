@@ -164,6 +164,12 @@ int f3d_get_coef(f3d_ctx *ctx, double *vx, double *rho, double *rkhor, double *r
  * scaling; classes above 100 um keep the vdep f3d_set_meteo was given.
  * Without it vdep is the caller's. */
 int f3d_set_landuse(f3d_ctx *ctx, const double *ldu, const double *znt);
+/* Terrain height per column (nx,ny), Master topg.  On LON-LAT grids
+ * (f3d_grid.lonlat = 1) every later meteo read runs corvver
+ * (corvver.f90, called at meteo.f90:116-124): vz minus the terrain-slope
+ * projection of the scaled horizontal wind.  Without it the terrain is
+ * taken as flat (no correction); UTM grids never correct. */
+int f3d_set_topography(f3d_ctx *ctx, const double *topg);
 /* vdep (nx,ny,nc) as the last meteo read left it (Master vdep). */
 int f3d_get_vdep(f3d_ctx *ctx, double *vdep);
 int f3d_get_clock(f3d_ctx *ctx, double *time, int *iiter, double *erumass, double *outmass);

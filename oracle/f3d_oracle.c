@@ -820,6 +820,25 @@ void or_setvdrydep(int nx, int ny, int nc, double zlayer1, const double *diam,
         }
 }
 
+/* corvver.f90:16-36 (meteo.f90:116-124, LON-LAT grids only): the vertical
+ * wind minus the terrain-slope projection of the horizontal wind, on the
+ * scaled vx; forward differences of topg, clamped at the last column/row */
+void or_corvver(int nx, int ny, int nz, const double *Hm1, const double *topg, double dX1,
+                double dX2, const double *vx, const double *vy, double *vz)
+{
+    const double dyinv = 1.0 / dX2;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            const double dxinv = 1.0 / (Hm1[PI2(i, j)] * dX1);
+            const double zx = dxinv * (topg[PI2(i < nx ? i + 1 : nx, j)] - topg[PI2(i, j)]);
+            const double zy = dyinv * (topg[PI2(i, j < ny ? j + 1 : ny)] - topg[PI2(i, j)]);
+            for (int k = 1; k <= nz; ++k) {
+                const double vs = zx * vx[MI(i, j, k)] + zy * vy[MI(i, j, k)];
+                vz[MI(i, j, k)] = vz[MI(i, j, k)] - vs;
+            }
+        }
+}
+
 /* setsrc.f90:20-35 (1-based source indices) */
 void or_setsrc(double *c, int nx, int ny, int nz, int nc, int nsou, const int *isrc,
                const int *jsrc, const int *ksrc, const double *tmrat, const double *dz,

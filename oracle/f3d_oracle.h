@@ -73,6 +73,10 @@ void or_setvdrydep(int nx, int ny, int nc, double zlayer1, const double *diam,
                    const double *ldu, const double *znt, const double *rl, const double *ustar,
                    const double *tempe1, const double *rho1, double *vdep);
 
+/* corvver.f90:16-36 on the scaled vx (LON-LAT grids) */
+void or_corvver(int nx, int ny, int nz, const double *Hm1, const double *topg, double dX1,
+                double dX2, const double *vx, const double *vy, double *vz);
+
 void or_setsrc(double *c, int nx, int ny, int nz, int nc, int nsou, const int *isrc,
                const int *jsrc, const int *ksrc, const double *tmrat, const double *dz,
                const double *zlayer, const double *Hm1, double dX1, double dX2, double dt);

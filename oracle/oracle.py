@@ -53,6 +53,7 @@ def lib():
         L.or_ganser_consts.argtypes = [d, d, d, dp, dp]
         L.or_meteo_scale.argtypes = [i, i, i, dp, dp, dp, dp, dp]
         L.or_setvdrydep.argtypes = [i, i, i, d, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.or_corvver.argtypes = [i, i, i, dp, dp, d, d, dp, dp, dp]
         L.or_setsrc.argtypes = [dp, i, i, i, i, i, ip, ip, ip, dp, dp, dp, dp, d, d, d]
         L.or_sizloop.argtypes = [dp, dp, i, i, i, i, i, d, dp, dp, dp, dp, dp, dp, dp, dp, dp,
                                  dp, d, d, dp, i]
@@ -123,14 +124,15 @@ class OracleRun(_RunBase):
         self.tmrat = F(sc.tmrat)  # (nsou,nc); UTM: tmrat*Hm1 = tmrat (source.f90:75-77)
 
     def meteo_update(self):
-        """kappa3 -> setvset -> (setvdrydep: vdep given) -> Hm1 scaling -> finddt
-        (meteo.f90:27,81,100,108-112,149)."""
+        """kappa3 -> setvset -> (setvdrydep: vdep given) -> Hm1 scaling ->
+        corvver (LON-LAT) -> finddt (meteo.f90:27,81,100,108-112,116-124,149)."""
         L, sc = lib(), self.sc
         nx, ny, nz, nc = self.nx, self.ny, self.nz, self.nc
         self.vx = F(sc.vx).copy(order="F")
         self.vy = F(sc.vy)
-        self.vz = F(sc.vz)
+        self.vz = F(sc.vz).copy(order="F")
         self.rho = F(sc.rho).copy(order="F")
+        lonlat = int(getattr(sc, "lonlat", 0))
         self.rkver = np.zeros((nx, ny, nz), order="F")
         self.rkhor = np.zeros((nx, ny, nz), order="F")
         self.rkh1 = np.zeros((nx, ny, nz), order="F")
@@ -138,7 +140,7 @@ class OracleRun(_RunBase):
         self.vdep = F(sc.vdep)
         L.or_kappa3(sc.modkv, sc.modkh, nx, ny, nz, P(self.dzv), P(self.zl), P(self.vx),
                     P(self.vy), P(F(sc.vptemp)), P(F(sc.zi)), P(F(sc.rl)), P(F(sc.ustar)),
-                    P(self.hm1), sc.dx, sc.dy, sc.dx, sc.dy, sc.rkv0, sc.rkh0, sc.csh, 0,
+                    P(self.hm1), sc.dx, sc.dy, sc.dx, sc.dy, sc.rkv0, sc.rkh0, sc.csh, lonlat,
                     P(self.rkver), P(self.rkhor))
         st = L.or_setvset(nx, ny, nz, nc, P(F(sc.tempe)), P(self.rho), P(F(sc.diam)),
                           P(F(sc.rhop)), P(self.psi), P(F(sc.sphe)), sc.modv, P(self.vset),
@@ -147,6 +149,9 @@ class OracleRun(_RunBase):
             raise RuntimeError(f"vsettl did not converge (cell {-st - 1})")
         L.or_meteo_scale(nx, ny, nz, P(self.hm1), P(self.vx), P(self.rho), P(self.rkhor),
                          P(self.rkh1))
+        if lonlat:
+            L.or_corvver(nx, ny, nz, P(self.hm1), P(F(sc.topg)), sc.dx, sc.dy, P(self.vx),
+                         P(self.vy), P(self.vz))
         dtc = ctypes.c_double()
         self.dt = L.or_finddt(nx, ny, nz, nc, P(self.dzv), sc.dx, sc.dy, P(self.vx),
                               P(self.vy), P(self.vz), P(self.rkh1), P(self.rkhor),
@@ -215,6 +220,7 @@ class RefRun(_RunBase):
             L.ref_get_coef.argtypes = [dp] * 6
             L.ref_set_coef.argtypes = [dp] * 9
             L.ref_setvdrydep.argtypes = [dp, dp, dp]
+            L.ref_corvver.argtypes = [dp] * 5
             L.ref_set_clock.argtypes = [d, i, d, d]
             cls._L = L
         return cls._L

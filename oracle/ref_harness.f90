@@ -127,6 +127,15 @@ contains
     vdep_ = vdep
   end subroutine ref_setvdrydep
 
+  ! meteo.f90:116-124: corvver on given Hm1, topg and (scaled) winds; vz_
+  ! in/out
+  subroutine ref_corvver(hm1_, topg_, vx_, vy_, vz_) bind(C, name='ref_corvver')
+    real(c_double) :: hm1_(nx,ny), topg_(nx,ny), vx_(nx,ny,nz), vy_(nx,ny,nz), vz_(nx,ny,nz)
+    Hm1 = hm1_; topg = topg_; vx = vx_; vy = vy_; vz = vz_
+    call corvver
+    vz_ = vz
+  end subroutine ref_corvver
+
   subroutine ref_kappa3() bind(C, name='ref_kappa3')
     call kappa3
   end subroutine ref_kappa3

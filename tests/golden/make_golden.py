@@ -186,6 +186,25 @@ def make_drydep():
     np.savez_compressed(os.path.join(OUT, "drydep_12x9x6.npz"), **rec)
 
 
+def make_corvver():
+    """corvver (meteo.f90:116-124) on seeded terrain, LON-LAT Hm1 per row
+    and mixed-sign scaled winds."""
+    L = RefRun.L()
+    rng = np.random.default_rng(88)
+    nx, ny, nz = 11, 8, 5
+    L.ref_alloc(nx, ny, nz, 1, 1)
+    L.ref_set_grid(5000.0, 4000.0, P(F(np.arange(nz) * 800.0)))
+    hm1 = F(np.repeat(np.sin(np.radians(np.linspace(30.0, 55.0, ny)))[None, :], nx, axis=0))
+    topg = F(rng.uniform(0.0, 2500.0, (nx, ny)))
+    vx = F(rng.normal(0.0, 12.0, (nx, ny, nz)))
+    vy = F(rng.normal(0.0, 12.0, (nx, ny, nz)))
+    vz0 = F(rng.normal(0.0, 0.3, (nx, ny, nz)))
+    vz = vz0.copy(order="F")
+    L.ref_corvver(P(hm1), P(topg), P(vx), P(vy), P(vz))
+    np.savez_compressed(os.path.join(OUT, "corvver_11x8x5.npz"), nx=nx, ny=ny, nz=nz,
+                        dx=5000.0, dy=4000.0, hm1=hm1, topg=topg, vx=vx, vy=vy, vz0=vz0, vz=vz)
+
+
 if __name__ == "__main__":
     if not os.path.exists(REF_PATH):
         sys.exit(f"{REF_PATH} missing: build it with `make -C oracle ref`")
@@ -195,9 +214,11 @@ if __name__ == "__main__":
             make_routines()
         elif which == "drydep":
             make_drydep()
+        elif which == "corvver":
+            make_corvver()
         else:
             make_run(which)
     else:
-        for name in ["routines", "drydep"] + list(RUNS):
+        for name in ["routines", "drydep", "corvver"] + list(RUNS):
             subprocess.run([sys.executable, os.path.abspath(__file__), name], check=True)
             print("wrote", name)

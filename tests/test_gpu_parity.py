@@ -213,6 +213,36 @@ def test_fp32_vs_oracle_lonlat_hm1():
     assert m["rho_m"] <= 1e-5, m
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_lonlat_terrain_corvver_vs_oracle(prec):
+    # a LON-LAT run over terrain: Hm1 per row, the CMAQ constant of LON-LAT
+    # grids and corvver's terrain-slope correction of vz (meteo.f90:116-124)
+    sc = _lonlat_band(S.config_small(nx=66, ny=29, nz=8, nc=3, seed=51, mean_u=6.0, mean_v=3.0,
+                                     modkh=S.KH_CMAQ))
+    x = np.arange(sc.nx)[:, None] / sc.nx
+    y = np.arange(sc.ny)[None, :] / sc.ny
+    topg = np.asfortranarray(1200.0 * (1.0 + np.sin(2 * np.pi * x) * np.cos(3 * np.pi * y)))
+    sc = dataclasses.replace(sc, lonlat=1, topg=topg)
+    o = OracleRun(sc)
+    o.meteo_update()
+    r = F.Run(sc, precision=F.FP64 if prec == "fp64" else F.FP32)
+    try:
+        r.meteo_update()
+        assert abs(r.dt - o.dt) <= 1e-11 * o.dt
+        for _ in range(8):
+            o.step()
+        r.step(8)
+        c, cdry = r.state()
+        if prec == "fp64":
+            assert rel_linf(interior(c), interior(o.c)) <= 1e-11
+            assert rel_linf(cdry[:, :, 1:], o.cdry[:, :, 1:]) <= 1e-11
+        else:
+            assert rel_l1(interior(c), interior(o.c)) <= 1e-4
+            assert rel_l1(cdry[:, :, 1:].sum(-1), o.cdry[:, :, 1:].sum(-1)) <= 1e-4
+    finally:
+        r.close()
+
+
 def test_fp64_vs_oracle_lonlat_hm1():
     sc = _lonlat_band(S.config_small(nx=21, ny=17, nz=7, nc=3, seed=34, mean_u=-5.0, mean_v=2.0))
     o = OracleRun(sc)

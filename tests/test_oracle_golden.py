@@ -138,3 +138,16 @@ def test_setvdrydep_bitexact_vs_reference():
         assert np.array_equal(vdep.view(np.uint64), np.asfortranarray(ref).view(np.uint64)), n
     assert (d["vdep_0"][:, :, 4] == 7.0).all()  # the 200 um class is untouched
 
+
+def test_corvver_bitexact_vs_reference():
+    # corvver.f90 on seeded terrain, LON-LAT Hm1 and mixed-sign winds
+    from oracle.oracle import lib as olib
+    from oracle.oracle import P as OP
+    d = np.load(os.path.join(GOLDEN, "corvver_11x8x5.npz"))
+    F_ = lambda a: np.asfortranarray(a, dtype=np.float64)  # noqa: E731
+    vz = F_(d["vz0"]).copy(order="F")
+    olib().or_corvver(int(d["nx"]), int(d["ny"]), int(d["nz"]), OP(F_(d["hm1"])), OP(F_(d["topg"])),
+                      float(d["dx"]), float(d["dy"]), OP(F_(d["vx"])), OP(F_(d["vy"])), OP(vz))
+    assert np.array_equal(vz.view(np.uint64), np.asfortranarray(d["vz"]).view(np.uint64))
+    assert not np.array_equal(d["vz"], d["vz0"])
+
