@@ -370,6 +370,30 @@ void accum_(const double *c, double *cdry, int *ic, int *nx, int *ny, int *nz, i
     copy_back(cdry + slot, dd, g.NXY);
 }
 
+void wetdep_(double *c, double *cwet, int *ic, int *nx, int *ny, int *nz, int *nz_loc,
+             int *iz_loc, double *dt, const double *diam, const double *zlayer, const double *dz,
+             const double *zi, const double *prate)
+{
+    DROPIN_GUARD("wetdep");
+    if (diam[*ic - 1] > 100.0 * 1e-6) return;  // wetdep.f90:33-35
+    const Grid g = make_grid(*nx, *ny, *nz, 1, 0.0, 0.0);
+    const size_t slot = (size_t)*ic * g.NXY;
+    const int one = 1;
+    Stage st;
+    double *dc = st.up(c, g.G);
+    double *dw = st.up(cwet + slot, g.NXY);
+    const int *dwet = st.up(&one, 1);
+    const double *dzl = st.up(zlayer, g.nz), *ddz = st.up(dz, g.nz + 1);
+    const double *dzi = st.up(zi, g.NXY), *dp = st.up(prate, g.NXY);
+    if (!dc || !dw || !dwet || !dzl || !ddz || !dzi || !dp) {
+        set_error("wetdep: device staging failed");
+        return;
+    }
+    if (p64::wetdep(0, g, 1, dc, dw, dwet, *dt, dzl, ddz, dzi, dp)) return;
+    copy_back(c, dc, g.G);
+    copy_back(cwet + slot, dw, g.NXY);
+}
+
 void cmass2d_(const double *c, const double *vx, const double *vy, const double *vz,
               const double *vset, int *nx, int *ny, int *nz, int *nc, int *ic_loc, int *nc_loc,
               int *iz_loc, int *nz_loc, double *dx, double *dy, const double *dz, double *dt,

@@ -143,6 +143,11 @@ int step_fp64(f3d_ctx *c)
         TRY(p64::diffy(s, g, nc, cc, g.dy, dt, rkhor, rho));
         TRY(p64::diffx(s, g, nc, cc, g.dx, dt, rkh1, rho));
     }
+    // sizloop.f90:65-67 wet deposition (all classes are particles: np = nc)
+    if (c->wet_on)
+        TRY(p64::wetdep(s, g, nc, cc, c->d_cwet.as<double>() + g.NXY, c->d_wet.as<int>(), dt,
+                        c->d_zlayer.as<double>(), dz, c->d_zi.as<double>(),
+                        c->d_prate.as<double>()));
     TRY(p64::accum(s, g, nc, cc, c->d_cdry.as<double>(), 1, dt, vset, vdep));
     // sizloop.f90:77-83
     TRY(p64::clip(s, cc, g.G * nc));
@@ -364,6 +369,7 @@ int f3d_set_classes(f3d_ctx *c, const double *diam, const double *rhop, const do
         set_error("f3d_set_classes: null argument");
         return F3D_EINVAL;
     }
+    c->diam_h.assign(diam, diam + c->g.nc);
     std::vector<met::ClassConst> cls(c->g.nc);
     for (int q = 0; q < c->g.nc; ++q) TRY(class_constants(modv, diam[q], rhop[q], sphe[q], &cls[q]));
     c->cls = cls;
@@ -832,6 +838,37 @@ int f3d_set_topography(f3d_ctx *c, const double *topg)
     TRY(upload(c->d_topg, topg, b, c->stream));
     F3D_HIP(hipStreamSynchronize(c->stream));
     c->topg_set = true;
+    return F3D_OK;
+}
+
+int f3d_set_precipitation(f3d_ctx *c, const double *prate)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    if (c->precision != F3D_FP64) {
+        set_error("f3d_set_precipitation: wet deposition runs on the FP64 path only");
+        return F3D_EINVAL;
+    }
+    if (!c->classes_set) {
+        set_error("f3d_set_precipitation: f3d_set_classes must come first");
+        return F3D_ESTATE;
+    }
+    if (!prate) {
+        set_error("f3d_set_precipitation: null field");
+        return F3D_EINVAL;
+    }
+    const Grid &g = c->g;
+    if (!c->d_prate.p && (c->d_prate.alloc(sizeof(double) * g.NXY, &c->bytes) ||
+                          c->d_wet.alloc(sizeof(int) * g.nc, &c->bytes))) {
+        set_error("f3d_set_precipitation: out of device memory");
+        return F3D_ENOMEM;
+    }
+    // wetdep.f90:33-35: particles above 100 um are not scavenged
+    std::vector<int> wet(g.nc);
+    for (int q = 0; q < g.nc; ++q) wet[q] = c->diam_h[q] > 100.0 * 1e-6 ? 0 : 1;
+    TRY(upload(c->d_prate, prate, sizeof(double) * g.NXY, c->stream));
+    TRY(upload(c->d_wet, wet.data(), sizeof(int) * g.nc, c->stream));
+    F3D_HIP(hipStreamSynchronize(c->stream));
+    c->wet_on = true;
     return F3D_OK;
 }
 

@@ -230,6 +230,11 @@ struct f3d_ctx {
     // corrects vz for the terrain slope (corvver, meteo.f90:116-124)
     f3d::DeviceBuffer d_topg;
     bool topg_set = false;
+    // precipitation rate (nx,ny, mm/h) and per-class wet flags (diameter
+    // up to 100 um): wet deposition in the FP64 step (sizloop.f90:65-67)
+    f3d::DeviceBuffer d_prate, d_wet;
+    bool wet_on = false;
+    std::vector<double> diam_h;
     f3d::DeviceBuffer d_rkhor, d_rkh1, d_rkver, d_vset, d_visc;
     f3d::DeviceBuffer d_cls, d_cd, d_status, d_dtcn;
     // device: state

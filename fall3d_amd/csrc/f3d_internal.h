@@ -100,6 +100,11 @@ int diffy(hipStream_t s, const Grid &g, int ncls, double *c, double dy, double d
 // cdry slot of class q is cdry + (ic0 + q)*NXY
 int accum(hipStream_t s, const Grid &g, int ncls, const double *c, double *cdry, int ic0,
           double dt, const double *vset, const double *vdep);
+// wetdep.f90:28-50 for ncls classes from the slab c and deposit slot cwet
+// of the first (grid.y = class); wet[q] = 0 skips class q
+int wetdep(hipStream_t s, const Grid &g, int ncls, double *c, double *cwet, const int *wet,
+           double dt, const double *zlayer, const double *dz, const double *zi,
+           const double *prate);
 int clip(hipStream_t s, double *c, long n);
 int setsrc(hipStream_t s, const Grid &g, double *c, int nsou, const int *isrc, const int *jsrc,
            const int *ksrc, const double *tmrat, const double *zlayer, const double *dz,

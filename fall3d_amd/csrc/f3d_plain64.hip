@@ -369,6 +369,34 @@ __global__ void k_diffz(Grid g, double *c, const ZCoef *zc, const double *ldz, d
 }
 
 // ---------------- accum.f90:17-25 ----------------
+// wetdep.f90:28-50 (Jung and Shao 2006): one thread per column and class
+// (grid.y); below the PBL height zi a fraction lambda = min(dt*a*P**b, 1)
+// of c goes into cwet, level by level in the reference's order.  c points
+// at the first class's slab, cwet at its deposit slot; classes whose wet
+// flag is 0 (coarser than 100 um) are left alone.
+__global__ void k_wetdep(Grid g, double *c, double *cwet, const int *wet, double dt,
+                         const double *zlayer, const double *dz, const double *zi,
+                         const double *prate)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (t >= g.NXY || !wet[q]) return;
+    const int i = (int)(t % g.nx) + 1, j = (int)(t / g.nx) + 1;
+    const double a = 8.4e-5, b = 0.79;
+    double lambda = dt * a * pow(prate[t], b);
+    lambda = lambda < 1.0 ? lambda : 1.0;
+    double *cc = c + (long)g.G * q;
+    double *w = cwet + (long)g.NXY * q + t;
+    double acc = *w;
+    for (int k = 1; k <= g.nz; ++k)
+        if (zlayer[k - 1] <= zi[t]) {
+            const long o = cidx(g.nx, g.ny, i, j, k);
+            acc = acc + dz[k] * lambda * cc[o];
+            cc[o] = cc[o] * (1.0 - lambda);
+        }
+    *w = acc;
+}
+
 __global__ void k_accum(Grid g, const double *c, double *cdry, int ic0, double dt,
                         const double *vset, const double *vdep)
 {
@@ -524,6 +552,16 @@ int accum(hipStream_t s, const Grid &g, int ncls, const double *c, double *cdry,
 {
     hipLaunchKernelGGL(k_accum, dim3(blocks_for(g.NXY, 256), ncls), dim3(256), 0, s, g, c, cdry,
                        ic0, dt, vset, vdep);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+int wetdep(hipStream_t s, const Grid &g, int ncls, double *c, double *cwet, const int *wet,
+           double dt, const double *zlayer, const double *dz, const double *zi,
+           const double *prate)
+{
+    hipLaunchKernelGGL(k_wetdep, dim3(blocks_for(g.NXY, 256), ncls), dim3(256), 0, s, g, c, cwet,
+                       wet, dt, zlayer, dz, zi, prate);
     F3D_CHECK_LAUNCH();
     return F3D_OK;
 }

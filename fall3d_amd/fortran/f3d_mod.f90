@@ -85,6 +85,13 @@ module fall3d_amd
        real(c_double), intent(in) :: topg(*)
      end function f3d_set_topography
 
+     ! prate (Master, mm/h): wet deposition in later steps (FP64 contexts)
+     integer(c_int) function f3d_set_precipitation(ctx, prate) bind(C, name='f3d_set_precipitation')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: prate(*)
+     end function f3d_set_precipitation
+
      integer(c_int) function f3d_get_vdep(ctx, vdep) bind(C, name='f3d_get_vdep')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

@@ -24,12 +24,13 @@ _L = None
 
 # every symbol include/fall3d_amd.h declares
 DROPIN_SYMBOLS = ["setbcc_", "advctzc_", "advctx_", "advcty_", "diffz_", "diffx_", "diffy_",
-                  "accum_", "cmass2d_", "cmass3d_"]
+                  "accum_", "wetdep_", "cmass2d_", "cmass3d_"]
 CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                "f3d_ensemble_budget", "f3d_get_member_state", "f3d_destroy", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
                "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_set_landuse",
-               "f3d_get_vdep", "f3d_set_topography", "f3d_get_clock",
+               "f3d_get_vdep", "f3d_set_topography", "f3d_set_precipitation",
+               "f3d_get_clock",
                "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
                "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time", "f3d_device_bytes",
                "f3d_last_error", "f3d_hbm_copy", "f3d_batch_create", "f3d_batch_destroy",
@@ -83,6 +84,7 @@ def lib():
         L.f3d_set_landuse.argtypes = [vp, dp, dp]
         L.f3d_get_vdep.argtypes = [vp, dp]
         L.f3d_set_topography.argtypes = [vp, dp]
+        L.f3d_set_precipitation.argtypes = [vp, dp]
         L.f3d_get_clock.argtypes = [vp, dp, ip, dp, dp]
         L.f3d_set_clock.argtypes = [vp, d, i, d, d]
         L.f3d_get_cd_state.argtypes = [vp, dp]
@@ -114,7 +116,8 @@ def lib():
                      "f3d_ensemble_budget", "f3d_get_member_state", "f3d_set_classes", "f3d_set_models", "f3d_set_meteo",
                      "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                      "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_set_landuse",
-                     "f3d_get_vdep", "f3d_set_topography", "f3d_get_clock",
+                     "f3d_get_vdep", "f3d_set_topography", "f3d_set_precipitation",
+                     "f3d_get_clock",
                      "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
                      "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time"):
             getattr(L, name).restype = c
@@ -296,6 +299,12 @@ class Context:
         self._topg = _F(topg)
         _check(lib().f3d_set_topography(self.h, _P(self._topg)), "f3d_set_topography")
 
+    def set_precipitation(self, prate):
+        """Precipitation rate per column (mm/h): wet deposition in every later
+        step (FP64 contexts)."""
+        self._prate = _F(prate)
+        _check(lib().f3d_set_precipitation(self.h, _P(self._prate)), "f3d_set_precipitation")
+
     def vdep(self):
         out = np.zeros((self.nx, self.ny, self.nc), order="F")
         _check(lib().f3d_get_vdep(self.h, _P(out)), "f3d_get_vdep")
@@ -363,6 +372,8 @@ class Run:
         self.ctx.set_classes(sc.diam, sc.rhop, sc.sphe, sc.modv)
         self.ctx.set_models(sc.modkv, sc.modkh, sc.rkv0, sc.rkh0, sc.csh)
         self.ctx.set_source(sc.isrc, sc.jsrc, sc.ksrc, sc.tmrat, sc.flowrate)
+        if getattr(sc, "prate", None) is not None:
+            self.ctx.set_precipitation(sc.prate)
         self.ctx.cd_state = cd_state
         self.nsteps = 0
 

@@ -80,6 +80,8 @@ class Scenario:
     # LON-LAT coordinates: kappa3's CMAQ constant and the terrain-slope
     # correction of vz (corvver) follow them
     lonlat: int = 0
+    # precipitation rate (nx,ny) in mm/h: wet deposition on (wetdep.f90)
+    prate: Optional[np.ndarray] = None
 
     @property
     def dz(self) -> np.ndarray:

@@ -67,6 +67,12 @@ void diffx_(double *c, int *nx, int *ny, int *nz, int *nz_loc, int *iz_loc, doub
 void diffy_(double *c, int *nx, int *ny, int *nz, int *nz_loc, int *iz_loc, double *dy,
             double *dt, const double *rkhor, const double *rho, double *work);
 /* Sources/accum.f90:1-15; cdry is (nx,ny,0:*) and ic selects the slot */
+/* Sources/wetdep.f90:1 (called at sizloop.f90:65-67): wet deposition of
+ * class ic (1-based) into cwet(nx,ny,0:*); classes above 100 um return at
+ * once, as the reference does. */
+void wetdep_(double *c, double *cwet, int *ic, int *nx, int *ny, int *nz, int *nz_loc,
+             int *iz_loc, double *dt, const double *diam, const double *zlayer, const double *dz,
+             const double *zi, const double *prate);
 void accum_(const double *c, double *cdry, int *ic, int *nx, int *ny, int *nz, int *nz_loc,
             int *iz_loc, const double *vz, double *dt, const double *vset, const double *vdep);
 /* Sources/cmass2d.f90:1-30; outmass accumulates */
@@ -170,6 +176,11 @@ int f3d_set_landuse(f3d_ctx *ctx, const double *ldu, const double *znt);
  * projection of the scaled horizontal wind.  Without it the terrain is
  * taken as flat (no correction); UTM grids never correct. */
 int f3d_set_topography(f3d_ctx *ctx, const double *topg);
+/* Precipitation rate per column (nx,ny) in mm/h, Master prate: turns on
+ * the wet deposition of sizloop.f90:65-67 (wetdep.f90, Jung and Shao) in
+ * every later f3d_step, into cwet and the budgets' masswet.  FP64 contexts
+ * only: an FP32 context returns F3D_EINVAL.  Needs f3d_set_classes first. */
+int f3d_set_precipitation(f3d_ctx *ctx, const double *prate);
 /* vdep (nx,ny,nc) as the last meteo read left it (Master vdep). */
 int f3d_get_vdep(f3d_ctx *ctx, double *vdep);
 int f3d_get_clock(f3d_ctx *ctx, double *time, int *iiter, double *erumass, double *outmass);

@@ -856,12 +856,38 @@ void or_setsrc(double *c, int nx, int ny, int nz, int nc, int nsou, const int *i
     }
 }
 
+/* wetdep.f90:28-50 (Jung and Shao 2006; ic 1-based, classes up to 100 um):
+ * below the PBL height zi, a fraction lambda = min(dt*a*P**b, 1) of c goes
+ * to cwet(:,:,ic) each step, weighted by dz(iz) */
+void or_wetdep(double *c, double *cwet, int ic, int nx, int ny, int nz, double dt,
+               const double *diam, const double *zlayer, const double *dz, const double *zi,
+               const double *prate)
+{
+    const double diam_max = 100.0 * 1e-6;
+    if (diam[ic - 1] > diam_max) return;
+    const double a = 8.4e-5, b = 0.79;
+    for (int i = 1; i <= nx; ++i)
+        for (int j = 1; j <= ny; ++j) {
+            double lambda = dt * a * pow(prate[PI2(i, j)], b);
+            lambda = lambda < 1.0 ? lambda : 1.0;
+            for (int k = 1; k <= nz; ++k)
+                if (zlayer[k - 1] <= zi[PI2(i, j)]) {
+                    double *w = cwet + (long)nx * ny * ic + PI2(i, j);
+                    *w = *w + dz[k] * lambda * c[CI(i, j, k)];
+                    c[CI(i, j, k)] = c[CI(i, j, k)] * (1.0 - lambda);
+                }
+        }
+}
+
 /* ---- sizloop ---- */
 typedef struct {
     double *c, *cdry;
     int nx, ny, nz, ic0, ic1, iiter;
     double dt, dX1, dX2;
     const double *vx, *vy, *vz, *vset, *vdep, *rkver, *rkhor, *rkh1, *rho, *dz;
+    /* wet deposition (sizloop.f90:65-67): on when prate is given */
+    double *cwet;
+    const double *diam, *zlayer, *zi, *prate;
 } class_job;
 
 static void run_classes(const class_job *J)
@@ -890,6 +916,9 @@ static void run_classes(const class_job *J)
             or_diffy(cc, nx, ny, nz, J->dX2, J->dt, J->rkhor, J->rho);
             or_diffx(cc, nx, ny, nz, J->dX1, J->dt, J->rkh1, J->rho);
         }
+        if (J->prate)
+            or_wetdep(cc, J->cwet, ic + 1, nx, ny, nz, J->dt, J->diam, J->zlayer, J->dz, J->zi,
+                      J->prate);
         or_accum(cc, J->cdry, ic + 1, nx, ny, nz, J->dt, vs, vd);
         or_clip(cc, G); /* elementwise: per-class clip == whole-array clip */
     }
@@ -905,10 +934,12 @@ void or_sizloop(double *c, double *cdry, int nx, int ny, int nz, int nc, int iit
                 const double *vx, const double *vy, const double *vz, const double *vset,
                 const double *vdep, const double *rkver, const double *rkhor,
                 const double *rkh1, const double *rho, const double *dz, double dX1,
-                double dX2, double *outmass, int nthreads)
+                double dX2, double *outmass, int nthreads, double *cwet, const double *diam,
+                const double *zlayer, const double *zi, const double *prate)
 {
     class_job base = {c, cdry, nx, ny, nz, 0, nc, iiter, dt, dX1, dX2,
-                      vx, vy, vz, vset, vdep, rkver, rkhor, rkh1, rho, dz};
+                      vx, vy, vz, vset, vdep, rkver, rkhor, rkh1, rho, dz,
+                      cwet, diam, zlayer, zi, prate};
     if (nthreads <= 1 || nc <= 1) {
         run_classes(&base);
     } else {

@@ -81,13 +81,20 @@ void or_setsrc(double *c, int nx, int ny, int nz, int nc, int nsou, const int *i
                const int *jsrc, const int *ksrc, const double *tmrat, const double *dz,
                const double *zlayer, const double *Hm1, double dX1, double dX2, double dt);
 
+/* wetdep.f90:28-50 (ic 1-based; cwet(nx,ny,0:nc)) */
+void or_wetdep(double *c, double *cwet, int ic, int nx, int ny, int nz, double dt,
+               const double *diam, const double *zlayer, const double *dz, const double *zi,
+               const double *prate);
+
 /* sizloop.f90:31-83 for all classes; nthreads > 1 splits the class loop
- * over POSIX threads (classes are independent until the clip). */
+ * over POSIX threads (classes are independent until the clip).  prate
+ * non-null: wet deposition into cwet (sizloop.f90:65-67). */
 void or_sizloop(double *c, double *cdry, int nx, int ny, int nz, int nc, int iiter, double dt,
                 const double *vx, const double *vy, const double *vz, const double *vset,
                 const double *vdep, const double *rkver, const double *rkhor,
                 const double *rkh1, const double *rho, const double *dz, double dX1,
-                double dX2, double *outmass, int nthreads);
+                double dX2, double *outmass, int nthreads, double *cwet, const double *diam,
+                const double *zlayer, const double *zi, const double *prate);
 
 #ifdef __cplusplus
 }

@@ -56,7 +56,8 @@ def lib():
         L.or_corvver.argtypes = [i, i, i, dp, dp, d, d, dp, dp, dp]
         L.or_setsrc.argtypes = [dp, i, i, i, i, i, ip, ip, ip, dp, dp, dp, dp, d, d, d]
         L.or_sizloop.argtypes = [dp, dp, i, i, i, i, i, d, dp, dp, dp, dp, dp, dp, dp, dp, dp,
-                                 dp, d, d, dp, i]
+                                 dp, d, d, dp, i, dp, dp, dp, dp, dp]
+        L.or_wetdep.argtypes = [dp, dp, i, i, i, i, d, dp, dp, dp, dp, dp]
         L.or_cmass3d.argtypes = [dp, dp, dp, dp, i, i, i, i, d, d, dp, dp, dp, dp, dp]
         L.or_cmass2d.argtypes = [dp, dp, dp, dp, dp, i, i, i, i, d, d, dp, d, dp]
         L.or_getc0.argtypes = [dp, dp, i, i, i]
@@ -115,6 +116,10 @@ class OracleRun(_RunBase):
         self.c = np.zeros((nx + 2, ny + 2, nz + 2, nc), order="F")
         self.cdry = np.zeros((nx, ny, nc + 1), order="F")
         self.cwet = np.zeros((nx, ny, nc + 1), order="F")
+        # wet deposition inputs (sizloop.f90:65-67); prate None: off
+        self._diam = F(sc.diam)
+        self._zi = F(sc.zi)
+        self._prate = None if getattr(sc, "prate", None) is None else F(sc.prate)
         self.psi = np.zeros(nc)
         L.or_setpsi(P(self.psi), P(F(sc.sphe)), P(F(sc.diam)), sc.modv, nc)
         self.cd = ctypes.c_double(cd_state)
@@ -171,7 +176,9 @@ class OracleRun(_RunBase):
         L.or_sizloop(P(self.c), P(self.cdry), self.nx, self.ny, self.nz, self.nc, self.iiter,
                      self.dt, P(self.vx), P(self.vy), P(self.vz), P(self.vset), P(self.vdep),
                      P(self.rkver), P(self.rkhor), P(self.rkh1), P(self.rho), P(self.dzv),
-                     sc.dx, sc.dy, ctypes.byref(om), self.nthreads)
+                     sc.dx, sc.dy, ctypes.byref(om), self.nthreads, P(self.cwet),
+                     P(self._diam), P(self.zl), P(self._zi),
+                     None if self._prate is None else P(self._prate))
         self.outmass = om.value
 
     def budget(self):
@@ -220,6 +227,8 @@ class RefRun(_RunBase):
             L.ref_get_coef.argtypes = [dp] * 6
             L.ref_set_coef.argtypes = [dp] * 9
             L.ref_setvdrydep.argtypes = [dp, dp, dp]
+            L.ref_set_wetdep.argtypes = [dp]
+            L.ref_get_cwet.argtypes = [dp]
             L.ref_corvver.argtypes = [dp] * 5
             L.ref_set_clock.argtypes = [d, i, d, d]
             cls._L = L
@@ -243,6 +252,9 @@ class RefRun(_RunBase):
         self._ksrc = np.ascontiguousarray(sc.ksrc, dtype=np.int32)
         self._tm = F(sc.tmrat)
         L.ref_set_source(P(self._isrc), P(self._jsrc), P(self._ksrc), P(self._tm), sc.flowrate)
+        if getattr(sc, "prate", None) is not None:
+            self._prate = F(sc.prate)
+            L.ref_set_wetdep(P(self._prate))
 
     def meteo_update(self):
         L = self.L()
@@ -277,6 +289,11 @@ class RefRun(_RunBase):
         it = ctypes.c_int()
         L.ref_get_state(P(c), P(cdry), ctypes.byref(t), ctypes.byref(it))
         return c, cdry
+
+    def cwet_state(self):
+        cwet = np.zeros((self.nx, self.ny, self.nc + 1), order="F")
+        self.L().ref_get_cwet(P(cwet))
+        return cwet
 
     def budget(self):
         L = self.L()

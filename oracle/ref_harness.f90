@@ -136,6 +136,21 @@ contains
     vz_ = vz
   end subroutine ref_corvver
 
+  ! wet deposition on (sizloop.f90:65-67) with the precipitation rate
+  ! prate(nx,ny) in mm/h; zi is the module state set above
+  subroutine ref_set_wetdep(prate_) bind(C, name='ref_set_wetdep')
+    real(c_double) :: prate_(nx,ny)
+    if (allocated(prate)) deallocate(prate)
+    allocate(prate(nx,ny))
+    prate = prate_
+    wetdeposition = .true.
+  end subroutine ref_set_wetdep
+
+  subroutine ref_get_cwet(cwet_) bind(C, name='ref_get_cwet')
+    real(c_double) :: cwet_(nx,ny,0:nc)
+    cwet_ = cwet
+  end subroutine ref_get_cwet
+
   subroutine ref_kappa3() bind(C, name='ref_kappa3')
     call kappa3
   end subroutine ref_kappa3

@@ -14,6 +14,7 @@ state in module variables, including the SAVEd vsettl drag coefficient
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import os
 import subprocess
 import sys
@@ -66,6 +67,37 @@ def make_run(name):
     for k, v in b.items():
         rec["budget_" + k] = v
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **rec)
+
+
+def make_wet_run():
+    """A composed run with wet deposition on (sizloop.f90:65-67): rain over
+    part of the domain, PBL heights that cut through the column."""
+    sc = S.config_small(nx=10, ny=8, nz=7, nc=3, seed=19, mean_u=4.0, mean_v=2.0)
+    rng = np.random.default_rng(91)
+    prate = np.asfortranarray(np.where(rng.random((sc.nx, sc.ny)) < 0.7,
+                                       rng.uniform(0.5, 30.0, (sc.nx, sc.ny)), 0.0))
+    sc.zi[:] = rng.uniform(800.0, 5000.0, sc.zi.shape)
+    sc = dataclasses.replace(sc, prate=prate)
+    nsteps = 12
+    r = RefRun(sc)
+    r.meteo_update()
+    rec = dict(nx=sc.nx, ny=sc.ny, nz=sc.nz, nc=sc.nc, dx=sc.dx, dy=sc.dy, zlayer=sc.zlayer,
+               diam=sc.diam, rhop=sc.rhop, sphe=sc.sphe, vx_in=sc.vx, vy=sc.vy, vz=sc.vz,
+               tempe=sc.tempe, vptemp=sc.vptemp, rho_in=sc.rho, ustar=sc.ustar, zi=sc.zi,
+               rl=sc.rl, topg=sc.topg, vdep=sc.vdep, isrc=sc.isrc, jsrc=sc.jsrc, ksrc=sc.ksrc,
+               tmrat=sc.tmrat, flowrate=sc.flowrate, modkv=sc.modkv, modkh=sc.modkh,
+               modv=sc.modv, rkv0=sc.rkv0, rkh0=sc.rkh0, csh=sc.csh, prate=prate,
+               psi=r.psi, dt=r.dt, dtc=r.dtc, vx=r.vx, rho=r.rho, rkhor=r.rkhor, rkh1=r.rkh1,
+               rkver=r.rkver, vset=r.vset, nsteps=nsteps)
+    r.step()
+    rec["c_step1"], rec["cdry_step1"] = r.state()
+    for _ in range(nsteps - 1):
+        r.step()
+    rec["c_final"], rec["cdry_final"] = r.state()
+    rec["cwet_final"] = r.cwet_state()
+    for k, v in r.budget().items():
+        rec["budget_" + k] = v
+    np.savez_compressed(os.path.join(OUT, "run_10x8x7x3_wetdep.npz"), **rec)
 
 
 def make_routines():
@@ -216,9 +248,11 @@ if __name__ == "__main__":
             make_drydep()
         elif which == "corvver":
             make_corvver()
+        elif which == "wet":
+            make_wet_run()
         else:
             make_run(which)
     else:
-        for name in ["routines", "drydep", "corvver"] + list(RUNS):
+        for name in ["routines", "drydep", "corvver", "wet"] + list(RUNS):
             subprocess.run([sys.executable, os.path.abspath(__file__), name], check=True)
             print("wrote", name)

@@ -27,4 +27,5 @@ def scenario_from(g):
         zi=g["zi"], rl=g["rl"], topg=g["topg"], vdep=g["vdep"], isrc=g["isrc"],
         jsrc=g["jsrc"], ksrc=g["ksrc"], tmrat=g["tmrat"], flowrate=float(g["flowrate"]),
         modkv=int(g["modkv"]), modkh=int(g["modkh"]), modv=int(g["modv"]),
-        rkv0=float(g["rkv0"]), rkh0=float(g["rkh0"]), csh=float(g["csh"]))
+        rkv0=float(g["rkv0"]), rkh0=float(g["rkh0"]), csh=float(g["csh"]),
+        prate=g.get("prate"))

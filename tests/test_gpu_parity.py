@@ -120,7 +120,7 @@ def test_fp64_context_vs_golden(path):
         assert rel_linf(c, g["c_final"]) <= 1e-11
         assert rel_linf(cdry, g["cdry_final"]) <= 1e-11
         b = r.budget()
-        for k in ("massvol", "massdep", "outmass", "erumass"):
+        for k in ("massvol", "massdep", "masswet", "outmass", "erumass"):
             ref = float(g["budget_" + k])
             if np.isnan(ref):
                 assert np.isnan(b[k])
@@ -132,7 +132,52 @@ def test_fp64_context_vs_golden(path):
 
 # ---------------- FP32 fused context ----------------
 
-FINITE = [p for p in run_fixtures() if "similarity" not in p]
+# the SIMILARITY fixture holds the reference's NaN; wet deposition runs on
+# the FP64 path only (the FP32 context refuses it, tested below)
+FINITE = [p for p in run_fixtures() if "similarity" not in p and "wetdep" not in p]
+WET = [p for p in run_fixtures() if "wetdep" in p]
+
+
+@pytest.mark.parametrize("path", WET, ids=os.path.basename)
+def test_fp32_context_refuses_wet_deposition(path):
+    sc = scenario_from(load(path))
+    with pytest.raises(F.F3DError, match="FP64 path only"):
+        F.Run(sc, precision=F.FP32)
+
+
+def test_wetdep_dropin_vs_oracle():
+    # the wetdep_ drop-in (Sources/wetdep.f90 argument list) against the
+    # oracle's restatement: rain over part of the domain, PBL heights that
+    # cut the column, a class above 100 um left alone; device pow: 1e-13
+    from oracle.oracle import lib as olib
+    from oracle.oracle import P as OP
+    rng = np.random.default_rng(5)
+    nx, ny, nz = 13, 9, 6
+    Fa = lambda a: np.asfortranarray(a, dtype=np.float64)  # noqa: E731
+    c0 = Fa(rng.uniform(0.0, 1.0, (nx + 2, ny + 2, nz + 2)))
+    cwet0 = Fa(rng.uniform(0.0, 0.1, (nx, ny, 3)))
+    zl = Fa(np.arange(nz) * 700.0)
+    dz = Fa(np.full(nz + 1, 700.0))
+    zi = Fa(rng.uniform(300.0, 4000.0, (nx, ny)))
+    prate = Fa(np.where(rng.random((nx, ny)) < 0.6, rng.uniform(0.1, 40.0, (nx, ny)), 0.0))
+    diam = Fa([5e-6, 2e-4])
+    L = F.lib()
+    ip = lambda v: ctypes.byref(ctypes.c_int(v))  # noqa: E731
+    for ic in (1, 2):
+        c, cw = c0.copy(order="F"), cwet0.copy(order="F")
+        L.wetdep_(F._P(c), F._P(cw), ip(ic), ip(nx), ip(ny), ip(nz), ip(nz), ip(1),
+                  ctypes.byref(ctypes.c_double(90.0)), F._P(diam), F._P(zl), F._P(dz),
+                  F._P(zi), F._P(prate))
+        assert F.last_error() in ("", None) or "wetdep" not in F.last_error()
+        co, cwo = c0.copy(order="F"), cwet0.copy(order="F")
+        olib().or_wetdep(OP(co), OP(cwo), ic, nx, ny, nz, 90.0, OP(diam), OP(zl), OP(dz),
+                         OP(zi), OP(prate))
+        assert np.all(np.abs(c - co) <= 1e-13 * np.abs(co)), ic
+        assert np.all(np.abs(cw - cwo) <= 1e-13 * np.abs(cwo)), ic
+        if ic == 2:
+            assert np.array_equal(c, c0) and np.array_equal(cw, cwo)
+        else:
+            assert not np.array_equal(c, c0)
 
 
 @pytest.mark.parametrize("path", FINITE, ids=os.path.basename)

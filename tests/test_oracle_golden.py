@@ -38,6 +38,9 @@ def test_oracle_matches_reference_run(path):
         o.step()
     assert _eq(o.c, g["c_final"])
     assert _eq(o.cdry, g["cdry_final"])
+    if "cwet_final" in g:  # wet deposition on (sizloop.f90:65-67)
+        assert _eq(o.cwet[:, :, 1:], g["cwet_final"][:, :, 1:])
+        assert np.any(g["cwet_final"][:, :, 1:] > 0)
     b = o.budget()
     for k in ("massvol", "massdep", "massdry", "masswet", "outmass", "erumass"):
         assert _eq(b[k], g["budget_" + k]), k
