@@ -872,6 +872,46 @@ int f3d_set_precipitation(f3d_ctx *c, const double *prate)
     return F3D_OK;
 }
 
+int f3d_output_products(f3d_ctx *c, int member, double *load, double *thick, double *colmass)
+{
+    if (!check_ctx(c)) return F3D_EINVAL;
+    const Grid &g = c->g;
+    if (member < 0 || member >= g.nm) {
+        set_error("f3d_output_products: member %d out of range (ensemble of %d)", member, g.nm);
+        return F3D_EINVAL;
+    }
+    if (!c->classes_set) {
+        set_error("f3d_output_products: f3d_set_classes must come first");
+        return F3D_ESTATE;
+    }
+    const size_t nl = (size_t)g.NXY * (g.nc + 1);
+    if (!c->d_products.p &&
+        c->d_products.alloc(sizeof(double) * (nl + 2 * (size_t)g.NXY), &c->bytes)) {
+        set_error("f3d_output_products: out of device memory");
+        return F3D_ENOMEM;
+    }
+    hipStream_t s = c->stream;
+    double *dl = c->d_products.as<double>(), *dt = dl + nl, *dm = dt + g.NXY;
+    const long slots = (long)member * (g.nc + 1) * g.NXY;
+    const double *cdry = c->d_cdry.as<double>() + slots, *cwet = c->d_cwet.as<double>() + slots;
+    const long slab = (long)member * g.nc * g.G;
+    const met::ClassConst *cls = c->d_cls.as<met::ClassConst>();
+    if (c->precision == F3D_FP64)
+        TRY(red::products(s, g, c->d_c64.as<double>() + slab, cdry, cwet, c->d_hm1.as<double>(), cls,
+                          c->d_dz.as<double>(), load ? dl : nullptr, thick ? dt : nullptr,
+                          colmass ? dm : nullptr));
+    else
+        TRY(red::products(s, g, c->d_c32[c->cur].as<float>() + slab, cdry, cwet,
+                          c->d_hm1.as<double>(), cls, c->d_dz.as<double>(), load ? dl : nullptr,
+                          thick ? dt : nullptr, colmass ? dm : nullptr));
+    if (load) F3D_HIP(hipMemcpyAsync(load, dl, sizeof(double) * nl, hipMemcpyDeviceToHost, s));
+    if (thick) F3D_HIP(hipMemcpyAsync(thick, dt, sizeof(double) * g.NXY, hipMemcpyDeviceToHost, s));
+    if (colmass)
+        F3D_HIP(hipMemcpyAsync(colmass, dm, sizeof(double) * g.NXY, hipMemcpyDeviceToHost, s));
+    F3D_HIP(hipStreamSynchronize(s));
+    return F3D_OK;
+}
+
 int f3d_get_vdep(f3d_ctx *c, double *vdep)
 {
     if (!check_ctx(c)) return F3D_EINVAL;

@@ -233,6 +233,7 @@ struct f3d_ctx {
     // precipitation rate (nx,ny, mm/h) and per-class wet flags (diameter
     // up to 100 um): wet deposition in the FP64 step (sizloop.f90:65-67)
     f3d::DeviceBuffer d_prate, d_wet;
+    f3d::DeviceBuffer d_products;  // output products of one member (lazily allocated)
     bool wet_on = false;
     std::vector<double> diam_h;
     f3d::DeviceBuffer d_rkhor, d_rkh1, d_rkver, d_vset, d_visc;

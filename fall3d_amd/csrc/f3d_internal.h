@@ -196,4 +196,17 @@ int finddt(hipStream_t s, const Grid &g, const double *dz, double dX1, double dX
            unsigned long long *dtcn_bits);
 }  // namespace met
 
+namespace red {
+// Output products of printres_nc.f90:150-241 for one member (c, cdry, cwet
+// at that member's slabs and slots): load (nx,ny,0:nc) in kg/m2, deposit
+// thickness (nx,ny) in cm, z-cumulative concentration (nx,ny) in g/m2;
+// null outputs are skipped
+int products(hipStream_t s, const Grid &g, const double *c, const double *cdry,
+             const double *cwet, const double *hm1, const met::ClassConst *cls, const double *dz,
+             double *load, double *thick, double *colmass);
+int products(hipStream_t s, const Grid &g, const float *c, const double *cdry,
+             const double *cwet, const double *hm1, const met::ClassConst *cls, const double *dz,
+             double *load, double *thick, double *colmass);
+}  // namespace red
+
 }  // namespace f3d

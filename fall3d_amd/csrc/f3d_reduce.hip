@@ -177,6 +177,78 @@ int getc0(hipStream_t s, const Grid &g, double *cdry, double *cwet)
     return F3D_OK;
 }
 
+// Output products of printres_nc.f90:150-241 for one member, one thread
+// per column, in the reference's summation order: load (nx,ny,0:nc) =
+// cload/Hm1 with cload = cdry + cwet and slot 0 from the class sums
+// (endstep.f90:35, getc0); the deposit thickness 1e2*sum(cload/rhop)/Hm1
+// (cm); the z-cumulative particle concentration 1e3*trapezoid of
+// sum_ic(c)/Hm1 over levels 1..nz (g/m2).  Null outputs are skipped.
+template <typename T>
+__global__ void k_products(Grid g, const T *c, const double *cdry, const double *cwet,
+                           const double *hm1, const met::ClassConst *cls, const double *dz,
+                           double *load, double *thick, double *colmass)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.NXY) return;
+    const int nc = g.nc;
+    const int i = (int)(t % g.nx) + 1, j = (int)(t / g.nx) + 1;
+    const double h = hm1[t];
+    if (load) {
+        double d0 = 0.0, w0 = 0.0;
+        for (int q = 1; q <= nc; ++q) {
+            d0 = d0 + cdry[q * g.NXY + t];
+            w0 = w0 + cwet[q * g.NXY + t];
+        }
+        load[t] = (d0 + w0) / h;
+        for (int q = 1; q <= nc; ++q)
+            load[q * g.NXY + t] = (cdry[q * g.NXY + t] + cwet[q * g.NXY + t]) / h;
+    }
+    if (thick) {
+        double s = 0.0;
+        for (int q = 1; q <= nc; ++q)
+            s = s + (cdry[q * g.NXY + t] + cwet[q * g.NXY + t]) / cls[q - 1].rhop;
+        thick[t] = 1e2 * s / h;
+    }
+    if (colmass) {
+        auto level = [&](int k) {
+            double w = 0.0;
+            for (int q = 0; q < nc; ++q) w = w + (double)c[(long)q * g.G + cidx(g.nx, g.ny, i, j, k)];
+            return w / h;
+        };
+        double acc = 0.0, wk = level(1);
+        for (int k = 1; k <= g.nz - 1; ++k) {
+            const double wn = level(k + 1);
+            acc = acc + dz[k] * 0.5 * (wk + wn);
+            wk = wn;
+        }
+        colmass[t] = 1e3 * acc;
+    }
+}
+
+template <typename T>
+static int products_impl(hipStream_t s, const Grid &g, const T *c, const double *cdry,
+                         const double *cwet, const double *hm1, const met::ClassConst *cls,
+                         const double *dz, double *load, double *thick, double *colmass)
+{
+    hipLaunchKernelGGL((k_products<T>), dim3((unsigned)((g.NXY + 255) / 256)), dim3(256), 0, s, g, c,
+                       cdry, cwet, hm1, cls, dz, load, thick, colmass);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+int products(hipStream_t s, const Grid &g, const double *c, const double *cdry,
+             const double *cwet, const double *hm1, const met::ClassConst *cls, const double *dz,
+             double *load, double *thick, double *colmass)
+{
+    return products_impl<double>(s, g, c, cdry, cwet, hm1, cls, dz, load, thick, colmass);
+}
+int products(hipStream_t s, const Grid &g, const float *c, const double *cdry,
+             const double *cwet, const double *hm1, const met::ClassConst *cls, const double *dz,
+             double *load, double *thick, double *colmass)
+{
+    return products_impl<float>(s, g, c, cdry, cwet, hm1, cls, dz, load, thick, colmass);
+}
+
 template <typename T>
 static int cmass3d_impl(hipStream_t s, const Grid &g, const T *c, const double *cload0,
                         const double *cdry0, const double *cwet0, long dstride, const double *dz,

@@ -92,6 +92,15 @@ module fall3d_amd
        real(c_double), intent(in) :: prate(*)
      end function f3d_set_precipitation
 
+     ! printres_nc products: load(nx,ny,0:nc), thick(nx,ny), colmass(nx,ny)
+     integer(c_int) function f3d_output_products(ctx, member, load, thick, colmass) &
+          bind(C, name='f3d_output_products')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: member
+       real(c_double), intent(out) :: load(*), thick(*), colmass(*)
+     end function f3d_output_products
+
      integer(c_int) function f3d_get_vdep(ctx, vdep) bind(C, name='f3d_get_vdep')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

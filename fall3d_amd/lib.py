@@ -30,6 +30,7 @@ CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_set_landuse",
                "f3d_get_vdep", "f3d_set_topography", "f3d_set_precipitation",
+               "f3d_output_products",
                "f3d_get_clock",
                "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
                "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time", "f3d_device_bytes",
@@ -85,6 +86,7 @@ def lib():
         L.f3d_get_vdep.argtypes = [vp, dp]
         L.f3d_set_topography.argtypes = [vp, dp]
         L.f3d_set_precipitation.argtypes = [vp, dp]
+        L.f3d_output_products.argtypes = [vp, i, dp, dp, dp]
         L.f3d_get_clock.argtypes = [vp, dp, ip, dp, dp]
         L.f3d_set_clock.argtypes = [vp, d, i, d, d]
         L.f3d_get_cd_state.argtypes = [vp, dp]
@@ -117,6 +119,7 @@ def lib():
                      "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                      "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_set_landuse",
                      "f3d_get_vdep", "f3d_set_topography", "f3d_set_precipitation",
+                     "f3d_output_products",
                      "f3d_get_clock",
                      "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
                      "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time"):
@@ -304,6 +307,16 @@ class Context:
         step (FP64 contexts)."""
         self._prate = _F(prate)
         _check(lib().f3d_set_precipitation(self.h, _P(self._prate)), "f3d_set_precipitation")
+
+    def products(self, member=0):
+        """Output products of a member: load (nx,ny,0:nc) kg/m2, deposit
+        thickness (nx,ny) cm, z-cumulative concentration (nx,ny) g/m2."""
+        load = np.zeros((self.nx, self.ny, self.nc + 1), order="F")
+        thick = np.zeros((self.nx, self.ny), order="F")
+        colmass = np.zeros((self.nx, self.ny), order="F")
+        _check(lib().f3d_output_products(self.h, int(member), _P(load), _P(thick), _P(colmass)),
+               "f3d_output_products")
+        return load, thick, colmass
 
     def vdep(self):
         out = np.zeros((self.nx, self.ny, self.nc), order="F")

@@ -181,6 +181,14 @@ int f3d_set_topography(f3d_ctx *ctx, const double *topg);
  * every later f3d_step, into cwet and the budgets' masswet.  FP64 contexts
  * only: an FP32 context returns F3D_EINVAL.  Needs f3d_set_classes first. */
 int f3d_set_precipitation(f3d_ctx *ctx, const double *prate);
+/* Output products of one member from the device state, as printres_nc.f90
+ * (:150-241) forms them before writing: load (nx,ny,0:nc) = cload/Hm1 in
+ * kg/m2 with cload = cdry + cwet (endstep.f90:35) and slot 0 the class sum;
+ * thick (nx,ny) = 1e2*sum(cload/rhop)/Hm1, the deposit thickness in cm;
+ * colmass (nx,ny) = 1e3*trapezoid over the levels of sum_ic(c)/Hm1, the
+ * z-cumulative concentration in g/m2.  Any output may be NULL.  The deposit
+ * slots as the steps left them (slot 0 is formed here, as getc0 would). */
+int f3d_output_products(f3d_ctx *ctx, int member, double *load, double *thick, double *colmass);
 /* vdep (nx,ny,nc) as the last meteo read left it (Master vdep). */
 int f3d_get_vdep(f3d_ctx *ctx, double *vdep);
 int f3d_get_clock(f3d_ctx *ctx, double *time, int *iiter, double *erumass, double *outmass);

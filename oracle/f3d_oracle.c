@@ -879,6 +879,50 @@ void or_wetdep(double *c, double *cwet, int ic, int nx, int ny, int nz, double d
         }
 }
 
+/* Output products of printres_nc.f90:150-241 (before the netCDF writes):
+ * load(nx,ny,0:nc) = cload/Hm1 with cload = cdry + cwet (endstep.f90:35)
+ * and slot 0 from the class sums (getc0, endstep.f90:80-106); the deposit
+ * thickness 1d2*sum_ic(cload/rhop)/Hm1 (cm, :194-199); the z-cumulative
+ * concentration 1d3*sum_{iz<nz} dz(iz)*0.5*(w(iz)+w(iz+1)), w =
+ * sum_ic(c)/Hm1 (g/m2, :218-230).  Null outputs are skipped. */
+void or_output_products(int nx, int ny, int nz, int nc, const double *c, const double *cdry,
+                        const double *cwet, const double *Hm1, const double *rhop,
+                        const double *dz, double *load, double *thick, double *colmass)
+{
+    const long nxy = (long)nx * ny, G = (long)(nx + 2) * (ny + 2) * (nz + 2);
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            const long t = PI2(i, j);
+            const double h = Hm1[t];
+            if (load) {
+                double d0 = 0.0, w0 = 0.0;
+                for (int q = 1; q <= nc; ++q) {
+                    d0 = d0 + cdry[q * nxy + t];
+                    w0 = w0 + cwet[q * nxy + t];
+                }
+                load[t] = (d0 + w0) / h;
+                for (int q = 1; q <= nc; ++q) load[q * nxy + t] = (cdry[q * nxy + t] + cwet[q * nxy + t]) / h;
+            }
+            if (thick) {
+                double s = 0.0;
+                for (int q = 1; q <= nc; ++q) s = s + (cdry[q * nxy + t] + cwet[q * nxy + t]) / rhop[q - 1];
+                thick[t] = 1e2 * s / h;
+            }
+            if (colmass) {
+                double acc = 0.0;
+                for (int k = 1; k <= nz - 1; ++k) {
+                    double w = 0.0, wn = 0.0;
+                    for (int q = 0; q < nc; ++q) {
+                        w = w + c[q * G + CI(i, j, k)];
+                        wn = wn + c[q * G + CI(i, j, k + 1)];
+                    }
+                    acc = acc + dz[k] * 0.5 * (w / h + wn / h);
+                }
+                colmass[t] = 1e3 * acc;
+            }
+        }
+}
+
 /* ---- sizloop ---- */
 typedef struct {
     double *c, *cdry;

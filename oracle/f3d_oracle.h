@@ -81,6 +81,12 @@ void or_setsrc(double *c, int nx, int ny, int nz, int nc, int nsou, const int *i
                const int *jsrc, const int *ksrc, const double *tmrat, const double *dz,
                const double *zlayer, const double *Hm1, double dX1, double dX2, double dt);
 
+/* printres_nc.f90:150-241 output products: load (nx,ny,0:nc) kg/m2,
+ * thick (nx,ny) cm, colmass (nx,ny) g/m2; c (0:nx+1,0:ny+1,0:nz+1,nc) */
+void or_output_products(int nx, int ny, int nz, int nc, const double *c, const double *cdry,
+                        const double *cwet, const double *Hm1, const double *rhop,
+                        const double *dz, double *load, double *thick, double *colmass);
+
 /* wetdep.f90:28-50 (ic 1-based; cwet(nx,ny,0:nc)) */
 void or_wetdep(double *c, double *cwet, int ic, int nx, int ny, int nz, double dt,
                const double *diam, const double *zlayer, const double *dz, const double *zi,

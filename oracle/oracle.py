@@ -58,6 +58,7 @@ def lib():
         L.or_sizloop.argtypes = [dp, dp, i, i, i, i, i, d, dp, dp, dp, dp, dp, dp, dp, dp, dp,
                                  dp, d, d, dp, i, dp, dp, dp, dp, dp]
         L.or_wetdep.argtypes = [dp, dp, i, i, i, i, d, dp, dp, dp, dp, dp]
+        L.or_output_products.argtypes = [i, i, i, i, dp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.or_cmass3d.argtypes = [dp, dp, dp, dp, i, i, i, i, d, d, dp, dp, dp, dp, dp]
         L.or_cmass2d.argtypes = [dp, dp, dp, dp, dp, i, i, i, i, d, d, dp, d, dp]
         L.or_getc0.argtypes = [dp, dp, i, i, i]
@@ -180,6 +181,18 @@ class OracleRun(_RunBase):
                      P(self._diam), P(self.zl), P(self._zi),
                      None if self._prate is None else P(self._prate))
         self.outmass = om.value
+
+    def products(self):
+        """Output products (printres_nc.f90:150-241): load (nx,ny,0:nc),
+        deposit thickness (nx,ny), z-cumulative concentration (nx,ny)."""
+        nx, ny, nc = self.nx, self.ny, self.nc
+        load = np.zeros((nx, ny, nc + 1), order="F")
+        thick = np.zeros((nx, ny), order="F")
+        colmass = np.zeros((nx, ny), order="F")
+        lib().or_output_products(nx, ny, self.nz, nc, P(self.c), P(self.cdry), P(self.cwet),
+                                 P(self.hm1), P(F(self.sc.rhop)), P(self.dzv), P(load),
+                                 P(thick), P(colmass))
+        return load, thick, colmass
 
     def budget(self):
         """getc0 + cload + cmass3d (endstep.f90:35,80-106; writim.f90:18)."""

@@ -145,6 +145,35 @@ def test_fp32_context_refuses_wet_deposition(path):
         F.Run(sc, precision=F.FP32)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_output_products_vs_oracle(prec):
+    # load, deposit thickness and z-cumulative concentration
+    # (printres_nc.f90:150-241) from the device state against the oracle's
+    # own run; FP64 with wet deposition on, so cwet enters the load
+    path = WET[0] if prec == "fp64" else FINITE[0]
+    sc = scenario_from(load(path))
+    o = OracleRun(sc)
+    o.meteo_update()
+    for _ in range(6):
+        o.step()
+    r = F.Run(sc, precision=F.FP64 if prec == "fp64" else F.FP32)
+    try:
+        r.meteo_update()
+        r.step(6)
+        got = r.ctx.products()
+        ref = o.products()
+        for name, a, b in zip(("load", "thick", "colmass"), got, ref):
+            assert np.all(np.isfinite(a)), name
+            if prec == "fp64":
+                assert rel_linf(a, b) <= 1e-11, name
+            else:
+                assert rel_l1(a, b) <= 1e-4, name
+        if prec == "fp64":
+            assert np.any(o.cwet[:, :, 1:] > 0)
+    finally:
+        r.close()
+
+
 def test_wetdep_dropin_vs_oracle():
     # the wetdep_ drop-in (Sources/wetdep.f90 argument list) against the
     # oracle's restatement: rain over part of the domain, PBL heights that

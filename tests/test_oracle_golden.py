@@ -154,3 +154,40 @@ def test_corvver_bitexact_vs_reference():
     assert np.array_equal(vz.view(np.uint64), np.asfortranarray(d["vz"]).view(np.uint64))
     assert not np.array_equal(d["vz"], d["vz0"])
 
+
+def test_output_products_closed_forms():
+    # printres_nc.f90:150-241 has no reference vectors here (it is the
+    # netCDF writer); the oracle's products are pinned by closed forms: a
+    # column linear in z makes the trapezoid exact, the loads are plain
+    # sums.  Ghost cells of c and deposit slot 0 hold NaN: they must not be
+    # read (slot 0 is formed from the classes).
+    from oracle.oracle import lib as olib
+    from oracle.oracle import P as OP
+    rng = np.random.default_rng(3)
+    nx, ny, nz, nc = 5, 4, 6, 3
+    dzl = rng.uniform(200.0, 900.0, nz - 1)
+    z = np.concatenate([[0.0], np.cumsum(dzl)])          # level heights z(1..nz)
+    dz = np.concatenate([[dzl[0]], dzl, [dzl[-1]]])       # dz(0:nz), reagrd.f90:216-220
+    hm1 = rng.uniform(0.5, 1.0, (nx, ny))
+    alpha, beta = rng.uniform(-1e-6, 1e-6, nc), rng.uniform(1e-3, 2e-3, nc)
+    c = np.full((nx + 2, ny + 2, nz + 2, nc), np.nan)
+    for q in range(nc):
+        c[1:-1, 1:-1, 1:-1, q] = (alpha[q] * z + beta[q])[None, None, :]
+    cdry = rng.uniform(0.0, 5.0, (nx, ny, nc + 1))
+    cwet = rng.uniform(0.0, 1.0, (nx, ny, nc + 1))
+    cdry[:, :, 0] = cwet[:, :, 0] = np.nan
+    rhop = rng.uniform(1000.0, 2500.0, nc)
+    F_ = lambda a: np.asfortranarray(a, dtype=np.float64)  # noqa: E731
+    load = np.zeros((nx, ny, nc + 1), order="F")
+    thick = np.zeros((nx, ny), order="F")
+    colmass = np.zeros((nx, ny), order="F")
+    olib().or_output_products(nx, ny, nz, nc, OP(F_(c)), OP(F_(cdry)), OP(F_(cwet)), OP(F_(hm1)),
+                              OP(F_(rhop)), OP(F_(dz)), OP(load), OP(thick), OP(colmass))
+    A, B = alpha.sum() / hm1, beta.sum() / hm1
+    col = 1e3 * (A * (z[-1] ** 2 - z[0] ** 2) / 2 + B * (z[-1] - z[0]))
+    np.testing.assert_allclose(colmass, col, rtol=1e-12)
+    cl = cdry[:, :, 1:] + cwet[:, :, 1:]
+    np.testing.assert_allclose(load[:, :, 1:], cl / hm1[:, :, None], rtol=1e-15)
+    np.testing.assert_allclose(load[:, :, 0], cl.sum(-1) / hm1, rtol=1e-14)
+    np.testing.assert_allclose(thick, 1e2 * (cl / rhop[None, None, :]).sum(-1) / hm1, rtol=1e-14)
+
