@@ -102,7 +102,10 @@ void step_lists(f3d_batch *b, std::vector<f32::StepArgs> &even, std::vector<f32:
         b->group[k] = next_odd != next0;
         odd[k] = fp32_args(c, cur_odd, 1);
         even[k] = fp32_args(c, 1 - cur_odd, 0);
-        if (cdry_alt) odd[k].cdry = even[k].cdry = (*cdry_alt)[k];
+        if (cdry_alt) {  // tuning runs: deposits to scratch, no wet deposition
+            odd[k].cdry = even[k].cdry = (*cdry_alt)[k];
+            odd[k].wet = even[k].wet = 0;
+        }
         f[k] = fp32_fields(c);
     }
 }
@@ -246,6 +249,11 @@ int f3d_batch_create(f3d_batch **out, f3d_ctx *const *ctxs, int n)
     }
     *out = nullptr;
     for (int k = 0; k < n; ++k) {
+        if (ctxs[k] && ctxs[k]->wet_on) {
+            set_error("f3d_batch_create: context %d has wet deposition, which runs through "
+                      "f3d_step only", k);
+            return F3D_EINVAL;
+        }
         if (!ctxs[k] || ctxs[k]->precision != F3D_FP32) {
             set_error("f3d_batch_create: context %d is not an FP32 context", k);
             return F3D_EINVAL;
@@ -408,6 +416,7 @@ int f3d_autotune(f3d_ctx *c, int trials, int *tx, int *ty, int *slabs, double *b
     // next step overwrites
     f32::StepArgs a = fp32_args(c, c->cur, c->iiter + 1);
     a.cdry = scr;
+    a.wet = 0;  // tuning leaves cwet alone
     const f32::Fields f = fp32_fields(c);
     std::vector<f32::TileShape> cand(std::begin(kCandidates), std::end(kCandidates));
     cand.push_back({16, 16, 0, 0});

@@ -25,8 +25,44 @@ f32::StepArgs fp32_args(f3d_ctx *c, int cur_in, int iiter)
     f32::StepArgs a{c->g,      c->d_c32[cur_in].as<float>(), c->d_c32[1 - cur_in].as<float>(),
                     c->d_cdry.as<double>(), c->dt, iiter, c->d_zc.as<ZCoef>(), c->side,
                     c->ev_fork, c->ev_join};
+    if (c->wet_on) {
+        a.wet = 1;
+        a.lam = c->d_lam.as<double>();
+        a.kwet = c->d_kwet.as<int>();
+        a.wetcls = c->d_wet.as<int>();
+        a.cwet = c->d_cwet.as<double>();
+        a.dz64 = c->d_dz.as<double>();
+    }
     return a;
 }
+
+namespace f32 {
+// wetdep.f90:38-46 per column: lambda = min(dt*a*P**b, 1) and the top level
+// below the PBL height (the levels rise, so the wet ones are 1..k)
+__global__ void k_wet_columns(Grid g, double dt, const double *prate, const double *zi,
+                              const double *zlayer, double *lam, int *kwet)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.NXY) return;
+    const double a = 8.4e-5, b = 0.79;
+    double l = dt * a * pow(prate[t], b);
+    l = l < 1.0 ? l : 1.0;
+    int kw = 0;
+    for (int k = 1; k <= g.nz; ++k)
+        if (zlayer[k - 1] <= zi[t]) kw = k;
+    lam[t] = l;
+    kwet[t] = l > 0.0 ? kw : 0;
+}
+
+int wet_columns(hipStream_t s, const Grid &g, double dt, const double *prate, const double *zi,
+                const double *zlayer, double *lam, int *kwet)
+{
+    hipLaunchKernelGGL(k_wet_columns, dim3((unsigned)((g.NXY + 255) / 256)), dim3(256), 0, s, g, dt,
+                       prate, zi, zlayer, lam, kwet);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+}  // namespace f32
 
 f32::SrcArgs fp32_src_args(f3d_ctx *c, const f32::StepArgs &a)
 {
@@ -193,6 +229,10 @@ int rebuild_fields(f3d_ctx *c)
 {
     if (c->precision != F3D_FP32) return F3D_OK;
     ++c->gen;
+    if (c->wet_on)  // lambda depends on dt, the wet levels on zi
+        TRY(f32::wet_columns(c->stream, c->g, c->dt, c->d_prate.as<double>(), c->d_zi.as<double>(),
+                             c->d_zlayer.as<double>(), c->d_lam.as<double>(),
+                             c->d_kwet.as<int>()));
     const Grid &g = c->g;
     TRY(f32::build_fields(c->stream, g, c->d_vx.as<double>(), c->d_vy.as<double>(),
                           c->d_vz.as<double>(), c->d_rho.as<double>(), c->d_rkh1.as<double>(),
@@ -844,10 +884,6 @@ int f3d_set_topography(f3d_ctx *c, const double *topg)
 int f3d_set_precipitation(f3d_ctx *c, const double *prate)
 {
     if (!check_ctx(c)) return F3D_EINVAL;
-    if (c->precision != F3D_FP64) {
-        set_error("f3d_set_precipitation: wet deposition runs on the FP64 path only");
-        return F3D_EINVAL;
-    }
     if (!c->classes_set) {
         set_error("f3d_set_precipitation: f3d_set_classes must come first");
         return F3D_ESTATE;
@@ -858,7 +894,9 @@ int f3d_set_precipitation(f3d_ctx *c, const double *prate)
     }
     const Grid &g = c->g;
     if (!c->d_prate.p && (c->d_prate.alloc(sizeof(double) * g.NXY, &c->bytes) ||
-                          c->d_wet.alloc(sizeof(int) * g.nc, &c->bytes))) {
+                          c->d_wet.alloc(sizeof(int) * g.nc, &c->bytes) ||
+                          c->d_lam.alloc(sizeof(double) * g.NXY, &c->bytes) ||
+                          c->d_kwet.alloc(sizeof(int) * g.NXY, &c->bytes))) {
         set_error("f3d_set_precipitation: out of device memory");
         return F3D_ENOMEM;
     }
@@ -869,6 +907,9 @@ int f3d_set_precipitation(f3d_ctx *c, const double *prate)
     TRY(upload(c->d_wet, wet.data(), sizeof(int) * g.nc, c->stream));
     F3D_HIP(hipStreamSynchronize(c->stream));
     c->wet_on = true;
+    // the FP32 step's per-column form; rebuilt with the fields on every meteo
+    // read and dt change
+    if (c->meteo_ready) TRY(rebuild_fields(c));
     return F3D_OK;
 }
 

@@ -102,7 +102,22 @@ struct StepArgs {
     // kernel run beside the interior tiles
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // wet deposition (sizloop.f90:65-67), when wet: per column the
+    // scavenged fraction lambda (FP64) and the top level below the PBL
+    // height (0: dry column), per class 1 for particles up to 100 um;
+    // cwet (nx,ny,0:nc) per member like cdry; dz64 = dz(0:nz)
+    int wet = 0;
+    const double *lam = nullptr;
+    const int *kwet = nullptr, *wetcls = nullptr;
+    double *cwet = nullptr;
+    const double *dz64 = nullptr;
 };
+
+// wet deposition columns for the fused step: lambda = min(dt*a*P**b, 1)
+// (wetdep.f90:38-40) and the top level with zlayer(k) <= zi (0 when
+// lambda is 0)
+int wet_columns(hipStream_t s, const Grid &g, double dt, const double *prate, const double *zi,
+                const double *zlayer, double *lam, int *kwet);
 
 // Tile geometry of the fused kernel (chosen by the autotuner).  tx = 58
 // selects the flux kernel (64-lane region rows, ty interior rows).
@@ -233,6 +248,7 @@ struct f3d_ctx {
     // precipitation rate (nx,ny, mm/h) and per-class wet flags (diameter
     // up to 100 um): wet deposition in the FP64 step (sizloop.f90:65-67)
     f3d::DeviceBuffer d_prate, d_wet;
+    f3d::DeviceBuffer d_lam, d_kwet;  // FP32 step: per-column lambda and top wet level
     f3d::DeviceBuffer d_products;  // output products of one member (lazily allocated)
     bool wet_on = false;
     std::vector<double> diam_h;

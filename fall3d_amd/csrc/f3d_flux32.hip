@@ -67,6 +67,11 @@ struct FluxParams {
     unsigned oA, oD, oW, oBF, oTF, oH;
     unsigned ftx, ftl;  // tile columns and records per level of the tile-major layout
     const double *vsum1;
+    // wet deposition (sizloop.f90:65-67), read by the WET tile code only
+    const double *lam;
+    const int *kwet, *wetcls;
+    double *cwet;
+    const double *dz64;
     const ZLev *zl;
     double dt64;
     float dt, dtdx2, dtdy2;
@@ -203,7 +208,7 @@ enum : unsigned {
 // steps, the two SC planes
 constexpr int flux_planes(int mc, bool odd) { return 3 * mc + 2 + (odd ? 2 : 0); }
 
-template <int RY, int NW, int MC, bool ODD, bool EX, bool EY>
+template <int RY, int NW, int MC, bool ODD, bool EX, bool EY, bool WET>
 struct FluxTile {
     static constexpr int NS = RY / NW;
     static constexpr bool EDGE = EX || EY;
@@ -260,6 +265,7 @@ struct FluxTile {
     float3 Dn[NS], D0[NS];
     float h2[NS];
     float gv[NS];
+    int kwl[NS];  // WET: the column's top wet level (0: dry column)
     // per class: advctzc window c(s), c(s+1), w(s), U(s), W2(s), delta(s) (after
     // level nz, c0 holds the top ghost c(nz+1)), prefetched c(s+2), w(s+1);
     // diffz window b(k), q(k) = b/rho and the face flux it leaves for the
@@ -362,6 +368,7 @@ struct FluxTile {
             Dn[sl] = bld3(rf, mt12[sl], p.oD);
             D0[sl] = make_float3(0.f, 0.f, 0.f);
             h2[sl] = bld(rf, m4[sl], p.oH);
+            if (WET) kwl[sl] = p.kwet[m4[sl] / 4u];
             gv[sl] = EDGE ? bld(rf, mt8[sl] + (has(sl, F_GX) ? 0u : 4u), p.oA) : 0.f;
 #pragma unroll
             for (int c = 0; c < MC; ++c) {
@@ -561,6 +568,16 @@ struct FluxTile {
                     if (ODD)
                         out = diff_x<EX>(out, elr[sl], Dx, epk[sl], p.dtdx2, inner(sl),
                                          has(sl, F_XLO), has(sl, F_XHI));
+                    // sizloop.f90:65-67, wetdep.f90:28-50 (WET launches only):
+                    // below the PBL height a fraction lambda of c goes to cwet,
+                    // level by level; the output lane owns its column's slot
+                    if (WET && k <= kwl[sl] && p.wetcls[icv[c]] && stored(sl, c)) {
+                        const size_t t2 = m4[sl] / 4u;
+                        const double lam = p.lam[t2];
+                        double *cw = p.cwet + ((size_t)memv[c] * (p.g.nc + 1) + icv[c] + 1) * NXY + t2;
+                        *cw = *cw + p.dz64[k] * lam * (double)out;
+                        out = (float)((double)out * (1.0 - lam));
+                    }
                     // sizloop.f90:77-78; interior tiles store in flush()
                     if (PIPE)
                         sto[c][sl] = fmaxf(0.f, out);
@@ -737,11 +754,11 @@ struct FluxTile {
     }
 };
 
-template <int RY, int NW, int MC, bool ODD, bool EX, bool EY>
+template <int RY, int NW, int MC, bool ODD, bool EX, bool EY, bool WET = false>
 __device__ __forceinline__ void flux_tile(const FluxParams &p, const int *qv, const int gi0,
                                           const int gj0, float *sm, const ZLev *zt)
 {
-    FluxTile<RY, NW, MC, ODD, EX, EY> t(p, qv, gi0, gj0, sm, zt);
+    FluxTile<RY, NW, MC, ODD, EX, EY, WET> t(p, qv, gi0, gj0, sm, zt);
     t.run();
 }
 
@@ -766,7 +783,7 @@ __device__ __forceinline__ int edge_row(const FluxParams &p, int r)
 // One workgroup per (tile of kind KIND, group of MC consecutive
 // member*class slabs).  Consecutive tiles run down y, so the 6-row halos of
 // neighbours meet in L2.
-template <int RY, int NW, int MC, bool ODD, int KIND>
+template <int RY, int NW, int MC, bool ODD, int KIND, bool WET>
 __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
 {
     const int b = blockIdx.x;
@@ -811,17 +828,17 @@ __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
     for (int k = threadIdx.x; k <= p.g.nz + 1; k += NW * 64) zt[k] = p.zl[k];
     __syncthreads();
     if (KIND == K_INTERIOR)
-        flux_tile<RY, NW, MC, ODD, false, false>(p, qv, gi0, gj0, sm, zt);
+        flux_tile<RY, NW, MC, ODD, false, false, WET>(p, qv, gi0, gj0, sm, zt);
     else if (kind == 1)
-        flux_tile<RY, NW, MC, ODD, true, false>(p, qv, gi0, gj0, sm, zt);
+        flux_tile<RY, NW, MC, ODD, true, false, WET>(p, qv, gi0, gj0, sm, zt);
     else if (kind == 2)
-        flux_tile<RY, NW, MC, ODD, false, true>(p, qv, gi0, gj0, sm, zt);
+        flux_tile<RY, NW, MC, ODD, false, true, WET>(p, qv, gi0, gj0, sm, zt);
     else
-        flux_tile<RY, NW, MC, ODD, true, true>(p, qv, gi0, gj0, sm, zt);
+        flux_tile<RY, NW, MC, ODD, true, true, WET>(p, qv, gi0, gj0, sm, zt);
 }
 
-template <int RY, int NW, int MC, int KIND>
-void launch_part(hipStream_t s, FluxParams p, int ntiles, bool odd)
+template <int RY, int NW, int MC, int KIND, bool WET>
+void launch_part_t(hipStream_t s, FluxParams p, int ntiles, bool odd)
 {
     if (ntiles <= 0) return;
     const int ngrp = (p.g.nc * p.g.nm + MC - 1) / MC;
@@ -829,9 +846,19 @@ void launch_part(hipStream_t s, FluxParams p, int ntiles, bool odd)
     p.per_xcd = (p.nlogical + kXCD - 1) / kXCD;
     const unsigned nb = (unsigned)(p.per_xcd * kXCD);
     if (odd)
-        hipLaunchKernelGGL((k_flux<RY, NW, MC, true, KIND>), dim3(nb), dim3(NW * 64), 0, s, p);
+        hipLaunchKernelGGL((k_flux<RY, NW, MC, true, KIND, WET>), dim3(nb), dim3(NW * 64), 0, s, p);
     else
-        hipLaunchKernelGGL((k_flux<RY, NW, MC, false, KIND>), dim3(nb), dim3(NW * 64), 0, s, p);
+        hipLaunchKernelGGL((k_flux<RY, NW, MC, false, KIND, WET>), dim3(nb), dim3(NW * 64), 0, s, p);
+}
+
+// the wet-deposition instantiation only when the step has precipitation
+template <int RY, int NW, int MC, int KIND>
+void launch_part(hipStream_t s, const FluxParams &p, int ntiles, bool odd, bool wet)
+{
+    if (wet)
+        launch_part_t<RY, NW, MC, KIND, true>(s, p, ntiles, odd);
+    else
+        launch_part_t<RY, NW, MC, KIND, false>(s, p, ntiles, odd);
 }
 
 template <int RY, int NW, int MC>
@@ -869,9 +896,9 @@ int launch_flux(hipStream_t s, const StepArgs &a, const FluxParams &p0, bool odd
         F3D_HIP(hipEventRecord(a.fork, s));
         F3D_HIP(hipStreamWaitEvent(se, a.fork, 0));
     }
-    launch_part<RY, NW, MCE, K_EDGE>(se, p, nedge, odd);
+    launch_part<RY, NW, MCE, K_EDGE>(se, p, nedge, odd, a.wet != 0);
     F3D_CHECK_LAUNCH();
-    launch_part<RY, NW, MC, K_INTERIOR>(s, p, nint, odd);
+    launch_part<RY, NW, MC, K_INTERIOR>(s, p, nint, odd, a.wet != 0);
     F3D_CHECK_LAUNCH();
     if (fork) {
         F3D_HIP(hipEventRecord(a.join, se));
@@ -903,6 +930,11 @@ static int make_params(const StepArgs &a, const Fields &f, FluxParams &p)
     p.cin = a.cin;
     p.cout = a.cout;
     p.cdry = a.cdry;
+    p.lam = a.lam;
+    p.kwet = a.kwet;
+    p.wetcls = a.wetcls;
+    p.cwet = a.cwet;
+    p.dz64 = a.dz64;
     const char *fb = reinterpret_cast<const char *>(f.base);
     auto off = [&](const void *q) {
         return (unsigned)(reinterpret_cast<const char *>(q) - fb);

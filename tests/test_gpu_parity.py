@@ -132,25 +132,31 @@ def test_fp64_context_vs_golden(path):
 
 # ---------------- FP32 fused context ----------------
 
-# the SIMILARITY fixture holds the reference's NaN; wet deposition runs on
-# the FP64 path only (the FP32 context refuses it, tested below)
-FINITE = [p for p in run_fixtures() if "similarity" not in p and "wetdep" not in p]
+# the SIMILARITY fixture holds the reference's NaN
+FINITE = [p for p in run_fixtures() if "similarity" not in p]
 WET = [p for p in run_fixtures() if "wetdep" in p]
 
 
 @pytest.mark.parametrize("path", WET, ids=os.path.basename)
-def test_fp32_context_refuses_wet_deposition(path):
+def test_batch_refuses_wet_context(path):
+    # the persistent batched kernel carries no wet deposition: a context
+    # with precipitation steps through f3d_step only
     sc = scenario_from(load(path))
-    with pytest.raises(F.F3DError, match="FP64 path only"):
-        F.Run(sc, precision=F.FP32)
+    r = F.Run(sc, precision=F.FP32)
+    try:
+        r.meteo_update()
+        with pytest.raises(F.F3DError, match="wet deposition"):
+            F.Batch([r.ctx])
+    finally:
+        r.close()
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_output_products_vs_oracle(prec):
     # load, deposit thickness and z-cumulative concentration
     # (printres_nc.f90:150-241) from the device state against the oracle's
-    # own run; FP64 with wet deposition on, so cwet enters the load
-    path = WET[0] if prec == "fp64" else FINITE[0]
+    # own run, with wet deposition on, so cwet enters the load
+    path = WET[0]
     sc = scenario_from(load(path))
     o = OracleRun(sc)
     o.meteo_update()
@@ -168,8 +174,7 @@ def test_output_products_vs_oracle(prec):
                 assert rel_linf(a, b) <= 1e-11, name
             else:
                 assert rel_l1(a, b) <= 1e-4, name
-        if prec == "fp64":
-            assert np.any(o.cwet[:, :, 1:] > 0)
+        assert np.any(o.cwet[:, :, 1:] > 0)
     finally:
         r.close()
 
@@ -224,7 +229,7 @@ def test_fp32_context_vs_golden(path):
         assert e_c <= 1e-4, e_c
         assert e_load <= 1e-4, e_load
         b = r.budget()
-        for k in ("massvol", "massdep", "outmass"):
+        for k in ("massvol", "massdep", "masswet", "outmass"):
             ref = float(g["budget_" + k])
             assert abs(b[k] - ref) <= 1e-4 * abs(ref) + 1e-12 * float(g["budget_erumass"]), k
     finally:
