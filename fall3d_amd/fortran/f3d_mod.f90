@@ -85,7 +85,7 @@ module fall3d_amd
        real(c_double), intent(in) :: topg(*)
      end function f3d_set_topography
 
-     ! prate (Master, mm/h): wet deposition in later steps (FP64 contexts)
+     ! prate (Master, mm/h): wet deposition in later f3d_step calls (not in batches)
      integer(c_int) function f3d_set_precipitation(ctx, prate) bind(C, name='f3d_set_precipitation')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

@@ -304,7 +304,7 @@ class Context:
 
     def set_precipitation(self, prate):
         """Precipitation rate per column (mm/h): wet deposition in every later
-        step (FP64 contexts)."""
+        step (through f3d_step; batches refuse such contexts)."""
         self._prate = _F(prate)
         _check(lib().f3d_set_precipitation(self.h, _P(self._prate)), "f3d_set_precipitation")
 

@@ -178,8 +178,9 @@ int f3d_set_landuse(f3d_ctx *ctx, const double *ldu, const double *znt);
 int f3d_set_topography(f3d_ctx *ctx, const double *topg);
 /* Precipitation rate per column (nx,ny) in mm/h, Master prate: turns on
  * the wet deposition of sizloop.f90:65-67 (wetdep.f90, Jung and Shao) in
- * every later f3d_step, into cwet and the budgets' masswet.  FP64 contexts
- * only: an FP32 context returns F3D_EINVAL.  Needs f3d_set_classes first. */
+ * every later f3d_step, into cwet and the budgets' masswet, on the FP64 and
+ * the FP32 path.  A context with precipitation steps through f3d_step only:
+ * f3d_batch_create refuses it.  Needs f3d_set_classes first. */
 int f3d_set_precipitation(f3d_ctx *ctx, const double *prate);
 /* Output products of one member from the device state, as printres_nc.f90
  * (:150-241) forms them before writing: load (nx,ny,0:nc) = cload/Hm1 in
