@@ -11,7 +11,9 @@ roofline  = the fused step kernels' algorithmic HBM bytes per step over their
             HIP-event average time, against 8 TB/s
 fp64_plain= the routine-by-routine FP64 drop-in on the same workload
 cpu_baseline = the oracle's FP64 C restatement (class-parallel threads) on
-            this host, on a bounded number of steps of the same workload
+            this host, on a bounded number of steps of the same workload;
+            beside it (reference_1core) the reference's own Sources/*.f90
+            routines, compiled by amdflang -O2 into oracle/_ref, on one core
 ensemble  = 64 members x 200x200x30 x 16 classes on one meteorology, one
             fused launch per step
 uneven_batch = scenarios of 80..320 squared x 16..48 levels x 4..24 classes,
@@ -59,6 +61,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--ref-seconds", type=float, default=10.0,
+                    help="step time of the one-core reference leg (0: skip it)")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--tile", default="58x18")
     ap.add_argument("--waves", type=int, default=0,
@@ -161,7 +165,7 @@ def main():
         o.vy = np.asfortranarray(sc.vy)
         o.vz = np.asfortranarray(sc.vz)
         o.vdep = np.asfortranarray(sc.vdep)
-        o.dt = r_dt(sc, coef)
+        o.dt = o_dt = r_dt(sc, coef)
         nsteps, tcpu = 0, 0.0
         while tcpu < args.cpu_seconds and nsteps < 50:
             t0 = time.perf_counter()
@@ -174,6 +178,10 @@ def main():
                    sample=f"{nsteps} steps of the {sc.name} workload (FP64 C restatement, "
                           f"{nthreads} class-parallel threads, coefficients from the device "
                           f"meteo stage)")
+        del o
+        if args.ref_seconds > 0:
+            cpu["reference_1core"] = ref_cpu_baseline(sc, upd, args.ref_seconds, coef,
+                                                      o_dt)
 
     out = {
         "metric": "cell·class updates/s + % HBM peak; ensemble p50/p99 forecast latency",
@@ -209,6 +217,37 @@ def main():
     if not args.skip_serve:
         out["serving"] = bench_serving(args, fp64)
     print(json.dumps(out), flush=True)
+
+
+def ref_cpu_baseline(sc, upd, seconds, coef, dt):
+    """The reference's own routines (Sources/*.f90 compiled by amdflang -O2
+    into oracle/_ref by oracle/Makefile), serial on one core as one process of
+    the reference runs, on the coefficients and dt of the device meteo stage
+    (as the port leg): steps until `seconds` of step time (at most 5).  None
+    where oracle/_ref was not built."""
+    from oracle.oracle import RefRun, P
+    from oracle.oracle import F as FF
+
+    if not RefRun.available():
+        return None
+    r = RefRun(sc)
+    arrs = [coef["vx"], FF(sc.vy), FF(sc.vz), coef["rho"], coef["rkhor"], coef["rkh1"],
+            coef["rkver"], coef["vset"], FF(sc.vdep)]
+    r.L().ref_set_coef(*[P(a) for a in arrs])
+    r.L().ref_set_dt(dt)
+    r.dt = dt
+    nsteps, tcpu = 0, 0.0
+    while tcpu < seconds and nsteps < 5:
+        t0 = time.perf_counter()
+        r.step()
+        tcpu += time.perf_counter() - t0
+        nsteps += 1
+        log(f"cpu reference step {nsteps}: {tcpu:.1f}s")
+    return dict(value=upd * nsteps / tcpu, unit="cell*class updates/s", cores=1,
+                kind="reference",
+                sample=f"{nsteps} steps of the {sc.name} workload (the reference's "
+                       f"Sources/*.f90 routines, amdflang -O2, serial; coefficients from "
+                       f"the device meteo stage)")
 
 
 def _speedup(v, fp64):
