@@ -9,7 +9,10 @@ scenario, meteo stage included.
 * FP32 fused context: relative L1 of concentration and ground load
   <= 1e-4, rho_M <= 1e-5 (FP32 state drift);
 * FP32 from two fresh contexts: bit-identical state and budgets;
-* after the clip (sizloop.f90:77-78) no concentration is negative or NaN.
+* after the clip (sizloop.f90:77-78) no concentration is negative or NaN;
+* the ensemble benchmark's grid (200x200x30, 16 classes) and the uneven
+  batch benchmark's extreme shapes (320x320x48x24, 80x80x16x4): every
+  member or scenario bit-identical to the same scenario stepped alone.
 
 The budget does not close against the injected mass, B != erumass, in the
 reference either: the scheme is not discretely conservative and the
@@ -111,3 +114,76 @@ def test_fullsize_fp32_bit_identical_on_repeat(runs):
     assert np.array_equal(again["c"], r["c"])
     assert np.array_equal(again["cdry"], r["cdry"])
     assert again["b"] == r["b"]
+
+
+def _state_budget(r):
+    c, cdry = r.state()
+    return c, cdry[:, :, 1:], r.budget(), r.ctx.clock()
+
+
+def _assert_same(a, b, what):
+    assert np.array_equal(a[0], b[0]), what
+    assert np.array_equal(a[1], b[1]), what
+    for k in ("massvol", "massdep", "outmass", "erumass"):
+        assert a[2][k] == b[2][k], (what, k)
+    assert a[3]["iiter"] == b[3]["iiter"] and a[3]["time"] == b[3]["time"], what
+
+
+def test_ensemble_at_benchmark_size_members_equal_alone():
+    # the ensemble benchmark's grid and members (200x200x30, 16 classes,
+    # member seeds 5000+k), 8 of them: members at large slab offsets are
+    # bit-identical to the same scenario run alone
+    base = S.config_c2(nx=200, ny=200, nz=30, nc=16, seed=0xF3D0003)
+    mem = [S.config_ensemble_member(base, 5000 + k) for k in range(8)]
+    ens = F.Ensemble(base, mem)
+    try:
+        ens.meteo_update()
+        ens.step(10)
+        mix = {m: ens.state(m) for m in (0, 7)}
+        bmix = ens.budgets()
+    finally:
+        ens.close()
+    for m in (0, 7):
+        r = F.Run(mem[m], precision=F.FP32)
+        try:
+            r.meteo_update()
+            r.step(10)
+            got = _state_budget(r)
+        finally:
+            r.close()
+        assert got[0].max() > 0
+        _assert_same(got, (mix[m][0], mix[m][1][:, :, 1:], bmix[m], got[3]), m)
+
+
+def test_uneven_batch_at_benchmark_extremes_equal_alone():
+    # the uneven benchmark's largest and smallest scenario shapes
+    # (320x320x48 with 24 classes, 80x80x16 with 4) and one drawn between,
+    # each with its own meteorology and dt, in one persistent launch per
+    # step: each bit-identical to stepping alone with the same tile shape
+    scs = [S.config_c2(nx=320, ny=320, nz=48, nc=24, seed=0xF3D0601),
+           S.config_c2(nx=80, ny=80, nz=16, nc=4, seed=0xF3D0602),
+           S.config_uneven(0xF3D0603)]
+    tile = (58, 18, 0, 2)
+    mix = [F.Run(sc, precision=F.FP32, tile=tile) for sc in scs]
+    try:
+        for r in mix:
+            r.meteo_update()
+        assert len({r.dt for r in mix}) == 3
+        b = F.Batch([r.ctx for r in mix], tile=(58, 18, 2))
+        b.step(6)
+        b.sync()
+        got = [_state_budget(r) for r in mix]
+        b.close()
+    finally:
+        for r in mix:
+            r.close()
+    for k, sc in enumerate(scs):
+        r = F.Run(sc, precision=F.FP32, tile=tile)
+        try:
+            r.meteo_update()
+            r.step(6)
+            alone = _state_budget(r)
+        finally:
+            r.close()
+        assert alone[0].max() > 0
+        _assert_same(alone, got[k], sc.name)
