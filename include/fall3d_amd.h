@@ -67,14 +67,14 @@ void diffx_(double *c, int *nx, int *ny, int *nz, int *nz_loc, int *iz_loc, doub
 void diffy_(double *c, int *nx, int *ny, int *nz, int *nz_loc, int *iz_loc, double *dy,
             double *dt, const double *rkhor, const double *rho, double *work);
 /* Sources/accum.f90:1-15; cdry is (nx,ny,0:*) and ic selects the slot */
+void accum_(const double *c, double *cdry, int *ic, int *nx, int *ny, int *nz, int *nz_loc,
+            int *iz_loc, const double *vz, double *dt, const double *vset, const double *vdep);
 /* Sources/wetdep.f90:1 (called at sizloop.f90:65-67): wet deposition of
  * class ic (1-based) into cwet(nx,ny,0:*); classes above 100 um return at
  * once, as the reference does. */
 void wetdep_(double *c, double *cwet, int *ic, int *nx, int *ny, int *nz, int *nz_loc,
              int *iz_loc, double *dt, const double *diam, const double *zlayer, const double *dz,
              const double *zi, const double *prate);
-void accum_(const double *c, double *cdry, int *ic, int *nx, int *ny, int *nz, int *nz_loc,
-            int *iz_loc, const double *vz, double *dt, const double *vset, const double *vdep);
 /* Sources/cmass2d.f90:1-30; outmass accumulates */
 void cmass2d_(const double *c, const double *vx, const double *vy, const double *vz,
               const double *vset, int *nx, int *ny, int *nz, int *nc, int *ic_loc, int *nc_loc,
