@@ -28,6 +28,10 @@ struct f3d_batch {
     std::vector<f3d_ctx *> ctx;
     hipStream_t stream = nullptr, side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // orders the batch stream after each context's stream and back: a
+    // context's own calls (f3d_step, f3d_budget, f3d_get_state, a meteo
+    // read) and the batch's steps act on one state in call order
+    hipEvent_t order = nullptr;
     f32::TileShape ts{58, 18, 0, 2};
     f32::FluxBatchPlan plan;
     bool planned = false;
@@ -69,14 +73,45 @@ void drop_graphs(f3d_batch *b)
         }
 }
 
-// every context has its meteorology
+// every context has its meteorology and no wet deposition (the batched
+// kernel has no wet instantiation: a context that set precipitation after
+// joining the batch is refused here rather than stepped without it)
 int check_members(const f3d_batch *b)
 {
-    for (const f3d_ctx *c : b->ctx)
+    for (size_t k = 0; k < b->ctx.size(); ++k) {
+        const f3d_ctx *c = b->ctx[k];
         if (!c->meteo_ready) {
             set_error("batch: a context has no meteo read yet");
             return F3D_ESTATE;
         }
+        if (c->wet_on) {
+            set_error("batch: context %zu has wet deposition, which runs through f3d_step only", k);
+            return F3D_EINVAL;
+        }
+    }
+    return F3D_OK;
+}
+
+// the batch stream waits for the work queued so far on every context's
+// stream (and side stream)
+int after_members(f3d_batch *b)
+{
+    for (f3d_ctx *c : b->ctx) {
+        F3D_HIP(hipEventRecord(b->order, c->stream));
+        F3D_HIP(hipStreamWaitEvent(b->stream, b->order, 0));
+        if (c->side) {
+            F3D_HIP(hipEventRecord(b->order, c->side));
+            F3D_HIP(hipStreamWaitEvent(b->stream, b->order, 0));
+        }
+    }
+    return F3D_OK;
+}
+
+// every context's stream waits for the batch's work queued so far
+int members_after(f3d_batch *b)
+{
+    F3D_HIP(hipEventRecord(b->order, b->stream));
+    for (f3d_ctx *c : b->ctx) F3D_HIP(hipStreamWaitEvent(c->stream, b->order, 0));
     return F3D_OK;
 }
 
@@ -269,7 +304,8 @@ int f3d_batch_create(f3d_batch **out, f3d_ctx *const *ctxs, int n)
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->order, hipEventDisableTiming) != hipSuccess) {
         set_error("f3d_batch_create: hipStreamCreate / hipEventCreate failed");
         f3d_batch_destroy(b);
         return F3D_EHIP;
@@ -293,6 +329,7 @@ void f3d_batch_destroy(f3d_batch *b)
     drop_graphs(b);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->join) (void)hipEventDestroy(b->join);
+    if (b->order) (void)hipEventDestroy(b->order);
     if (b->side) (void)hipStreamDestroy(b->side);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -332,6 +369,9 @@ int f3d_batch_use_graph(f3d_batch *b, int on)
 int f3d_batch_step(f3d_batch *b, int nsteps)
 {
     if (!check_batch(b)) return F3D_EINVAL;
+    if (nsteps < 1) return F3D_OK;
+    TRY(check_members(b));
+    TRY(after_members(b));
     for (int n = 0; n < nsteps; ++n) {
         TRY(check_members(b));
         if (!plan_current(b)) TRY(replan(b));
@@ -350,6 +390,7 @@ int f3d_batch_step(f3d_batch *b, int nsteps)
         }
         for (f3d_ctx *c : b->ctx) c->cur = 1 - c->cur;
     }
+    TRY(members_after(b));
     return F3D_OK;
 }
 
@@ -364,6 +405,7 @@ int f3d_batch_autotune(f3d_batch *b, int trials, int *tx, int *ty, int *slabs, d
 {
     if (!check_batch(b)) return F3D_EINVAL;
     TRY(check_members(b));
+    TRY(after_members(b));
     std::vector<double *> scr(b->ctx.size(), nullptr);
     int rc = F3D_OK;
     for (size_t k = 0; k < b->ctx.size() && !rc; ++k) {

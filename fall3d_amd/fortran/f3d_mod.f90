@@ -27,11 +27,20 @@ module fall3d_amd
   end type f3d_grid
 
   public :: f3d_create, f3d_destroy, f3d_set_classes, f3d_set_models, f3d_set_meteo, &
-            f3d_get_dt, f3d_set_source, f3d_step, f3d_budget, f3d_get_state, f3d_sync, &
-            f3d_get_clock, f3d_error_message
-  ! batching and tuning (no counterpart in the reference)
+            f3d_get_dt, f3d_set_dt, f3d_set_source, f3d_step, f3d_budget, f3d_get_state, &
+            f3d_set_state, f3d_get_coef, f3d_sync, f3d_get_clock, f3d_set_clock, &
+            f3d_get_cd_state, f3d_set_cd_state, f3d_error_message
+  ! meteo-stage extensions and output products
+  public :: f3d_set_landuse, f3d_set_topography, f3d_set_precipitation, &
+            f3d_output_products, f3d_get_vdep
+  ! ensembles and resident meteorology (no counterpart in the reference)
+  public :: f3d_create_ensemble, f3d_set_member_source, f3d_ensemble_budget, &
+            f3d_get_member_state, f3d_set_meteo_window
+  ! batching, tuning, timing and memory (no counterpart in the reference)
   public :: f3d_batch_create, f3d_batch_destroy, f3d_batch_step, f3d_batch_sync, &
-            f3d_batch_autotune, f3d_autotune
+            f3d_batch_autotune, f3d_autotune, f3d_batch_set_tile, f3d_batch_get_tile, &
+            f3d_batch_use_graph, f3d_set_tile, f3d_set_tile_config, f3d_timing, &
+            f3d_kernel_time, f3d_device_bytes, f3d_mem_info, f3d_hbm_copy
 
   interface
      integer(c_int) function f3d_create(ctx, grid, precision) bind(C, name='f3d_create')
@@ -195,6 +204,153 @@ module fall3d_amd
        integer(c_int), intent(out) :: tx, ty, slabs
        real(c_double), intent(out) :: best_ms
      end function f3d_autotune
+
+     integer(c_int) function f3d_create_ensemble(ctx, grid, nmembers) &
+          bind(C, name='f3d_create_ensemble')
+       import :: c_int, c_ptr, f3d_grid
+       type(c_ptr), intent(out) :: ctx
+       type(f3d_grid), intent(in) :: grid
+       integer(c_int), value :: nmembers
+     end function f3d_create_ensemble
+
+     ! meteorology from the window (i0+1.., j0+1.., 1..nz) of a region
+     ! context's fields in HBM
+     integer(c_int) function f3d_set_meteo_window(ctx, region, i0, j0) &
+          bind(C, name='f3d_set_meteo_window')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, region
+       integer(c_int), value :: i0, j0
+     end function f3d_set_meteo_window
+
+     integer(c_int) function f3d_set_dt(ctx, dt) bind(C, name='f3d_set_dt')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: dt
+     end function f3d_set_dt
+
+     integer(c_int) function f3d_set_member_source(ctx, member, nsou, isrc, jsrc, ksrc, &
+          tmrat, flowrate) bind(C, name='f3d_set_member_source')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: member, nsou
+       integer(c_int), intent(in) :: isrc(*), jsrc(*), ksrc(*)
+       real(c_double), intent(in) :: tmrat(*)
+       real(c_double), value :: flowrate
+     end function f3d_set_member_source
+
+     integer(c_int) function f3d_ensemble_budget(ctx, out) bind(C, name='f3d_ensemble_budget')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: out(*)
+     end function f3d_ensemble_budget
+
+     integer(c_int) function f3d_get_member_state(ctx, member, c, cdry) &
+          bind(C, name='f3d_get_member_state')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: member
+       real(c_double), intent(out) :: c(*), cdry(*)
+     end function f3d_get_member_state
+
+     ! restart: c (0:nx+1,0:ny+1,0:nz+1,nc) and cdry (nx,ny,0:nc)
+     integer(c_int) function f3d_set_state(ctx, c, cdry) bind(C, name='f3d_set_state')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: c(*), cdry(*)
+     end function f3d_set_state
+
+     integer(c_int) function f3d_get_coef(ctx, vx, rho, rkhor, rkh1, rkver, vset) &
+          bind(C, name='f3d_get_coef')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: vx(*), rho(*), rkhor(*), rkh1(*), rkver(*), vset(*)
+     end function f3d_get_coef
+
+     integer(c_int) function f3d_set_clock(ctx, time, iiter, erumass, outmass) &
+          bind(C, name='f3d_set_clock')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: time, erumass, outmass
+       integer(c_int), value :: iiter
+     end function f3d_set_clock
+
+     ! the SAVEd vsettl drag coefficient (setvset.f90:89)
+     integer(c_int) function f3d_get_cd_state(ctx, cd) bind(C, name='f3d_get_cd_state')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: cd
+     end function f3d_get_cd_state
+
+     integer(c_int) function f3d_set_cd_state(ctx, cd) bind(C, name='f3d_set_cd_state')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: cd
+     end function f3d_set_cd_state
+
+     integer(c_int) function f3d_set_tile(ctx, tx, ty) bind(C, name='f3d_set_tile')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: tx, ty
+     end function f3d_set_tile
+
+     integer(c_int) function f3d_set_tile_config(ctx, tx, ty, waves, slabs) &
+          bind(C, name='f3d_set_tile_config')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: tx, ty, waves, slabs
+     end function f3d_set_tile_config
+
+     integer(c_int) function f3d_timing(ctx, enable) bind(C, name='f3d_timing')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: enable
+     end function f3d_timing
+
+     integer(c_int) function f3d_kernel_time(ctx, reset, avg_ms, count) &
+          bind(C, name='f3d_kernel_time')
+       import :: c_int, c_ptr, c_double, c_long
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: reset
+       real(c_double), intent(out) :: avg_ms
+       integer(c_long), intent(out) :: count
+     end function f3d_kernel_time
+
+     integer(c_long_long) function f3d_device_bytes(ctx) bind(C, name='f3d_device_bytes')
+       import :: c_long_long, c_ptr
+       type(c_ptr), value :: ctx
+     end function f3d_device_bytes
+
+     integer(c_int) function f3d_mem_info(free_bytes, total_bytes) bind(C, name='f3d_mem_info')
+       import :: c_int, c_long_long
+       integer(c_long_long), intent(out) :: free_bytes, total_bytes
+     end function f3d_mem_info
+
+     integer(c_int) function f3d_hbm_copy(nbytes, reps, avg_ms) bind(C, name='f3d_hbm_copy')
+       import :: c_int, c_long_long, c_double
+       integer(c_long_long), value :: nbytes
+       integer(c_int), value :: reps
+       real(c_double), intent(out) :: avg_ms
+     end function f3d_hbm_copy
+
+     integer(c_int) function f3d_batch_set_tile(batch, tx, ty, slabs) &
+          bind(C, name='f3d_batch_set_tile')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+       integer(c_int), value :: tx, ty, slabs
+     end function f3d_batch_set_tile
+
+     integer(c_int) function f3d_batch_get_tile(batch, tx, ty, slabs) &
+          bind(C, name='f3d_batch_get_tile')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+       integer(c_int), intent(out) :: tx, ty, slabs
+     end function f3d_batch_get_tile
+
+     integer(c_int) function f3d_batch_use_graph(batch, on) bind(C, name='f3d_batch_use_graph')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+       integer(c_int), value :: on
+     end function f3d_batch_use_graph
 
      type(c_ptr) function f3d_last_error_c() bind(C, name='f3d_last_error')
        import :: c_ptr

@@ -25,7 +25,7 @@ from typing import Optional
 
 import numpy as np
 
-__all__ = ["Scenario", "standard_atm", "make_classes", "suzuki_source", "config_c1",
+__all__ = ["Scenario", "standard_atm", "make_classes", "suzuki_profile", "getshape", "suzuki_source", "config_c1",
            "config_c2", "config_small", "config_ensemble_member", "config_uneven"]
 
 KV_CONSTANT, KV_SIMILARITY, KV_SURFACE = 0, 1, 2
@@ -163,23 +163,38 @@ def make_classes(nc, dmin_m, dmax_m, rho_fine=2500.0, rho_coarse=1000.0, sphe=0.
     return d, rhop, np.full(nc, sphe), fc
 
 
-def suzuki_source(nx, ny, dx, dy, zlayer, x0, y0, H, mer, A, L, fc, ns=100, topg=None):
-    """Suzuki column (solvesuzuki.f90:25-45) of ns points above (x0,y0),
-    distributed to nodes by bilinear shape functions in (x,y) and linear
-    weights in z (getsrc.f90:23-106).  Returns 1-based (i,j,k) and tmrat."""
-    nz = len(zlayer)
+def suzuki_profile(H, mer, A, L, ns=100):
+    """Suzuki column (solvesuzuki.f90:25-45): ns points at z = is*H/ns,
+    is = 1..ns, with S(z) = ((1 - z/H) exp(A (z/H - 1)))^L normalised so
+    that sum(S) = mer.  Returns (z, S)."""
     dzs = H / ns
     z = dzs * np.arange(1, ns + 1)
     S = ((1.0 - z / H) * np.exp(A * (z / H - 1.0))) ** L
-    S = mer * S / S.sum()
+    return z, mer * S / S.sum()
+
+
+def getshape(s, t):
+    """Bilinear shape functions of the 4-node cell at (s,t) in (-1,1)^2,
+    nodes ordered (ix,iy), (ix+1,iy), (ix+1,iy+1), (ix,iy+1) (getsrc.f90:
+    getshape)."""
+    return [0.25 * (1 - s) * (1 - t), 0.25 * (1 + s) * (1 - t),
+            0.25 * (1 + s) * (1 + t), 0.25 * (1 - s) * (1 + t)]
+
+
+def suzuki_source(nx, ny, dx, dy, zlayer, x0, y0, H, mer, A, L, fc, ns=100, topg=None):
+    """Suzuki column (solvesuzuki.f90:25-45) of ns points above (x0,y0),
+    distributed to nodes by bilinear shape functions in (x,y) and linear
+    weights in z (getsrc.f90:23-106).  Returns 1-based (i,j,k) and tmrat.
+    A point above the top level is an error, as in getsrc (runend)."""
+    nz = len(zlayer)
+    z, S = suzuki_profile(H, mer, A, L, ns)
     ix = int(math.floor(x0 / dx)) + 1
     iy = int(math.floor(y0 / dy)) + 1
     ix = min(max(ix, 1), nx - 1)
     iy = min(max(iy, 1), ny - 1)
     s = 2.0 * ((x0 - (ix - 1) * dx) / dx) - 1.0
     t = 2.0 * ((y0 - (iy - 1) * dy) / dy) - 1.0
-    shape = [0.25 * (1 - s) * (1 - t), 0.25 * (1 + s) * (1 - t),
-             0.25 * (1 + s) * (1 + t), 0.25 * (1 - s) * (1 + t)]
+    shape = getshape(s, t)
     corners = [(ix, iy), (ix + 1, iy), (ix + 1, iy + 1), (ix, iy + 1)]
     src = {}
     for q, (ci, cj) in enumerate(corners):
@@ -188,11 +203,12 @@ def suzuki_source(nx, ny, dx, dy, zlayer, x0, y0, H, mer, A, L, fc, ns=100, topg
             zz = z[p] - zp
             if zz < zlayer[0]:
                 k, w = 1, 0.0
+            elif zz >= zlayer[nz - 1]:
+                raise ValueError(f"source point at {zz:.1f} m above the top level "
+                                 f"{zlayer[nz - 1]:.1f} m")
             else:
                 k = int(np.searchsorted(zlayer, zz, side="right"))
-                k = min(max(k, 1), nz - 1)
                 w = (zz - zlayer[k - 1]) / (zlayer[k] - zlayer[k - 1])
-                w = min(max(w, 0.0), 1.0)
             for kk, ww in ((k, 1.0 - w), (k + 1, w)):
                 if ww > 0.0 and shape[q] > 0.0:
                     src[(ci, cj, kk)] = src.get((ci, cj, kk), 0.0) + ww * shape[q] * S[p]

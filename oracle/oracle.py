@@ -19,6 +19,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libf3dref.so")
+# the reference's sizloop/setsrc/meteo-stage objects and the harness linked
+# against libfall3d_amd.so for setbcc_ ... cmass3d_ (oracle/Makefile dropin)
+DROPIN_PATH = os.path.join(HERE, "_ref", "libf3dref_dropin.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -215,15 +218,16 @@ class RefRun(_RunBase):
     for the life of the process (setvset.f90:89)."""
 
     _L = None
+    PATH = REF_PATH
 
     @classmethod
     def available(cls):
-        return os.path.exists(REF_PATH)
+        return os.path.exists(cls.PATH)
 
     @classmethod
     def L(cls):
         if cls._L is None:
-            L = ctypes.CDLL(REF_PATH)
+            L = ctypes.CDLL(cls.PATH)
             d, i, dp, ip = ctypes.c_double, ctypes.c_int, _dp, _ip
             L.ref_alloc.argtypes = [i, i, i, i, i]
             L.ref_set_grid.argtypes = [d, d, dp]
@@ -314,3 +318,15 @@ class RefRun(_RunBase):
         L.ref_budget(*[ctypes.byref(o) for o in out])
         keys = ["massvol", "massdep", "massdry", "masswet", "outmass", "erumass"]
         return {k: o.value for k, o in zip(keys, out)}
+
+
+class DropinRun(RefRun):
+    """The reference's own driver-side code (setsrc.f90, sizloop.f90, the
+    meteo-stage routines, through the same harness) with setbcc_, advctzc_,
+    advctx_, advcty_, diffz_, diffx_, diffy_, accum_, wetdep_, cmass2d_ and
+    cmass3d_ resolved from libfall3d_amd.so: the routine-level drop-in as a
+    reference build would link it.  Its own module state, separate from
+    RefRun's (ctypes loads each library with local symbol scope)."""
+
+    _L = None
+    PATH = DROPIN_PATH

@@ -181,3 +181,50 @@ def test_batch_mixed_parities_equal_alone():
             assert np.array_equal(alone.state()[0], got[k]), k
         finally:
             alone.close()
+
+
+def test_batch_refuses_a_context_that_turns_wet():
+    # precipitation set on a context after it joined the batch: the next
+    # batched step is refused (the batched kernel has no wet deposition)
+    scs = _mixture(2, seed=41)
+    runs = _runs(scs, None)
+    try:
+        b = F.Batch([r.ctx for r in runs])
+        b.step(1)
+        runs[1].ctx.set_precipitation(np.full((scs[1].nx, scs[1].ny), 2.0, order="F"))
+        with pytest.raises(F.F3DError, match="wet deposition"):
+            b.step(1)
+        b.close()
+    finally:
+        _close(runs)
+
+
+def test_context_calls_are_ordered_after_batched_steps():
+    # budgets, state reads and f3d_step issued right after f3d_batch_step,
+    # with no f3d_batch_sync, act on the state the batched steps left
+    scs = _mixture(3, seed=63)
+    a = _runs(scs, None)
+    try:
+        b = F.Batch([r.ctx for r in a])
+        b.step(3)
+        bud = [r.budget() for r in a]          # no batch sync
+        a[1].step(1)                            # f3d_step on a member, no sync
+        b.step(2)
+        st = [r.state()[0] for r in a]
+        bud2 = [r.budget() for r in a]
+        b.close()
+    finally:
+        _close(a)
+    for k, n in ((0, 5), (1, 6), (2, 5)):
+        alone = _runs([scs[k]], None)[0]
+        try:
+            alone.step(3)
+            b3 = alone.budget()
+            alone.step(n - 3)
+            assert np.array_equal(alone.state()[0], st[k]), k
+            b5 = alone.budget()
+        finally:
+            alone.close()
+        for key in ("massvol", "massdep", "outmass", "erumass"):
+            assert b3[key] == bud[k][key], (k, key)
+            assert b5[key] == bud2[k][key], (k, key)
