@@ -35,6 +35,7 @@ UNITS = {
     "f3d_fused32.hip": FAST,
     "f3d_flux32.hip": FAST,
     "f3d_batch.hip": EXACT,
+    "f3d_noalloc.hip": EXACT,
 }
 
 

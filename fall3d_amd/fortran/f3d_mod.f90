@@ -40,7 +40,8 @@ module fall3d_amd
   public :: f3d_batch_create, f3d_batch_destroy, f3d_batch_step, f3d_batch_sync, &
             f3d_batch_autotune, f3d_autotune, f3d_batch_set_tile, f3d_batch_get_tile, &
             f3d_batch_use_graph, f3d_set_tile, f3d_set_tile_config, f3d_timing, &
-            f3d_kernel_time, f3d_device_bytes, f3d_mem_info, f3d_hbm_copy
+            f3d_kernel_time, f3d_device_bytes, f3d_mem_info, f3d_hbm_copy, &
+            f3d_cmass3d_noalloc
 
   interface
      integer(c_int) function f3d_create(ctx, grid, precision) bind(C, name='f3d_create')
@@ -351,6 +352,16 @@ module fall3d_amd
        type(c_ptr), value :: batch
        integer(c_int), value :: on
      end function f3d_batch_use_graph
+
+     ! cmass3d over device arrays, allocating nothing (device pointers;
+     ! stream: a hipStream_t or c_null_ptr)
+     integer(c_int) function f3d_cmass3d_noalloc(c, cload, cdry, cwet, nx, ny, nz, nc, &
+          nmembers, dx, dy, dz, out, stream) bind(C, name='f3d_cmass3d_noalloc')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: c, cload, cdry, cwet, dz, out, stream
+       integer(c_int), value :: nx, ny, nz, nc, nmembers
+       real(c_double), value :: dx, dy
+     end function f3d_cmass3d_noalloc
 
      type(c_ptr) function f3d_last_error_c() bind(C, name='f3d_last_error')
        import :: c_ptr

@@ -37,7 +37,7 @@ CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                "f3d_last_error", "f3d_hbm_copy", "f3d_batch_create", "f3d_batch_destroy",
                "f3d_batch_set_tile", "f3d_batch_get_tile", "f3d_batch_use_graph",
                "f3d_batch_step", "f3d_batch_sync", "f3d_batch_autotune", "f3d_autotune",
-               "f3d_set_meteo_window", "f3d_mem_info"]
+               "f3d_set_meteo_window", "f3d_mem_info", "f3d_cmass3d_noalloc"]
 
 
 class F3DError(RuntimeError):
@@ -98,6 +98,8 @@ def lib():
         L.f3d_kernel_time.argtypes = [vp, i, dp, ctypes.POINTER(ctypes.c_long)]
         L.f3d_device_bytes.argtypes = [vp]
         L.f3d_device_bytes.restype = ctypes.c_longlong
+        L.f3d_cmass3d_noalloc.argtypes = [vp, vp, vp, vp, i, i, i, i, i, d, d, vp, vp, vp]
+        L.f3d_cmass3d_noalloc.restype = c
         L.f3d_hbm_copy.argtypes = [ctypes.c_longlong, i, dp]
         L.f3d_hbm_copy.restype = c
         L.f3d_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), i]

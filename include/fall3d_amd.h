@@ -85,6 +85,23 @@ void cmass3d_(const double *c, const double *cload, const double *cdry, const do
               int *nx, int *ny, int *nz, int *nc, double *dx, double *dy, const double *dz,
               double *massvol, double *massdep, double *massdry, double *masswet);
 
+/* No counterpart in the reference: cmass3d_ (Sources/cmass3d.f90:1-24) over
+ * arrays already in device memory, for callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc, hipFree or allocating routine: its partial sums
+ * live in one fixed device array of 512 KiB (4 x 256 partials per member,
+ * at most 64 members).  c: nmembers slabs of (0:nx+1,0:ny+1,0:nz+1,nc),
+ * member after member; cload, cdry, cwet: nmembers arrays (nx,ny,0:nc), of
+ * which slot 0 is read; dz: (0:nz); out: 4*nmembers doubles, massvol,
+ * massdep, massdry, masswet of each member (times dx*dy, as cmass3d_).  All
+ * pointers are device pointers; the work is queued on `stream` (a
+ * hipStream_t, NULL: the null stream) and the call returns at once.  Every
+ * call on one GPU shares the partials: calls that overlap on one GPU race.
+ * More than 64 members is refused (F3D_EINVAL). */
+int f3d_cmass3d_noalloc(const double *c, const double *cload, const double *cdry,
+                        const double *cwet, int nx, int ny, int nz, int nc, int nmembers,
+                        double dx, double dy, const double *dz, double *out, void *stream);
+
 /* ---------------- layer 2: device-resident context ---------------- */
 
 typedef struct f3d_ctx f3d_ctx;

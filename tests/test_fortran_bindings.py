@@ -87,6 +87,7 @@ def _call_program():
              "  rc = f3d_batch_step(batch, 1)", "  rc = f3d_batch_sync(batch)",
              "  rc = f3d_batch_autotune(batch, 2, i, j, k, s)", "  call f3d_batch_destroy(batch)",
              "  rc = f3d_mem_info(b1, b2)", "  rc = f3d_hbm_copy(b1, 1, s)",
+             "  rc = f3d_cmass3d_noalloc(ctx, ctx, ctx, ctx, 2, 2, 2, 1, 1, s, t, ctx, ctx, ctx)",
              "  msg = f3d_error_message()", "  call f3d_destroy(ctx)",
              "  print *, rc, msg, v", "end program abi_check"]
     called = {m.group(1) for m in re.finditer(r"(f3d_\w+)\(", "\n".join(lines))}
