@@ -39,8 +39,21 @@ UNITS = {
 }
 
 
-def _compile(src, flags):
+def _fresh(obj, src):
+    """The object is newer than its source and every header of csrc/."""
+    if not os.path.exists(obj):
+        return False
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC)
+                                        if h.endswith(".h")]
+    deps.append(os.path.join(HERE, "..", "include", "fall3d_amd.h"))
+    return all(os.path.getmtime(d) < t for d in deps)
+
+
+def _compile(src, flags, incremental=False):
     obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+    if incremental and _fresh(obj, src):
+        return obj
     cmd = [HIPCC, *COMMON, *flags, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -49,10 +62,12 @@ def _compile(src, flags):
     return obj
 
 
-def build(verbose=False):
+def build(verbose=False, incremental=False):
+    """Compile every unit (incremental: only those older than their source
+    or a header) and link the library and the Fortran driver."""
     os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(6, os.cpu_count() or 1)) as ex:
-        objs = list(ex.map(lambda kv: _compile(*kv), UNITS.items()))
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(lambda kv: _compile(*kv, incremental=incremental), UNITS.items()))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -92,5 +107,5 @@ def build_driver(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)

@@ -461,8 +461,6 @@ int f3d_autotune(f3d_ctx *c, int trials, int *tx, int *ty, int *slabs, double *b
     a.wet = 0;  // tuning leaves cwet alone
     const f32::Fields f = fp32_fields(c);
     std::vector<f32::TileShape> cand(std::begin(kCandidates), std::end(kCandidates));
-    cand.push_back({16, 16, 0, 0});
-    cand.push_back({32, 8, 0, 0});
     double best = DBL_MAX;
     f32::TileShape pick = c->tile;
     hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -219,9 +219,9 @@ int f3d_set_cd_state(f3d_ctx *ctx, double cd);
 int f3d_sync(f3d_ctx *ctx);
 /* Tile shape (columns x rows) of the fused FP32 step kernel.  No
  * counterpart in the reference: a tuning knob set by the autotuner.
- * Supported: 32x8, 32x16, 64x8, 64x4, 16x16 (column-tile kernel) and
- * 58x10, 58x18, 58x26 (flux kernel, 64-lane rows).  Default: 58x18 with two
- * member*class slabs per workgroup (f3d_set_tile_config). */
+ * Supported: 58x10, 58x18, 58x26 (the flux kernel: 64-lane region rows,
+ * 58 output columns).  Default: 58x18 with two member*class slabs per
+ * workgroup (f3d_set_tile_config). */
 int f3d_set_tile(f3d_ctx *ctx, int tx, int ty);
 /* The same with the 58-column kernel's options: waves per workgroup (0 or
  * (ty+6)/2: two region rows per wave; or 8 with ty = 18: three rows per
@@ -265,7 +265,7 @@ int f3d_batch_sync(f3d_batch *batch);
  * state changes) and keep the fastest; outputs may be null */
 int f3d_batch_autotune(f3d_batch *batch, int trials, int *tx, int *ty, int *slabs,
                        double *best_ms);
-/* the same for one context and its f3d_step (column-tile shapes included) */
+/* the same for one context and its f3d_step */
 int f3d_autotune(f3d_ctx *ctx, int trials, int *tx, int *ty, int *slabs, double *best_ms);
 
 /* Free and total device memory (admission control of the serving loop).  No

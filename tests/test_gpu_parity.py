@@ -236,8 +236,8 @@ def test_fp32_context_vs_golden(path):
         r.close()
 
 
-TILES = [(32, 8), (16, 16), (64, 4), (58, 10), (58, 18), (58, 26), (58, 10, 0, 2), (58, 18, 0, 2),
-         (58, 26, 0, 2), (58, 18, 8, 1), (58, 18, 8, 2)]
+TILES = [(58, 10), (58, 18), (58, 26), (58, 10, 0, 2), (58, 18, 0, 2), (58, 26, 0, 2),
+         (58, 18, 8, 1), (58, 18, 8, 2)]
 
 
 def _tile_id(t):
