@@ -48,6 +48,13 @@ struct f3d_batch {
     // item of each context in the one setsrc launch
     DeviceBuffer sdesc[2], sfirst;
     int nsrc = 0;
+    // HIP-event timing of the persistent flux launches (steps launch directly
+    // while it is on)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_start, ev_stop;
+    int ev_next = 0;
+    double t_ms = 0.0;
+    long t_count = 0;
 };
 
 namespace {
@@ -199,6 +206,20 @@ bool plan_current(const f3d_batch *b)
     return true;
 }
 
+void flush_timing(f3d_batch *b)
+{
+    if (b->ev_next == 0) return;
+    (void)hipEventSynchronize(b->ev_stop[b->ev_next - 1]);
+    for (int i = 0; i < b->ev_next; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, b->ev_start[i], b->ev_stop[i]) == hipSuccess) {
+            b->t_ms += ms;
+            b->t_count += 1;
+        }
+    }
+    b->ev_next = 0;
+}
+
 // device work of one step in which context 0 (and its parity group) takes
 // parity odd0
 int record_step(f3d_batch *b, bool odd0)
@@ -212,7 +233,16 @@ int record_step(f3d_batch *b, bool odd0)
     TRY(f32::setsrc_batch(s, b->sdesc[o].as<f32::SrcArgs>(), b->sfirst.as<int>(), (int)b->ctx.size(),
                           b->nsrc));
     TRY(f32::frame_batch(s, b->fdesc[o].as<f32::FrameArgs>(), b->frows.as<int2>(), b->nfrows));
+    // the events bracket the persistent flux launches (the side stream's
+    // list is joined back before the stop event); only when timing, which
+    // launches directly (never inside a capture)
+    if (b->timing) F3D_HIP(hipEventRecord(b->ev_start[b->ev_next], s));
     TRY(f32::flux_batch_step(s, b->side, b->fork, b->join, b->plan, odd0));
+    if (b->timing) {
+        F3D_HIP(hipEventRecord(b->ev_stop[b->ev_next], s));
+        b->ev_next += 1;
+        if (b->ev_next == (int)b->ev_stop.size()) flush_timing(b);
+    }
     return F3D_OK;
 }
 
@@ -327,6 +357,8 @@ void f3d_batch_destroy(f3d_batch *b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->side) (void)hipStreamSynchronize(b->side);
     drop_graphs(b);
+    for (hipEvent_t e : b->ev_start) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->ev_stop) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->join) (void)hipEventDestroy(b->join);
     if (b->order) (void)hipEventDestroy(b->order);
@@ -382,7 +414,7 @@ int f3d_batch_step(f3d_batch *b, int nsteps)
             for (int m = 0; m < c->g.nm; ++m) c->erumass[m] = c->erumass[m] + c->flowrate[m] * c->dt;
         }
         const bool odd = (b->ctx[0]->iiter & 1) != 0;
-        if (b->use_graph) {
+        if (b->use_graph && !b->timing) {
             if (!b->graph[odd ? 1 : 0]) TRY(capture(b, odd));
             F3D_HIP(hipGraphLaunch(b->graph[odd ? 1 : 0], b->stream));
         } else {
@@ -391,6 +423,35 @@ int f3d_batch_step(f3d_batch *b, int nsteps)
         for (f3d_ctx *c : b->ctx) c->cur = 1 - c->cur;
     }
     TRY(members_after(b));
+    return F3D_OK;
+}
+
+int f3d_batch_timing(f3d_batch *b, int enable)
+{
+    if (!check_batch(b)) return F3D_EINVAL;
+    if (enable && b->ev_start.empty()) {
+        b->ev_start.resize(256);
+        b->ev_stop.resize(256);
+        for (size_t i = 0; i < b->ev_start.size(); ++i) {
+            F3D_HIP(hipEventCreate(&b->ev_start[i]));
+            F3D_HIP(hipEventCreate(&b->ev_stop[i]));
+        }
+    }
+    if (!enable) flush_timing(b);
+    b->timing = enable != 0;
+    return F3D_OK;
+}
+
+int f3d_batch_kernel_time(f3d_batch *b, int reset, double *avg_ms, long *count)
+{
+    if (!check_batch(b)) return F3D_EINVAL;
+    flush_timing(b);
+    if (avg_ms) *avg_ms = b->t_count ? b->t_ms / (double)b->t_count : 0.0;
+    if (count) *count = b->t_count;
+    if (reset) {
+        b->t_ms = 0.0;
+        b->t_count = 0;
+    }
     return F3D_OK;
 }
 

@@ -41,7 +41,7 @@ module fall3d_amd
             f3d_batch_autotune, f3d_autotune, f3d_batch_set_tile, f3d_batch_get_tile, &
             f3d_batch_use_graph, f3d_set_tile, f3d_set_tile_config, f3d_timing, &
             f3d_kernel_time, f3d_device_bytes, f3d_mem_info, f3d_hbm_copy, &
-            f3d_cmass3d_noalloc
+            f3d_cmass3d_noalloc, f3d_batch_timing, f3d_batch_kernel_time
 
   interface
      integer(c_int) function f3d_create(ctx, grid, precision) bind(C, name='f3d_create')
@@ -346,6 +346,21 @@ module fall3d_amd
        type(c_ptr), value :: batch
        integer(c_int), intent(out) :: tx, ty, slabs
      end function f3d_batch_get_tile
+
+     integer(c_int) function f3d_batch_timing(batch, enable) bind(C, name='f3d_batch_timing')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+       integer(c_int), value :: enable
+     end function f3d_batch_timing
+
+     integer(c_int) function f3d_batch_kernel_time(batch, reset, avg_ms, count) &
+          bind(C, name='f3d_batch_kernel_time')
+       import :: c_int, c_ptr, c_double, c_long
+       type(c_ptr), value :: batch
+       integer(c_int), value :: reset
+       real(c_double), intent(out) :: avg_ms
+       integer(c_long), intent(out) :: count
+     end function f3d_batch_kernel_time
 
      integer(c_int) function f3d_batch_use_graph(batch, on) bind(C, name='f3d_batch_use_graph')
        import :: c_int, c_ptr

@@ -37,7 +37,8 @@ CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                "f3d_last_error", "f3d_hbm_copy", "f3d_batch_create", "f3d_batch_destroy",
                "f3d_batch_set_tile", "f3d_batch_get_tile", "f3d_batch_use_graph",
                "f3d_batch_step", "f3d_batch_sync", "f3d_batch_autotune", "f3d_autotune",
-               "f3d_set_meteo_window", "f3d_mem_info", "f3d_cmass3d_noalloc"]
+               "f3d_set_meteo_window", "f3d_mem_info", "f3d_cmass3d_noalloc",
+               "f3d_batch_timing", "f3d_batch_kernel_time"]
 
 
 class F3DError(RuntimeError):
@@ -110,6 +111,8 @@ def lib():
         L.f3d_batch_use_graph.argtypes = [vp, i]
         L.f3d_batch_step.argtypes = [vp, i]
         L.f3d_batch_sync.argtypes = [vp]
+        L.f3d_batch_timing.argtypes = [vp, i]
+        L.f3d_batch_kernel_time.argtypes = [vp, i, dp, ctypes.POINTER(ctypes.c_long)]
         L.f3d_batch_autotune.argtypes = [vp, i, ip, ip, ip, dp]
         L.f3d_autotune.argtypes = [vp, i, ip, ip, ip, dp]
         for name in ("f3d_batch_create", "f3d_batch_set_tile", "f3d_batch_get_tile",
@@ -495,6 +498,15 @@ class Batch:
 
     def step(self, n=1):
         _check(lib().f3d_batch_step(self.h, int(n)), "f3d_batch_step")
+
+    def timing(self, enable=True):
+        _check(lib().f3d_batch_timing(self.h, 1 if enable else 0), "f3d_batch_timing")
+
+    def kernel_time(self, reset=False):
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        _check(lib().f3d_batch_kernel_time(self.h, 1 if reset else 0, ctypes.byref(ms),
+                                           ctypes.byref(n)), "f3d_batch_kernel_time")
+        return ms.value, n.value
 
     def sync(self):
         _check(lib().f3d_batch_sync(self.h), "f3d_batch_sync")

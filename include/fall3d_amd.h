@@ -261,6 +261,11 @@ int f3d_batch_get_tile(const f3d_batch *batch, int *tx, int *ty, int *slabs);
 int f3d_batch_use_graph(f3d_batch *batch, int on);
 int f3d_batch_step(f3d_batch *batch, int nsteps);
 int f3d_batch_sync(f3d_batch *batch);
+/* HIP-event timing of the batch's persistent flux launches, as f3d_timing /
+ * f3d_kernel_time for a context: while it is on, steps launch directly
+ * instead of from graphs (the same kernels, one launch each) */
+int f3d_batch_timing(f3d_batch *batch, int enable);
+int f3d_batch_kernel_time(f3d_batch *batch, int reset, double *avg_ms, long *count);
 /* time each candidate tile shape over `trials` flux steps of the batch (no
  * state changes) and keep the fastest; outputs may be null */
 int f3d_batch_autotune(f3d_batch *batch, int trials, int *tx, int *ty, int *slabs,

@@ -28,7 +28,7 @@ def test_reference_leg_on_injected_coefficients_equals_oracle(monkeypatch):
         seen["run"] = self
 
     monkeypatch.setattr(RefRun, "step", spy)
-    res = bench.ref_cpu_baseline(sc, sc.updates_per_step, 0.2, coef, o.dt)
+    res = bench.ref_cpu_baseline(sc, 0.2, coef, o.dt)
     n = int(res["sample"].split()[0])
     assert res["value"] > 0 and res["cores"] == 1 and res["kind"] == "reference"
     for _ in range(n):

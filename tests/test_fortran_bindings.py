@@ -85,6 +85,7 @@ def _call_program():
              "  rc = f3d_batch_create(batch, ctxs, 1)", "  rc = f3d_batch_set_tile(batch, 58, 18, 2)",
              "  rc = f3d_batch_get_tile(batch, i, j, k)", "  rc = f3d_batch_use_graph(batch, 1)",
              "  rc = f3d_batch_step(batch, 1)", "  rc = f3d_batch_sync(batch)",
+             "  rc = f3d_batch_timing(batch, 1)", "  rc = f3d_batch_kernel_time(batch, 1, s, n)",
              "  rc = f3d_batch_autotune(batch, 2, i, j, k, s)", "  call f3d_batch_destroy(batch)",
              "  rc = f3d_mem_info(b1, b2)", "  rc = f3d_hbm_copy(b1, 1, s)",
              "  rc = f3d_cmass3d_noalloc(ctx, ctx, ctx, ctx, 2, 2, 2, 1, 1, s, t, ctx, ctx, ctx)",
