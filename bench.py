@@ -310,28 +310,35 @@ def _oracle_on_device_coef(sc):
 
 def uneven_cpu_baseline(scs, args):
     """The oracle over a sample of the uneven batch's own scenarios (every
-    16th of the seeded draw), one step each on the device meteo stage's
-    coefficients, until about cpu_seconds of step time; the reference's
-    routines on one core over the first of them."""
-    nupd, tcpu, used, first = 0, 0.0, 0, None
-    for sc in scs[::16]:
+    16th of the seeded draw) on the device meteo stage's coefficients, each
+    stepped until its equal share of about cpu_seconds of step time (at
+    least one step, at most 20); the reference's routines on one core over
+    the first of them."""
+    sample = scs[::16]
+    share = args.cpu_seconds / len(sample)
+    nupd, tcpu, nsteps, first = 0, 0.0, 0, None
+    for sc in sample:
         o, coef = _oracle_on_device_coef(sc)
         if first is None:
             first = (sc, coef, o.dt)
-        t0 = time.perf_counter()
-        o.step()
-        tcpu += time.perf_counter() - t0
-        nupd += sc.updates_per_step
-        used += 1
+        t_sc, n_sc = 0.0, 0
+        while n_sc == 0 or (t_sc < share and n_sc < 20):
+            t0 = time.perf_counter()
+            o.step()
+            t_sc += time.perf_counter() - t0
+            n_sc += 1
+        tcpu += t_sc
+        nsteps += n_sc
+        nupd += sc.updates_per_step * n_sc
         del o, coef
-        if tcpu >= args.cpu_seconds:
-            break
-    nthreads = max(_threads(sc.nc) for sc in scs[::16][:used])
-    log(f"uneven: cpu oracle {nupd / tcpu:.3e} updates/s over {used} scenarios ({tcpu:.1f}s)")
+    nthreads = max(_threads(sc.nc) for sc in sample)
+    log(f"uneven: cpu oracle {nupd / tcpu:.3e} updates/s, {nsteps} steps of {len(sample)} "
+        f"scenarios ({tcpu:.1f}s)")
     cpu = dict(value=nupd / tcpu, unit="cell*class updates/s", cores=nthreads, kind="port",
-               sample=f"one step of each of {used} scenarios of the uneven batch (every 16th "
-                      f"of the seeded draw; FP64 C restatement, up to {nthreads} "
-                      f"class-parallel threads, coefficients from the device meteo stage)")
+               sample=f"{nsteps} steps over {len(sample)} scenarios of the uneven batch (every "
+                      f"16th of the seeded draw, {tcpu:.1f} s of step time; FP64 C "
+                      f"restatement, up to {nthreads} class-parallel threads, coefficients "
+                      f"from the device meteo stage)")
     if args.ref_seconds > 0:
         cpu["reference_1core"] = ref_cpu_baseline(first[0], args.ref_seconds, first[1], first[2])
     return cpu

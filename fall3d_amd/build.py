@@ -36,6 +36,7 @@ UNITS = {
     "f3d_flux32.hip": FAST,
     "f3d_batch.hip": EXACT,
     "f3d_noalloc.hip": EXACT,
+    "f3d_output.hip": EXACT,
 }
 
 

@@ -288,6 +288,10 @@ static int create_impl(f3d_ctx **out, const f3d_grid *gr, int precision, int nm)
     c->flowrate.assign(nm, 0.0);
     c->erumass.assign(nm, 0.0);
     c->lonlat = gr->lonlat;
+    // reagrd.f90:155-158: dX1 = Rearth*(dx*pi/180) with dx in degrees
+    const double m_per_deg = 6356.0e3 * (3.14159265358979323846 / 180.0);
+    c->khn_dx = !gr->lonlat ? gr->dx : (gr->dlon > 0.0 ? gr->dlon : gr->dx / m_per_deg);
+    c->khn_dy = !gr->lonlat ? gr->dy : (gr->dlat > 0.0 ? gr->dlat : gr->dy / m_per_deg);
     const Grid &g = c->g;
     c->zlayer.assign(gr->zlayer, gr->zlayer + g.nz);
     vertical_coefs(g.nz, gr->zlayer, c->dz, c->zc, c->ldz);
@@ -482,7 +486,7 @@ static int meteo_stage(f3d_ctx *c)
     TRY(met::kappa3(s, g, c->modkv, c->modkh, c->d_dz.as<double>(), c->d_zlayer.as<double>(),
                     c->d_vx.as<double>(), c->d_vy.as<double>(), c->d_vptemp.as<double>(),
                     c->d_zi.as<double>(), c->d_rl.as<double>(), c->d_ustar.as<double>(),
-                    c->d_hm1.as<double>(), c->d_kmh.as<double>(), g.dx, g.dy, c->rkv0, c->rkh0,
+                    c->d_hm1.as<double>(), c->d_kmh.as<double>(), c->khn_dx, c->khn_dy, c->rkv0, c->rkh0,
                     c->csh, c->lonlat, c->d_rkver.as<double>(), c->d_rkhor.as<double>()));
     // meteo.f90:81 setvset, continuing the SAVEd drag-coefficient chain
     double cd2[2] = {c->cd_state, c->cd_state};

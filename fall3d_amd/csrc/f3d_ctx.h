@@ -215,6 +215,8 @@ struct f3d_ctx {
     int precision = F3D_FP64;
     f3d::Grid g{};
     int lonlat = 0;
+    // the spacing kappa3's CMAQ Khn takes: degrees on LON-LAT grids, m on UTM
+    double khn_dx = 0.0, khn_dy = 0.0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;                     // edge tiles of the flux kernel
     unsigned gen = 0;  // bumped when fields or sources change (batched steps re-plan)
@@ -245,6 +247,9 @@ struct f3d_ctx {
     // corrects vz for the terrain slope (corvver, meteo.f90:116-124)
     f3d::DeviceBuffer d_topg;
     bool topg_set = false;
+    // pressure (nx,ny,nz) in Pa (Master.f90:103), for the flight-level cuts
+    f3d::DeviceBuffer d_p;
+    bool p_set = false;
     // precipitation rate (nx,ny, mm/h) and per-class wet flags (diameter
     // up to 100 um): wet deposition in the FP64 step (sizloop.f90:65-67)
     f3d::DeviceBuffer d_prate, d_wet;

@@ -165,7 +165,7 @@ namespace met {
 int kappa3(hipStream_t s, const Grid &g, int modkv, int modkh, const double *dz,
            const double *zlayer, const double *vx, const double *vy, const double *vptemp,
            const double *zi, const double *rl, const double *ustar, const double *hm1,
-           const double *kmh, double dx_num, double dy_num, double rkv0, double rkh0,
+           const double *kmh, double khn_dx, double khn_dy, double rkv0, double rkh0,
            double csh, int lonlat, double *rkver, double *rkhor);
 // Per-class constants prepared on the host: rk1, rk2 (Ganser) and psi.
 struct ClassConst {

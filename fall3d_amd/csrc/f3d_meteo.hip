@@ -31,11 +31,12 @@ __device__ __forceinline__ void similarity_terms(int nx, int ny, int i, int j, i
            (vptemp[midx(nx, ny, i, j, k)] * dvdz2);
 }
 
-// kappa3.f90:50-236, one thread per cell
+// kappa3.f90:50-236, one thread per cell; khn_dx, khn_dy: the grid spacing
+// of CMAQ's Khn (kappa3.f90:226), degrees on LON-LAT grids
 __global__ void k_kappa3(Grid g, int modkv, int modkh, const double *dz, const double *zlayer,
                          const double *vx, const double *vy, const double *vptemp,
                          const double *zi, const double *rl, const double *ustar,
-                         const double *hm1, const double *kmh, double dx_num, double dy_num,
+                         const double *hm1, const double *kmh, double khn_dx, double khn_dy,
                          double rkv0, double rkh0, double csh, int lonlat, double *rkver,
                          double *rkhor)
 {
@@ -129,7 +130,7 @@ __global__ void k_kappa3(Grid g, int modkv, int modkh, const double *dz, const d
             const double Khn0 = lonlat ? 3.6e-2 : 4000.0;
             const double a = dvxdx - dvydy, b = dvydx + dvxdy;
             const double Kht = kAlfa * kAlfa * dXX * dX2 * sqrt(a * a + b * b);
-            const double Khn = kKhf * (Khn0 / hm1[p2] / dx_num) * (Khn0 / dy_num);
+            const double Khn = kKhf * (Khn0 / hm1[p2] / khn_dx) * (Khn0 / khn_dy);
             r = (1.0 / Kht) + (1.0 / Khn);
             r = fmax_f(1.0 / r, kRkhmin);
         }
@@ -397,11 +398,11 @@ namespace met {
 int kappa3(hipStream_t s, const Grid &g, int modkv, int modkh, const double *dz,
            const double *zlayer, const double *vx, const double *vy, const double *vptemp,
            const double *zi, const double *rl, const double *ustar, const double *hm1,
-           const double *kmh, double dx_num, double dy_num, double rkv0, double rkh0,
+           const double *kmh, double khn_dx, double khn_dy, double rkv0, double rkh0,
            double csh, int lonlat, double *rkver, double *rkhor)
 {
     hipLaunchKernelGGL(k_kappa3, dim3(blocks_for(g.N, 256)), dim3(256), 0, s, g, modkv, modkh, dz,
-                       zlayer, vx, vy, vptemp, zi, rl, ustar, hm1, kmh, dx_num, dy_num, rkv0,
+                       zlayer, vx, vy, vptemp, zi, rl, ustar, hm1, kmh, khn_dx, khn_dy, rkv0,
                        rkh0, csh, lonlat, rkver, rkhor);
     F3D_CHECK_LAUNCH();
     return F3D_OK;

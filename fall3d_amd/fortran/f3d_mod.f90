@@ -24,7 +24,16 @@ module fall3d_amd
      type(c_ptr)    :: zlayer = c_null_ptr
      type(c_ptr)    :: hm1 = c_null_ptr
      integer(c_int) :: lonlat = 0
+     real(c_double) :: dlon = 0.0_c_double, dlat = 0.0_c_double
   end type f3d_grid
+
+  ! run date, extent and clock of a restart file (wrirst.f90:120-172)
+  type, bind(C), public :: f3d_run_info
+     integer(c_int) :: year = 0, month = 0, day = 0, run_start = 0, run_end = 0
+     real(c_double) :: xmin = 0.0_c_double, ymin = 0.0_c_double
+     real(c_double) :: xmax = 0.0_c_double, ymax = 0.0_c_double
+     real(c_double) :: erumass = 0.0_c_double, outmass = 0.0_c_double
+  end type f3d_run_info
 
   public :: f3d_create, f3d_destroy, f3d_set_classes, f3d_set_models, f3d_set_meteo, &
             f3d_get_dt, f3d_set_dt, f3d_set_source, f3d_step, f3d_budget, f3d_get_state, &
@@ -32,7 +41,8 @@ module fall3d_amd
             f3d_get_cd_state, f3d_set_cd_state, f3d_error_message
   ! meteo-stage extensions and output products
   public :: f3d_set_landuse, f3d_set_topography, f3d_set_precipitation, &
-            f3d_output_products, f3d_get_vdep
+            f3d_output_products, f3d_get_vdep, f3d_set_pressure, f3d_output_concentration, &
+            f3d_write_restart, f3d_read_restart, f3d_restart_write, f3d_restart_read
   ! ensembles and resident meteorology (no counterpart in the reference)
   public :: f3d_create_ensemble, f3d_set_member_source, f3d_ensemble_budget, &
             f3d_get_member_state, f3d_set_meteo_window
@@ -110,6 +120,61 @@ module fall3d_amd
        integer(c_int), value :: member
        real(c_double), intent(out) :: load(*), thick(*), colmass(*)
      end function f3d_output_products
+
+     ! pressure p(nx,ny,nz) in Pa for the flight-level cuts
+     integer(c_int) function f3d_set_pressure(ctx, p) bind(C, name='f3d_set_pressure')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: p(*)
+     end function f3d_set_pressure
+
+     ! printres_nc concentrations: cgrnd(nx,ny), con(nx,ny,nz), cfl(nx,ny,nfl)
+     integer(c_int) function f3d_output_concentration(ctx, member, cgrnd, con, asl, nfl, fl, cfl) &
+          bind(C, name='f3d_output_concentration')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: member, asl, nfl
+       real(c_double), intent(out) :: cgrnd(*), con(*), cfl(*)
+       real(c_double), intent(in) :: fl(*)
+     end function f3d_output_concentration
+
+     ! wrirst / rearst (path: a NUL-terminated character string)
+     integer(c_int) function f3d_write_restart(ctx, member, path, info) &
+          bind(C, name='f3d_write_restart')
+       import :: c_int, c_ptr, c_char, f3d_run_info
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: member
+       character(kind=c_char), intent(in) :: path(*)
+       type(f3d_run_info), intent(in) :: info
+     end function f3d_write_restart
+
+     integer(c_int) function f3d_read_restart(ctx, path, cload, info) &
+          bind(C, name='f3d_read_restart')
+       import :: c_int, c_ptr, c_char, c_double, f3d_run_info
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+       real(c_double), intent(out) :: cload(*)
+       type(f3d_run_info), intent(out) :: info
+     end function f3d_read_restart
+
+     ! the restart file on host arrays c(nx+2,ny+2,nz+2,nc), cload(nx,ny,nc+1)
+     integer(c_int) function f3d_restart_write(path, nx, ny, nz, nc, lonlat, c, cload, info) &
+          bind(C, name='f3d_restart_write')
+       import :: c_int, c_char, c_double, f3d_run_info
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int), value :: nx, ny, nz, nc, lonlat
+       real(c_double), intent(in) :: c(*), cload(*)
+       type(f3d_run_info), intent(in) :: info
+     end function f3d_restart_write
+
+     integer(c_int) function f3d_restart_read(path, nx, ny, nz, nc, c, cload, info) &
+          bind(C, name='f3d_restart_read')
+       import :: c_int, c_char, c_double, f3d_run_info
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int), value :: nx, ny, nz, nc
+       real(c_double), intent(out) :: c(*), cload(*)
+       type(f3d_run_info), intent(out) :: info
+     end function f3d_restart_read
 
      integer(c_int) function f3d_get_vdep(ctx, vdep) bind(C, name='f3d_get_vdep')
        import :: c_int, c_ptr, c_double

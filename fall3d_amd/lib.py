@@ -30,7 +30,8 @@ CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_set_landuse",
                "f3d_get_vdep", "f3d_set_topography", "f3d_set_precipitation",
-               "f3d_output_products",
+               "f3d_output_products", "f3d_set_pressure", "f3d_output_concentration",
+               "f3d_write_restart", "f3d_read_restart", "f3d_restart_write", "f3d_restart_read",
                "f3d_get_clock",
                "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
                "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time", "f3d_device_bytes",
@@ -45,10 +46,19 @@ class F3DError(RuntimeError):
     pass
 
 
+class _RunInfo(ctypes.Structure):
+    _fields_ = [("year", ctypes.c_int), ("month", ctypes.c_int), ("day", ctypes.c_int),
+                ("run_start", ctypes.c_int), ("run_end", ctypes.c_int),
+                ("xmin", ctypes.c_double), ("ymin", ctypes.c_double),
+                ("xmax", ctypes.c_double), ("ymax", ctypes.c_double),
+                ("erumass", ctypes.c_double), ("outmass", ctypes.c_double)]
+
+
 class _Grid(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
                 ("nc", ctypes.c_int), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
-                ("zlayer", _dp), ("hm1", _dp), ("lonlat", ctypes.c_int)]
+                ("zlayer", _dp), ("hm1", _dp), ("lonlat", ctypes.c_int),
+                ("dlon", ctypes.c_double), ("dlat", ctypes.c_double)]
 
 
 def lib():
@@ -88,6 +98,14 @@ def lib():
         L.f3d_set_topography.argtypes = [vp, dp]
         L.f3d_set_precipitation.argtypes = [vp, dp]
         L.f3d_output_products.argtypes = [vp, i, dp, dp, dp]
+        L.f3d_set_pressure.argtypes = [vp, dp]
+        L.f3d_output_concentration.argtypes = [vp, i, dp, dp, i, i, dp, dp]
+        L.f3d_write_restart.argtypes = [vp, i, ctypes.c_char_p, ctypes.POINTER(_RunInfo)]
+        L.f3d_read_restart.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(_RunInfo)]
+        L.f3d_restart_write.argtypes = [ctypes.c_char_p, i, i, i, i, i, dp, dp,
+                                        ctypes.POINTER(_RunInfo)]
+        L.f3d_restart_read.argtypes = [ctypes.c_char_p, i, i, i, i, dp, dp,
+                                       ctypes.POINTER(_RunInfo)]
         L.f3d_get_clock.argtypes = [vp, dp, ip, dp, dp]
         L.f3d_set_clock.argtypes = [vp, d, i, d, d]
         L.f3d_get_cd_state.argtypes = [vp, dp]
@@ -124,13 +142,36 @@ def lib():
                      "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                      "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_set_landuse",
                      "f3d_get_vdep", "f3d_set_topography", "f3d_set_precipitation",
-                     "f3d_output_products",
+                     "f3d_output_products", "f3d_set_pressure", "f3d_output_concentration",
+                     "f3d_write_restart", "f3d_read_restart", "f3d_restart_write",
+                     "f3d_restart_read",
                      "f3d_get_clock",
                      "f3d_set_clock", "f3d_get_cd_state", "f3d_set_cd_state", "f3d_sync",
                      "f3d_set_tile", "f3d_set_tile_config", "f3d_timing", "f3d_kernel_time"):
             getattr(L, name).restype = c
         _L = L
     return _L
+
+
+def restart_write(path, c, cload, lonlat=0, **info):
+    """Restart file (netCDF classic, wrirst's layout) from host arrays c
+    (nx+2,ny+2,nz+2,nc) and cload (nx,ny,nc+1); info: year, month, day,
+    run_start, run_end, xmin, ymin, xmax, ymax, erumass, outmass."""
+    c, cload = _F(c), _F(cload)
+    nx, ny, nz, nc = c.shape[0] - 2, c.shape[1] - 2, c.shape[2] - 2, c.shape[3]
+    ri = _RunInfo(**{k: info.get(k, 0) for k, _ in _RunInfo._fields_})
+    _check(lib().f3d_restart_write(os.fsencode(path), nx, ny, nz, nc, int(lonlat), _P(c),
+                                   _P(cload), ctypes.byref(ri)), "f3d_restart_write")
+
+
+def restart_read(path, nx, ny, nz, nc):
+    """(c, cload, attributes) of a restart file of the given grid."""
+    c = np.zeros((nx + 2, ny + 2, nz + 2, nc), order="F")
+    cload = np.zeros((nx, ny, nc + 1), order="F")
+    ri = _RunInfo()
+    _check(lib().f3d_restart_read(os.fsencode(path), nx, ny, nz, nc, _P(c), _P(cload),
+                                  ctypes.byref(ri)), "f3d_restart_read")
+    return c, cload, {k: getattr(ri, k) for k, _ in _RunInfo._fields_}
 
 
 def hbm_copy(nbytes=1 << 30, reps=5):
@@ -175,14 +216,15 @@ class Context:
     """One device-resident scenario (f3d_ctx)."""
 
     def __init__(self, nx, ny, nz, nc, dx, dy, zlayer, precision=FP32, hm1=None, lonlat=0,
-                 members=1):
+                 members=1, dlon=0.0, dlat=0.0):
         L = lib()
         self.nx, self.ny, self.nz, self.nc = nx, ny, nz, nc
         self.precision = precision
         self.members = members
         self._zl = _F(zlayer)
         self._hm1 = None if hm1 is None else _F(hm1)
-        g = _Grid(nx, ny, nz, nc, float(dx), float(dy), _P(self._zl), _P(self._hm1), lonlat)
+        g = _Grid(nx, ny, nz, nc, float(dx), float(dy), _P(self._zl), _P(self._hm1), lonlat,
+                  float(dlon), float(dlat))
         h = ctypes.c_void_p()
         if members == 1:
             _check(L.f3d_create(ctypes.byref(h), ctypes.byref(g), precision), "f3d_create")
@@ -323,6 +365,44 @@ class Context:
                "f3d_output_products")
         return load, thick, colmass
 
+    def set_pressure(self, p):
+        """Pressure (nx,ny,nz) in Pa, for the flight-level cuts."""
+        self._p = _F(p)
+        _check(lib().f3d_set_pressure(self.h, _P(self._p)), "f3d_set_pressure")
+
+    def concentration(self, member=0, asl=False, fl=None):
+        """Concentration outputs of a member in g/m3: C_GRND (nx,ny), CON
+        (nx,ny,nz) (terrain following, or a.s.l. with asl) and, for the
+        flight levels fl (hundreds of feet), the cuts (nx,ny,len(fl))."""
+        cg = np.zeros((self.nx, self.ny), order="F")
+        con = np.zeros((self.nx, self.ny, self.nz), order="F")
+        fl = None if fl is None else np.ascontiguousarray(fl, dtype=np.float64)
+        nfl = 0 if fl is None else int(fl.size)
+        cfl = np.zeros((self.nx, self.ny, max(nfl, 1)), order="F")
+        _check(lib().f3d_output_concentration(self.h, int(member), _P(cg), _P(con), int(bool(asl)),
+                                              nfl, None if fl is None else _P(fl),
+                                              _P(cfl) if nfl else None),
+               "f3d_output_concentration")
+        return cg, con, (cfl if nfl else None)
+
+    def write_restart(self, path, member=0, year=2000, month=1, day=1, run_start=0, run_end=0,
+                      extent=(0.0, 0.0, 0.0, 0.0)):
+        """Restart file of a member (netCDF classic: CON, LOAD and the run's
+        attributes, as wrirst writes it)."""
+        info = _RunInfo(int(year), int(month), int(day), int(run_start), int(run_end),
+                        *[float(v) for v in extent], 0.0, 0.0)
+        _check(lib().f3d_write_restart(self.h, int(member), os.fsencode(path), ctypes.byref(info)),
+               "f3d_write_restart")
+
+    def read_restart(self, path):
+        """Restores the state and the erupted/outflow mass from a restart
+        file; returns (cload (nx,ny,0:nc), attributes)."""
+        cload = np.zeros((self.nx, self.ny, self.nc + 1), order="F")
+        info = _RunInfo()
+        _check(lib().f3d_read_restart(self.h, os.fsencode(path), _P(cload), ctypes.byref(info)),
+               "f3d_read_restart")
+        return cload, {k: getattr(info, k) for k, _ in _RunInfo._fields_}
+
     def vdep(self):
         out = np.zeros((self.nx, self.ny, self.nc), order="F")
         _check(lib().f3d_get_vdep(self.h, _P(out)), "f3d_get_vdep")
@@ -382,7 +462,8 @@ class Run:
         self.sc = sc
         lonlat = int(getattr(sc, "lonlat", 0))
         self.ctx = Context(sc.nx, sc.ny, sc.nz, sc.nc, sc.dx, sc.dy, sc.zlayer, precision,
-                           hm1=getattr(sc, "hm1_cols", None), lonlat=lonlat)
+                           hm1=getattr(sc, "hm1_cols", None), lonlat=lonlat,
+                           dlon=getattr(sc, "dlon", 0.0), dlat=getattr(sc, "dlat", 0.0))
         if lonlat:
             self.ctx.set_topography(sc.topg)
         if tile is not None:

@@ -26,11 +26,13 @@ from typing import Optional
 import numpy as np
 
 __all__ = ["Scenario", "standard_atm", "make_classes", "suzuki_profile", "getshape", "suzuki_source", "config_c1",
-           "config_c2", "config_small", "config_ensemble_member", "config_uneven"]
+           "config_c2", "config_small", "config_ensemble_member", "config_uneven",
+           "lonlat_grid", "khn_spacing", "R_EARTH"]
 
 KV_CONSTANT, KV_SIMILARITY, KV_SURFACE = 0, 1, 2
 KH_CONSTANT, KH_RAMS, KH_CMAQ = 0, 1, 2
 VS_ARASTOOPOUR, VS_GANSER, VS_WILSON, VS_DELLINO = 1, 2, 3, 4
+R_EARTH = 6356.0e3  # Numeric.f90:14
 
 
 @dataclasses.dataclass
@@ -80,6 +82,10 @@ class Scenario:
     # LON-LAT coordinates: kappa3's CMAQ constant and the terrain-slope
     # correction of vz (corvver) follow them
     lonlat: int = 0
+    # LON-LAT grid spacing in degrees (reagrd.f90:155-156); dx, dy hold the
+    # scaled spacing Rearth*(dlon*pi/180) in m.  0: not given
+    dlon: float = 0.0
+    dlat: float = 0.0
     # precipitation rate (nx,ny) in mm/h: wet deposition on (wetdep.f90)
     prate: Optional[np.ndarray] = None
 
@@ -286,6 +292,30 @@ def _fourier_wind(rng, nx, ny, zlayer, dx, dy, nmodes, rms_h, rms_v):
     v = field(rms_h)
     w = field(rms_v)
     return u, v, w
+
+
+def lonlat_grid(sc: Scenario, lat_min: float, dlon: float, dlat: float) -> Scenario:
+    """sc on a LON-LAT grid of dlon x dlat degrees whose first row lies at
+    latitude lat_min: dX1 = Rearth*(dx*pi/180) (reagrd.f90:155-158) and
+    Hm1(:,j) = sin(colat) with lat = lat_min + (j-1)*dy (reagrd.f90:247-262);
+    kappa3's CMAQ constant and corvver follow the coordinate system."""
+    dx = R_EARTH * (dlon * math.pi / 180.0)
+    dy = R_EARTH * (dlat * math.pi / 180.0)
+    lat = lat_min + np.arange(sc.ny) * dlat
+    colat = (90.0 - lat) * math.pi / 180.0
+    hm1 = np.asfortranarray(np.repeat(np.sin(colat)[None, :], sc.nx, axis=0))
+    return dataclasses.replace(sc, dx=dx, dy=dy, dlon=dlon, dlat=dlat, hm1_cols=hm1, lonlat=1)
+
+
+def khn_spacing(sc: Scenario):
+    """The grid spacing kappa3's CMAQ Khn takes (kappa3.f90:226): dx, dy in m
+    on UTM grids, in degrees on LON-LAT grids (given, or dX1 / (Rearth*pi/180)
+    as the library derives it)."""
+    if not sc.lonlat:
+        return sc.dx, sc.dy
+    m_per_deg = R_EARTH * (math.pi / 180.0)
+    return (sc.dlon if sc.dlon > 0 else sc.dx / m_per_deg,
+            sc.dlat if sc.dlat > 0 else sc.dy / m_per_deg)
 
 
 def config_c1():

@@ -111,7 +111,13 @@ typedef struct {
     double dx, dy;               /* dX1, dX2 in m (reagrd.f90:194-200) */
     const double *zlayer;        /* (nz) layer heights; dz follows reagrd.f90:216-220 */
     const double *hm1;           /* (nx,ny) scale factor, NULL for UTM (Hm1 = 1) */
-    int lonlat;                  /* 0 UTM, 1 LON-LAT (CMAQ constant only) */
+    int lonlat;                  /* 0 UTM, 1 LON-LAT: kappa3's CMAQ constant and
+                                    spacing, corvver (meteo.f90:116-124) */
+    double dlon, dlat;           /* LON-LAT: the spacing dx, dy in degrees of the
+                                    reference (reagrd.f90:155-158), which kappa3's
+                                    CMAQ Khn takes (kappa3.f90:226); 0 derives it
+                                    from dX1, dX2 and Rearth = 6356 km (Numeric.f90:14).
+                                    Ignored on UTM grids. */
 } f3d_grid;
 
 /* Allocates every device buffer (Master.f90:getmem1 equivalent). */
@@ -209,6 +215,59 @@ int f3d_set_precipitation(f3d_ctx *ctx, const double *prate);
 int f3d_output_products(f3d_ctx *ctx, int member, double *load, double *thick, double *colmass);
 /* vdep (nx,ny,nc) as the last meteo read left it (Master vdep). */
 int f3d_get_vdep(f3d_ctx *ctx, double *vdep);
+/* Pressure p(nx,ny,nz) in Pa (Master.f90:103, read with the meteo in
+ * reamet), which the flight-level cuts interpolate in. */
+int f3d_set_pressure(f3d_ctx *ctx, const double *p);
+/* Concentration outputs of printres_nc.f90 for one member, in g/m3 (every
+ * class is a particle class, np = nc), each in the reference's operation
+ * order; null outputs are skipped:
+ *   cgrnd (nx,ny)     C_GRND = 1e3*sum_ic c(:,:,1,ic)/Hm1 (printres_nc.f90:391-396)
+ *   con   (nx,ny,nz)  CON = 1e3*sum_ic c(:,:,1:nz,ic)/Hm1 (printres_nc.f90:574-584);
+ *                     asl = 1 interpolates it to the heights zlayer above sea
+ *                     level on the topography (ter2asl.f90, the reference's
+ *                     terrain_following = .false., printres_nc.f90:586;
+ *                     needs f3d_set_topography)
+ *   cfl   (nx,ny,nfl) C_FLxxx: sum_ic c/Hm1 cut at flight level fl[l]
+ *                     (hundreds of feet) by the pressure of f3d_set_pressure
+ *                     against the ICAO standard atmosphere, times 1e3
+ *                     (printres_nc.f90:545-560, setcut.f90:1-107) */
+int f3d_output_concentration(f3d_ctx *ctx, int member, double *cgrnd, double *con, int asl,
+                             int nfl, const double *fl, double *cfl);
+
+/* Run date, extent and clock recorded in a restart file (the global
+ * attributes of wrirst.f90:120-172). */
+typedef struct {
+    int year, month, day;           /* YEAR, MONTH, DAY */
+    int run_start, run_end;         /* RUN_START, RUN_END (s) */
+    double xmin, ymin, xmax, ymax;  /* XMIN..YMAX (UTM, m) or LONMIN..LATMAX (LON-LAT, deg) */
+    double erumass, outmass;        /* ERUPTED_MASS, OUT_MASS: read back only; written
+                                       from the member's clock */
+} f3d_run_info;
+/* Restart file of wrirst.f90 in the netCDF classic format (64-bit offsets;
+ * written without the netCDF library): dimensions nx, ny, nc, nc1, nx2,
+ * ny2, nz2; CON(nx2,ny2,nz2,nc) = the state c with its ghost frame and
+ * LOAD(nx,ny,nc1) = cload(nx,ny,0:nc) = cdry + cwet with the class sums in
+ * slot 0 (endstep.f90:35,80-106), both 32-bit floats as NF90_FLOAT makes
+ * them; the global attributes of info and the member's erupted mass and
+ * outflow. */
+int f3d_write_restart(f3d_ctx *ctx, int member, const char *path, const f3d_run_info *info);
+/* rearst.f90: checks the file's grid against the context's (nx, ny, nz2 - 2,
+ * nc; F3D_EINVAL on a mismatch), sets the state c from CON and the clock's
+ * erupted mass and outflow from ERUPTED_MASS and OUT_MASS, and zeroes the
+ * dry and wet deposits, which the reference also starts again from zero
+ * (rearst.f90:186 restores cload only, which the next endstep overwrites
+ * with cdry + cwet, endstep.f90:35).  cload (nx,ny,nc+1) receives LOAD and
+ * info the attributes; both may be NULL.  Single-member contexts. */
+int f3d_read_restart(f3d_ctx *ctx, const char *path, double *cload, f3d_run_info *info);
+/* The file side of the two above on host arrays (no device work): c
+ * (nx+2,ny+2,nz+2,nc) and cload (nx,ny,nc+1) in, the attributes of info
+ * (erumass and outmass included) and the coordinate system (lonlat) with
+ * them; the reader checks the grid and fills c, cload and info (each may
+ * be NULL). */
+int f3d_restart_write(const char *path, int nx, int ny, int nz, int nc, int lonlat,
+                      const double *c, const double *cload, const f3d_run_info *info);
+int f3d_restart_read(const char *path, int nx, int ny, int nz, int nc, double *c, double *cload,
+                     f3d_run_info *info);
 int f3d_get_clock(f3d_ctx *ctx, double *time, int *iiter, double *erumass, double *outmass);
 int f3d_set_clock(f3d_ctx *ctx, double time, int iiter, double erumass, double outmass);
 /* The SAVEd vsettl drag coefficient carried between meteo reads

@@ -923,6 +923,94 @@ void or_output_products(int nx, int ny, int nz, int nc, const double *c, const d
         }
 }
 
+/* ICAO standard atmosphere, three layers (setcut.f90:61-107): p (Pa) and
+ * T (K) at height z (m a.s.l.). */
+void or_standard_atm(double z, double *p, double *T)
+{
+    const double goo = 9.80665, Roo = 8.31432, Moo = 0.0289644;
+    const double zoo = 0.0, Too = 288.15, poo = 101325.0, Loo = -0.0065;
+    const double zo1 = 11000.0, To1 = 216.65, po1 = 22632.0;
+    const double zo2 = 20000.0, To2 = 216.65, po2 = 5474.0, Lo2 = 0.001;
+    if (z < zo1) {
+        *T = Too + Loo * (z - zoo);
+        *p = poo * pow(Too / *T, goo * Moo / (Roo * Loo));
+    } else if (z < zo2) {
+        *T = To1;
+        *p = po1 * exp(-goo * Moo * (z - zo1) / (Roo * To1));
+    } else {
+        *T = To2 + Lo2 * (z - zo2);
+        *p = po2 * pow(To2 / *T, goo * Moo / (Roo * Lo2));
+    }
+}
+
+/* Concentration outputs of printres_nc.f90 (g/m3; all classes are
+ * particles, np = nc), each from w(k) = sum_ic c(:,:,k,ic) summed in class
+ * order:
+ *   cgrnd (nx,ny)    C_GRND = 1d3*w(1)/Hm1 (:391-396);
+ *   con (nx,ny,nz)   CON = 1d3*(w/Hm1) (:574-584), and with asl the
+ *                    ter2asl interpolation to a.s.l. heights (:586,
+ *                    ter2asl.f90:25-50) on topg;
+ *   cfl (nx,ny,nfl)  the cut of w/Hm1 at flight level fl[l] (hundreds of
+ *                    feet, fl*30.48 m) on the pressure p, times 1d3 (:545-560,
+ *                    setcut.f90:24-57).
+ * Null outputs are skipped. */
+void or_output_concentration(int nx, int ny, int nz, int nc, const double *c, const double *Hm1,
+                             const double *zlayer, const double *topg, const double *p, int nfl,
+                             const double *fl, double *cgrnd, double *con, int asl, double *cfl)
+{
+    const long nxy = (long)nx * ny, G = (long)(nx + 2) * (ny + 2) * (nz + 2);
+    double *w = (double *)malloc(sizeof(double) * (size_t)(nz + 1));
+    double *col = (double *)malloc(sizeof(double) * (size_t)(nz + 1));
+    double *pnom = (double *)malloc(sizeof(double) * (size_t)(nfl > 0 ? nfl : 1));
+    for (int l = 0; l < nfl; ++l) {
+        double T;
+        or_standard_atm(fl[l] * 30.48, &pnom[l], &T);
+    }
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            const long t = PI2(i, j);
+            for (int k = 1; k <= nz; ++k) {
+                double s = 0.0;
+                for (int q = 0; q < nc; ++q) s = s + c[q * G + CI(i, j, k)];
+                w[k] = s;
+            }
+            if (cgrnd) cgrnd[t] = 1e3 * w[1] / Hm1[t];
+            if (con) {
+                for (int k = 1; k <= nz; ++k) col[k] = 1e3 * (w[k] / Hm1[t]);
+                for (int k = 1; k <= nz; ++k) {
+                    double v = col[k];
+                    if (asl) {
+                        const double zo = fmax_f(0.0, topg[t]), zasl = zlayer[k - 1];
+                        v = 0.0;
+                        if (!(zasl < zo + zlayer[0]))
+                            for (int izz = 1; izz + 1 <= nz; ++izz)
+                                if (zasl >= zo + zlayer[izz - 1] && zasl <= zo + zlayer[izz]) {
+                                    const double s = (zo + zlayer[izz] - zasl) / (zlayer[izz] - zlayer[izz - 1]);
+                                    v = s * col[izz] + (1.0 - s) * col[izz + 1];
+                                    break;
+                                }
+                    }
+                    con[(long)(k - 1) * nxy + t] = v;
+                }
+            }
+            for (int l = 0; l < nfl && cfl; ++l) {
+                const double pn = pnom[l];
+                double v = 0.0;
+                if (!(pn > p[MI(i, j, 1)]))
+                    for (int iz = 1; iz + 1 <= nz; ++iz)
+                        if (pn <= p[MI(i, j, iz)] && pn > p[MI(i, j, iz + 1)]) {
+                            const double s = (pn - p[MI(i, j, iz)]) / (p[MI(i, j, iz + 1)] - p[MI(i, j, iz)]);
+                            v = (w[iz] / Hm1[t]) * (1.0 - s) + (w[iz + 1] / Hm1[t]) * s;
+                            break;
+                        }
+                cfl[(long)l * nxy + t] = 1e3 * v;
+            }
+        }
+    free(w);
+    free(col);
+    free(pnom);
+}
+
 /* ---- sizloop ---- */
 typedef struct {
     double *c, *cdry;

@@ -83,6 +83,10 @@ void or_setsrc(double *c, int nx, int ny, int nz, int nc, int nsou, const int *i
 
 /* printres_nc.f90:150-241 output products: load (nx,ny,0:nc) kg/m2,
  * thick (nx,ny) cm, colmass (nx,ny) g/m2; c (0:nx+1,0:ny+1,0:nz+1,nc) */
+void or_standard_atm(double z, double *p, double *T);
+void or_output_concentration(int nx, int ny, int nz, int nc, const double *c, const double *Hm1,
+                             const double *zlayer, const double *topg, const double *p, int nfl,
+                             const double *fl, double *cgrnd, double *con, int asl, double *cfl);
 void or_output_products(int nx, int ny, int nz, int nc, const double *c, const double *cdry,
                         const double *cwet, const double *Hm1, const double *rhop,
                         const double *dz, double *load, double *thick, double *colmass);

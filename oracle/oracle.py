@@ -74,6 +74,9 @@ def lib():
         L.or_diffy.argtypes = [dp, i, i, i, d, d, dp, dp]
         L.or_accum.argtypes = [dp, dp, i, i, i, i, d, dp, dp]
         L.or_clip.argtypes = [dp, ctypes.c_long]
+        L.or_standard_atm.argtypes = [d, dp, dp]
+        L.or_output_concentration.argtypes = [i, i, i, i, dp, dp, dp, dp, dp, i, dp, dp, dp, i,
+                                              dp]
         _lib = L
     return _lib
 
@@ -104,6 +107,42 @@ class _RunBase:
             if nsteps is None and self.time >= end:
                 break
         return n
+
+
+def _khn_spacing(sc):
+    """(dx, dy) as kappa3's CMAQ Khn takes them (kappa3.f90:226): m on UTM
+    grids; on LON-LAT grids the spacing in degrees of reagrd.f90:155-156,
+    given as sc.dlon/sc.dlat or recovered from dX1 = Rearth*(dx*pi/180)
+    (reagrd.f90:157-158, Rearth = 6356 km, Numeric.f90:14)."""
+    if not int(getattr(sc, "lonlat", 0)):
+        return sc.dx, sc.dy
+    m_per_deg = 6356.0e3 * (3.14159265358979323846 / 180.0)
+    dlon, dlat = getattr(sc, "dlon", 0.0), getattr(sc, "dlat", 0.0)
+    return (dlon if dlon > 0 else sc.dx / m_per_deg, dlat if dlat > 0 else sc.dy / m_per_deg)
+
+
+def output_concentration(c, hm1, zlayer, topg=None, p=None, fl=None, asl=False):
+    """C_GRND (nx,ny), CON (nx,ny,nz) and the flight-level cuts (nx,ny,nfl)
+    of printres_nc.f90 (setcut.f90, ter2asl.f90) from a state c
+    (nx+2,ny+2,nz+2,nc) in the reference's operation order."""
+    nx, ny, nz, nc = c.shape[0] - 2, c.shape[1] - 2, c.shape[2] - 2, c.shape[3]
+    cg = np.zeros((nx, ny), order="F")
+    con = np.zeros((nx, ny, nz), order="F")
+    nfl = 0 if fl is None else len(fl)
+    cfl = np.zeros((nx, ny, max(nfl, 1)), order="F")
+    fla = np.ascontiguousarray(fl if fl is not None else [0.0], dtype=np.float64)
+    lib().or_output_concentration(nx, ny, nz, nc, P(F(c)), P(F(hm1)), P(F(zlayer)),
+                                  None if topg is None else P(F(topg)),
+                                  None if p is None else P(F(p)), nfl, P(fla), P(cg), P(con),
+                                  int(bool(asl)), P(cfl))
+    return cg, con, (cfl if nfl else None)
+
+
+def standard_atm(z):
+    """(p, T) of the ICAO standard atmosphere at z m (setcut.f90:61-107)."""
+    p, T = ctypes.c_double(), ctypes.c_double()
+    lib().or_standard_atm(float(z), ctypes.byref(p), ctypes.byref(T))
+    return p.value, T.value
 
 
 class OracleRun(_RunBase):
@@ -147,9 +186,12 @@ class OracleRun(_RunBase):
         self.rkh1 = np.zeros((nx, ny, nz), order="F")
         self.vset = np.zeros((nx, ny, nz, nc), order="F")
         self.vdep = F(sc.vdep)
+        # kappa3 takes (dX1, dX2) in m, and (dx, dy) for CMAQ's Khn: degrees
+        # on LON-LAT grids (kappa3.f90:151,193-226)
+        kdx, kdy = _khn_spacing(sc)
         L.or_kappa3(sc.modkv, sc.modkh, nx, ny, nz, P(self.dzv), P(self.zl), P(self.vx),
                     P(self.vy), P(F(sc.vptemp)), P(F(sc.zi)), P(F(sc.rl)), P(F(sc.ustar)),
-                    P(self.hm1), sc.dx, sc.dy, sc.dx, sc.dy, sc.rkv0, sc.rkh0, sc.csh, lonlat,
+                    P(self.hm1), sc.dx, sc.dy, kdx, kdy, sc.rkv0, sc.rkh0, sc.csh, lonlat,
                     P(self.rkver), P(self.rkhor))
         st = L.or_setvset(nx, ny, nz, nc, P(F(sc.tempe)), P(self.rho), P(F(sc.diam)),
                           P(F(sc.rhop)), P(self.psi), P(F(sc.sphe)), sc.modv, P(self.vset),
@@ -248,6 +290,12 @@ class RefRun(_RunBase):
             L.ref_get_cwet.argtypes = [dp]
             L.ref_corvver.argtypes = [dp] * 5
             L.ref_set_clock.argtypes = [d, i, d, d]
+            L.ref_set_hm1.argtypes = [i, d, d, dp]
+            L.ref_get_vz.argtypes = [dp]
+            L.ref_get_scaled.argtypes = [dp, dp]
+            L.ref_setcut_fl.argtypes = [dp, dp, dp, d, dp]
+            L.ref_standard_atm.argtypes = [d, dp, dp]
+            L.ref_ter2asl.argtypes = [dp, dp]
             cls._L = L
         return cls._L
 
@@ -257,6 +305,11 @@ class RefRun(_RunBase):
         nx, ny, nz, nc = self.nx, self.ny, self.nz, self.nc
         L.ref_alloc(nx, ny, nz, nc, sc.nsou)
         L.ref_set_grid(sc.dx, sc.dy, P(F(sc.zlayer)))
+        lonlat = int(getattr(sc, "lonlat", 0))
+        if lonlat or getattr(sc, "hm1_cols", None) is not None:
+            self._hm1 = F(sc.hm1)
+            kdx, kdy = _khn_spacing(sc)
+            L.ref_set_hm1(lonlat, kdx, kdy, P(self._hm1))
         L.ref_set_models(sc.modkv, sc.modkh, sc.modv, sc.rkv0, sc.rkh0)
         L.ref_set_classes(P(F(sc.diam)), P(F(sc.rhop)), P(F(sc.sphe)))
         self.psi = np.zeros(nc)
@@ -289,6 +342,8 @@ class RefRun(_RunBase):
         self.vset = np.zeros((nx, ny, nz, nc), order="F")
         L.ref_get_coef(P(self.vx), P(self.rho), P(self.rkhor), P(self.rkh1), P(self.rkver),
                        P(self.vset))
+        self.vz = np.zeros_like(self.vx)
+        L.ref_get_vz(P(self.vz))
         return self.dt
 
     def step(self):

@@ -87,6 +87,32 @@ contains
     dz(nz) = dz(nz-1)
   end subroutine ref_set_grid
 
+  ! Scale factor and coordinate system.  lonlat_ = 1: a LON-LAT grid whose
+  ! spacing is dlon_ x dlat_ degrees, dX1 = Rearth*(dx*pi/180) as
+  ! reagrd.f90:155-158 computes it (kappa3's CMAQ Khn takes dx, dy in
+  ! degrees, everything else dX1, dX2); lonlat_ = 0 keeps UTM and the
+  ! spacing of ref_set_grid.  Hm1 is given per column (reagrd.f90:247-270
+  ! reads it from the grid's latitudes).
+  subroutine ref_set_hm1(lonlat_, dlon_, dlat_, hm1_) bind(C, name='ref_set_hm1')
+    integer(c_int), value :: lonlat_
+    real(c_double), value :: dlon_, dlat_
+    real(c_double) :: hm1_(nx,ny)
+    if (lonlat_ /= 0) then
+       coord_sys = 'LON-LAT'
+       dx = dlon_; dy = dlat_
+       dX1 = Rearth*(dx*pi/180.0_rp)
+       dX2 = Rearth*(dy*pi/180.0_rp)
+    else
+       coord_sys = 'UTM'
+    end if
+    Hm1 = hm1_
+  end subroutine ref_set_hm1
+
+  subroutine ref_get_scaled(dx1_, dx2_) bind(C, name='ref_get_scaled')
+    real(c_double) :: dx1_, dx2_
+    dx1_ = dX1; dx2_ = dX2
+  end subroutine ref_get_scaled
+
   subroutine ref_set_models(modkv_, modkh_, modv_, rkv0_, rkh0_) bind(C, name='ref_set_models')
     integer(c_int), value :: modkv_, modkh_, modv_
     real(c_double), value :: rkv0_, rkh0_
@@ -159,7 +185,8 @@ contains
     call setvset
   end subroutine ref_setvset
 
-  ! meteo.f90:108-112 (the three scaling statements), then finddt (meteo.f90:149)
+  ! meteo.f90:108-112 (the three scaling statements), corvver on LON-LAT
+  ! grids (meteo.f90:116-124), then finddt (meteo.f90:149)
   subroutine ref_scale_and_finddt(dt_, dtc_) bind(C, name='ref_scale_and_finddt')
     real(c_double) :: dt_, dtc_
     integer :: k
@@ -168,6 +195,7 @@ contains
        rho (1:nx,1:ny,k) = rho  (1:nx,1:ny,k)* Hm1(1:nx,1:ny)
        rkh1(1:nx,1:ny,k) = rkhor(1:nx,1:ny,k)/(Hm1(1:nx,1:ny)*Hm1(1:nx,1:ny))
     end do
+    if (coord_sys == 'LON-LAT') call corvver
     call finddt
     dt_ = dt; dtc_ = dtc
   end subroutine ref_scale_and_finddt
@@ -225,6 +253,11 @@ contains
     c = c_; cdry = cdry_
   end subroutine ref_set_state
 
+  subroutine ref_get_vz(vz_) bind(C, name='ref_get_vz')
+    real(c_double) :: vz_(nx,ny,nz)
+    vz_ = vz
+  end subroutine ref_get_vz
+
   subroutine ref_get_coef(vx_, rho_, rkhor_, rkh1_, rkver_, vset_) bind(C, name='ref_get_coef')
     real(c_double) :: vx_(nx,ny,nz), rho_(nx,ny,nz), rkhor_(nx,ny,nz), rkh1_(nx,ny,nz)
     real(c_double) :: rkver_(nx,ny,nz), vset_(nx,ny,nz,nc)
@@ -240,6 +273,32 @@ contains
     vx = vx_; vy = vy_; vz = vz_; rho = rho_; rkhor = rkhor_; rkh1 = rkh1_; rkver = rkver_
     vset = vset_; vdep = vdep_
   end subroutine ref_set_coef
+
+  ! Output-side routines of printres_nc.f90 (which itself needs netCDF):
+  ! the flight-level cut setcut_FL (printres_nc.f90:556, setcut.f90:1-57) of
+  ! var3d on the pressure p at flight level fl (m), the ICAO standard
+  ! atmosphere it takes the nominal pressure from (setcut.f90:61-107), and
+  ! ter2asl (printres_nc.f90:586, ter2asl.f90) on the module's zlayer
+  subroutine ref_setcut_fl(var3d_, p_, topg_, fl_, var2d_) bind(C, name='ref_setcut_fl')
+    real(c_double) :: var3d_(nx,ny,nz), p_(nx,ny,nz), topg_(nx,ny), var2d_(nx,ny)
+    real(c_double), value :: fl_
+    real(rp) :: flm
+    flm = fl_
+    call setcut_FL(nx,ny,nz,zlayer,var2d_,var3d_,topg_,p_,flm)
+  end subroutine ref_setcut_fl
+
+  subroutine ref_standard_atm(z_, p_, t_) bind(C, name='ref_standard_atm')
+    real(c_double), value :: z_
+    real(c_double) :: p_, t_
+    real(rp) :: z
+    z = z_
+    call standard_atm(p_, t_, z)
+  end subroutine ref_standard_atm
+
+  subroutine ref_ter2asl(conc_, topg_) bind(C, name='ref_ter2asl')
+    real(c_double) :: conc_(nx,ny,nz), topg_(nx,ny)
+    call ter2asl(conc_, topg_, zlayer, nx, ny, nz)
+  end subroutine ref_ter2asl
 
   subroutine ref_set_clock(time_, iiter_, erumass_, outmass_) bind(C, name='ref_set_clock')
     real(c_double), value :: time_, erumass_, outmass_

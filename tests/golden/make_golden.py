@@ -28,24 +28,42 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 from fall3d_amd import scenario as S  # noqa: E402
 from oracle.oracle import RefRun, REF_PATH, F, P  # noqa: E402
 
-# name -> (scenario kwargs, nsteps, zlayer(1) override)
+# name -> (scenario kwargs, nsteps, zlayer(1) override, LON-LAT grid)
+# The LON-LAT grid is (latitude of the first row, dlon, dlat) in degrees,
+# with a seeded terrain for corvver (meteo.f90:116-124).
+LL = (36.0, 0.05, 0.04)
 RUNS = {
-    "run_9x7x6x3_const_rams": (dict(seed=11), 20, None),
-    "run_9x7x6x3_cmaq_surface": (dict(seed=12, modkv=S.KV_SURFACE, modkh=S.KH_CMAQ), 20, 50.0),
-    "run_9x7x6x3_similarity": (dict(seed=13, modkv=S.KV_SIMILARITY), 3, 50.0),
-    "run_9x7x6x3_arastoopour": (dict(seed=14, modv=S.VS_ARASTOOPOUR), 5, None),
-    "run_9x7x6x3_wilson": (dict(seed=15, modv=S.VS_WILSON), 5, None),
-    "run_9x7x6x3_dellino": (dict(seed=16, modv=S.VS_DELLINO), 5, None),
-    "run_12x10x8x4_wind": (dict(nx=12, ny=10, nz=8, nc=4, seed=17, mean_u=6.0, mean_v=-5.0), 20, None),
+    "run_9x7x6x3_const_rams": (dict(seed=11), 20, None, None),
+    "run_9x7x6x3_cmaq_surface": (dict(seed=12, modkv=S.KV_SURFACE, modkh=S.KH_CMAQ), 20, 50.0, None),
+    "run_9x7x6x3_similarity": (dict(seed=13, modkv=S.KV_SIMILARITY), 3, 50.0, None),
+    "run_9x7x6x3_arastoopour": (dict(seed=14, modv=S.VS_ARASTOOPOUR), 5, None, None),
+    "run_9x7x6x3_wilson": (dict(seed=15, modv=S.VS_WILSON), 5, None, None),
+    "run_9x7x6x3_dellino": (dict(seed=16, modv=S.VS_DELLINO), 5, None, None),
+    "run_12x10x8x4_wind": (dict(nx=12, ny=10, nz=8, nc=4, seed=17, mean_u=6.0, mean_v=-5.0), 20, None, None),
+    "run_9x7x6x3_khconst": (dict(seed=18, modkh=S.KH_CONSTANT, mean_u=3.0), 20, None, None),
+    "run_12x9x6x3_lonlat_rams": (dict(nx=12, ny=9, nz=6, seed=21, mean_u=5.0, mean_v=-3.0), 20, None, LL),
+    "run_12x9x6x3_lonlat_cmaq": (dict(nx=12, ny=9, nz=6, seed=22, mean_u=-4.0, mean_v=3.0,
+                                      modkv=S.KV_SURFACE, modkh=S.KH_CMAQ), 20, 50.0, LL),
 }
 
 
-def make_run(name):
-    kw, nsteps, z1 = RUNS[name]
+def scenario_of(name):
+    kw, _, z1, ll = RUNS[name]
     sc = S.config_small(**kw)
     if z1 is not None:
         sc.zlayer[0] = z1
         sc.rl[:] = 0.001
+    if ll is not None:
+        sc = S.lonlat_grid(sc, *ll)
+        rng = np.random.default_rng(kw["seed"] + 1000)
+        sc = dataclasses.replace(sc, topg=np.asfortranarray(
+            np.asarray(rng.uniform(0.0, 1500.0, (sc.nx, sc.ny)), dtype=np.float32).astype(np.float64)))
+    return sc
+
+
+def make_run(name):
+    nsteps = RUNS[name][1]
+    sc = scenario_of(name)
     r = RefRun(sc)
     r.meteo_update()
     rec = dict(nx=sc.nx, ny=sc.ny, nz=sc.nz, nc=sc.nc, dx=sc.dx, dy=sc.dy, zlayer=sc.zlayer,
@@ -55,7 +73,8 @@ def make_run(name):
                tmrat=sc.tmrat, flowrate=sc.flowrate, modkv=sc.modkv, modkh=sc.modkh,
                modv=sc.modv, rkv0=sc.rkv0, rkh0=sc.rkh0, csh=sc.csh,
                psi=r.psi, dt=r.dt, dtc=r.dtc, vx=r.vx, rho=r.rho, rkhor=r.rkhor, rkh1=r.rkh1,
-               rkver=r.rkver, vset=r.vset, nsteps=nsteps)
+               rkver=r.rkver, vset=r.vset, nsteps=nsteps, lonlat=sc.lonlat, dlon=sc.dlon,
+               dlat=sc.dlat, hm1=sc.hm1, vz_scaled=r.vz)
     r.step()
     c1, cdry1 = r.state()
     rec["c_step1"], rec["cdry_step1"] = c1, cdry1
@@ -237,6 +256,55 @@ def make_corvver():
                         dx=5000.0, dy=4000.0, hm1=hm1, topg=topg, vx=vx, vy=vy, vz0=vz0, vz=vz)
 
 
+def make_cuts():
+    """The output-side routines of printres_nc.f90 on seeded fields: the
+    ICAO standard atmosphere, setcut_FL at flight levels from the ground to
+    above the model top (setcut.f90) of sum_ic c/Hm1, and ter2asl of
+    1e3*sum_ic c/Hm1 (ter2asl.f90) over terrain with a negative height."""
+    L = RefRun.L()
+    rng = np.random.default_rng(99)
+    nx, ny, nz, nc = 10, 8, 12, 2
+    L.ref_alloc(nx, ny, nz, nc, 1)
+    zl = F(np.concatenate([[0.0], np.cumsum(rng.uniform(600.0, 2200.0, nz - 1))]))
+    L.ref_set_grid(5000.0, 5000.0, P(zl))
+    topg = rng.uniform(0.0, 2500.0, (nx, ny))
+    topg[0, 0] = -40.0
+    topg = F(topg)
+    hm1 = F(np.repeat(np.sin(np.radians(np.linspace(40.0, 55.0, ny)))[None, :], nx, axis=0))
+    c = F(rng.uniform(0.0, 1.0, (nx + 2, ny + 2, nz + 2, nc)) ** 3)
+    w = c[1:-1, 1:-1, 1:-1, 0] + c[1:-1, 1:-1, 1:-1, 1]
+    var3d = F(w / hm1[:, :, None])
+    pstd = np.zeros(7)
+    tstd = np.zeros(7)
+    zs = np.array([0.0, 5000.0, 10999.0, 11000.0, 15000.0, 20000.0, 26000.0])
+    for n, z in enumerate(zs):
+        p_, t_ = ctypes.c_double(), ctypes.c_double()
+        L.ref_standard_atm(float(z), ctypes.byref(p_), ctypes.byref(t_))
+        pstd[n], tstd[n] = p_.value, t_.value
+    # pressure decreasing up each column: the standard atmosphere at the
+    # level's height above sea level, perturbed by under 1%
+    p = np.empty((nx, ny, nz))
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                z = zl[k] + topg[i, j]
+                p_, t_ = ctypes.c_double(), ctypes.c_double()
+                L.ref_standard_atm(float(z), ctypes.byref(p_), ctypes.byref(t_))
+                p[i, j, k] = p_.value
+    p = F(p * rng.uniform(0.995, 1.0, p.shape))
+    fl = np.array([0.0, 50.0, 100.0, 150.0, 200.0, 250.0, 300.0, 350.0, 400.0, 700.0])
+    cfl = np.zeros((nx, ny, fl.size), order="F")
+    for n, f in enumerate(fl):
+        out = F(np.zeros((nx, ny)))
+        L.ref_setcut_fl(P(var3d), P(p), P(topg), float(f * 30.48), P(out))
+        cfl[:, :, n] = 1e3 * out
+    con = F(1e3 * (w / hm1[:, :, None]))
+    L.ref_ter2asl(P(con), P(topg))
+    np.savez_compressed(os.path.join(OUT, "cuts_10x8x12x2.npz"), nx=nx, ny=ny, nz=nz, nc=nc,
+                        zlayer=zl, topg=topg, hm1=hm1, c=c, p=p, fl=fl, cfl=cfl, con_asl=con,
+                        zstd=zs, pstd=pstd, tstd=tstd)
+
+
 if __name__ == "__main__":
     if not os.path.exists(REF_PATH):
         sys.exit(f"{REF_PATH} missing: build it with `make -C oracle ref`")
@@ -250,9 +318,11 @@ if __name__ == "__main__":
             make_corvver()
         elif which == "wet":
             make_wet_run()
+        elif which == "cuts":
+            make_cuts()
         else:
             make_run(which)
     else:
-        for name in ["routines", "drydep", "corvver", "wet"] + list(RUNS):
+        for name in ["routines", "drydep", "corvver", "wet", "cuts"] + list(RUNS):
             subprocess.run([sys.executable, os.path.abspath(__file__), name], check=True)
             print("wrote", name)

@@ -28,4 +28,6 @@ def scenario_from(g):
         jsrc=g["jsrc"], ksrc=g["ksrc"], tmrat=g["tmrat"], flowrate=float(g["flowrate"]),
         modkv=int(g["modkv"]), modkh=int(g["modkh"]), modv=int(g["modv"]),
         rkv0=float(g["rkv0"]), rkh0=float(g["rkh0"]), csh=float(g["csh"]),
-        prate=g.get("prate"))
+        prate=g.get("prate"), lonlat=int(g.get("lonlat", 0)),
+        dlon=float(g.get("dlon", 0.0)), dlat=float(g.get("dlat", 0.0)),
+        hm1_cols=g["hm1"] if int(g.get("lonlat", 0)) else None)

@@ -57,6 +57,7 @@ def _call_program():
     mod, _ = module_entries()
     lines = ["program abi_check", "  use iso_c_binding", "  use fall3d_amd", "  implicit none",
              "  type(c_ptr) :: ctx, reg, batch, ctxs(1)", "  type(f3d_grid) :: grid",
+             "  type(f3d_run_info) :: info",
              "  real(c_double) :: a(8), s, t, u, v", "  integer(c_int) :: rc, i, j, k, ia(2)",
              "  integer(c_long) :: n", "  integer(c_long_long) :: b1, b2",
              "  character(len=512) :: msg",
@@ -76,6 +77,12 @@ def _call_program():
              "  rc = f3d_get_coef(ctx, a, a, a, a, a, a)", "  rc = f3d_set_landuse(ctx, a, a)",
              "  rc = f3d_set_topography(ctx, a)", "  rc = f3d_set_precipitation(ctx, a)",
              "  rc = f3d_output_products(ctx, 0, a, a, a)", "  rc = f3d_get_vdep(ctx, a)",
+             "  rc = f3d_set_pressure(ctx, a)",
+             "  rc = f3d_output_concentration(ctx, 0, a, a, 0, 1, a, a)",
+             "  rc = f3d_write_restart(ctx, 0, 'r.nc' // c_null_char, info)",
+             "  rc = f3d_read_restart(ctx, 'r.nc' // c_null_char, a, info)",
+             "  rc = f3d_restart_write('r.nc' // c_null_char, 2, 2, 2, 1, 0, a, a, info)",
+             "  rc = f3d_restart_read('r.nc' // c_null_char, 2, 2, 2, 1, a, a, info)",
              "  rc = f3d_get_clock(ctx, s, i, t, u)", "  rc = f3d_set_clock(ctx, s, 1, t, u)",
              "  rc = f3d_get_cd_state(ctx, s)", "  rc = f3d_set_cd_state(ctx, s)",
              "  rc = f3d_sync(ctx)", "  rc = f3d_set_tile(ctx, 58, 18)",

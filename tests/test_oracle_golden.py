@@ -31,6 +31,10 @@ def test_oracle_matches_reference_run(path):
     assert o.dtc == g["dtc"] or np.isnan(g["dtc"])
     for name in ("vx", "rho", "rkhor", "rkh1", "rkver", "vset"):
         assert _eq(getattr(o, name), g[name]), name
+    if "vz_scaled" in g:  # corvver on LON-LAT grids (meteo.f90:116-124)
+        assert _eq(o.vz, g["vz_scaled"])
+        if int(g["lonlat"]):
+            assert not _eq(g["vz_scaled"], g["vz"])
     o.step()
     assert _eq(o.c, g["c_step1"])
     assert _eq(o.cdry, g["cdry_step1"])
