@@ -144,7 +144,8 @@ struct FluxBatchPlan {
     TileShape ts{58, 18, 0, 2};
     DeviceBuffer par[2];   // kernel parameters per scenario, even / odd iiter
     DeviceBuffer items;    // per group: interior items, then edge items (int4 each)
-    DeviceBuffer counter;  // work counters of the four lists
+    DeviceBuffer xoff;     // per list: the offsets of its per-XCD sublists
+    DeviceBuffer counter;  // work counters of the four lists' per-XCD sublists
     int nint[2] = {0, 0}, nedge[2] = {0, 0};
     int off_int[2] = {0, 0}, off_edge[2] = {0, 0};
     int grid_int = 0, grid_edge = 0;  // persistent grid sizes (resident capacity)

@@ -998,16 +998,25 @@ namespace {
 
 __global__ void k_zero_counters(unsigned *c) { c[threadIdx.x] = 0u; }
 
+// a list of the batch plan: kXCD sublists, kXCD + 1 offsets, kXCD counters;
+// four lists (two parity groups, interior and edge)
+constexpr int kListCtrs = kXCD;
+constexpr int kBatchCounters = 4 * kListCtrs;
+
 // Persistent form of k_flux over a work list: item = (scenario, bx, by,
-// slab group).  Each workgroup draws items from an atomic counter until the
-// list is exhausted; every item writes cells no other item writes, so the
+// slab group).  The list is cut into kXCD contiguous sublists of about equal
+// cost, one per XCD (workgroup b of a launch runs on XCD b % kXCD): a
+// workgroup draws items from its own XCD's sublist through that sublist's
+// atomic counter, so the slab groups of one tile, and neighbouring tiles,
+// share that XCD's L2; once its sublist is exhausted it draws from the
+// others in turn.  Every item writes cells no other item writes, so the
 // result does not depend on which workgroup takes which item.  LIST: 0
 // interior items only, 1 edge items only, 2 both (the tile's path is picked
 // per item).
 enum : int { L_INTERIOR = 0, L_EDGE = 1, L_MIXED = 2 };
 template <int RY, int NW, int MC, bool ODD, int LIST>
 __global__ void __launch_bounds__(NW * 64)
-    k_flux_batch(const FluxParams *scen, const int4 *items, int nitems, unsigned *counter)
+    k_flux_batch(const FluxParams *scen, const int4 *items, const int *xoff, unsigned *counter)
 {
     constexpr int PW = sizeof(FluxParams) / sizeof(int);
     static_assert(sizeof(FluxParams) % sizeof(int) == 0, "parameter block of whole dwords");
@@ -1016,10 +1025,21 @@ __global__ void __launch_bounds__(NW * 64)
     __shared__ ZLev zt[kMaxLevels + 2];
     __shared__ int sp[PW];
     __shared__ int sitem[4];
+    // thread 0's place: the sublist it draws from and how many it has left
+    int x = (int)(blockIdx.x % kXCD), left = kXCD;
     for (;;) {
         if (threadIdx.x == 0) {
-            const int it = (int)atomicAdd(counter, 1u);
-            const int4 w = (it < nitems) ? items[it] : make_int4(-1, 0, 0, 0);
+            int4 w = make_int4(-1, 0, 0, 0);
+            while (left > 0) {
+                const int n = xoff[x + 1] - xoff[x];
+                const int it = (int)atomicAdd(counter + x, 1u);
+                if (it < n) {
+                    w = items[xoff[x] + it];
+                    break;
+                }
+                x = (x + 1) % kXCD;
+                --left;
+            }
             sitem[0] = w.x;
             sitem[1] = w.y;
             sitem[2] = w.z;
@@ -1027,7 +1047,7 @@ __global__ void __launch_bounds__(NW * 64)
         }
         __syncthreads();
         const int sc = __builtin_amdgcn_readfirstlane(sitem[0]);
-        if (sc < 0) return;  // list exhausted (the same value for every wave)
+        if (sc < 0) return;  // every sublist exhausted (the same value for every wave)
         const int bx = __builtin_amdgcn_readfirstlane(sitem[1]);
         const int by = __builtin_amdgcn_readfirstlane(sitem[2]);
         const int grp = __builtin_amdgcn_readfirstlane(sitem[3]);
@@ -1073,56 +1093,64 @@ int batch_grid(int &grid)
 }
 
 // parity group gr steps with parity odd: one mixed list on s, or its edge
-// list on se and its interior list on s
+// list on se and its interior list on s.  List l = 2*gr (interior or mixed)
+// or 2*gr + 1 (edge) has its kXCD + 1 sublist offsets at xoff[l*(kXCD+1)]
+// and its counters at counter[l*kXCD].
 template <int RY, int NW, int MC>
 int batch_launch(hipStream_t s, hipStream_t se, const FluxBatchPlan &b, int gr, bool odd)
 {
     constexpr int MCE = (NW * 64 > 768) ? 1 : MC;
     const FluxParams *par = b.par[odd ? 1 : 0].as<FluxParams>();
     const int4 *items = b.items.as<int4>();
-    unsigned *ctr = b.counter.as<unsigned>() + 2 * gr;
+    const int *xo = b.xoff.as<int>();
+    unsigned *ctr = b.counter.as<unsigned>();
+    const int li = 2 * gr, le = 2 * gr + 1;
     const int ne = b.nedge[gr], ni = b.nint[gr];
     if (b.mixed) {
         if (ni > 0) {
             const unsigned g = (unsigned)std::min(b.grid_int, ni);
-            const int4 *it = items + b.off_int[gr];
+            const int *x = xo + li * (kXCD + 1);
+            unsigned *c = ctr + li * kListCtrs;
             if (odd)
                 hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, true, L_MIXED>), dim3(g), dim3(NW * 64), 0, s,
-                                   par, it, ni, ctr);
+                                   par, items, x, c);
             else
                 hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, false, L_MIXED>), dim3(g), dim3(NW * 64), 0, s,
-                                   par, it, ni, ctr);
+                                   par, items, x, c);
             F3D_CHECK_LAUNCH();
         }
         return F3D_OK;
     }
     if (ne > 0) {
         const unsigned g = (unsigned)std::min(b.grid_edge, ne);
-        const int4 *it = items + b.off_edge[gr];
+        const int *x = xo + le * (kXCD + 1);
+        unsigned *c = ctr + le * kListCtrs;
         if (odd)
             hipLaunchKernelGGL((k_flux_batch<RY, NW, MCE, true, L_EDGE>), dim3(g), dim3(NW * 64), 0, se,
-                               par, it, ne, ctr + 1);
+                               par, items, x, c);
         else
             hipLaunchKernelGGL((k_flux_batch<RY, NW, MCE, false, L_EDGE>), dim3(g), dim3(NW * 64), 0, se,
-                               par, it, ne, ctr + 1);
+                               par, items, x, c);
         F3D_CHECK_LAUNCH();
     }
     if (ni > 0) {
         const unsigned g = (unsigned)std::min(b.grid_int, ni);
-        const int4 *it = items + b.off_int[gr];
+        const int *x = xo + li * (kXCD + 1);
+        unsigned *c = ctr + li * kListCtrs;
         if (odd)
             hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, true, L_INTERIOR>), dim3(g), dim3(NW * 64), 0, s,
-                               par, it, ni, ctr);
+                               par, items, x, c);
         else
             hipLaunchKernelGGL((k_flux_batch<RY, NW, MC, false, L_INTERIOR>), dim3(g), dim3(NW * 64), 0, s,
-                               par, it, ni, ctr);
+                               par, items, x, c);
         F3D_CHECK_LAUNCH();
     }
     return F3D_OK;
 }
 
 // item lists of one scenario for region rows RY: interior tiles in groups of
-// mc slabs, edge tiles in groups of mce slabs
+// mc slabs, edge tiles in groups of mce slabs; the slab groups of a tile,
+// and the tiles down a tile column, are consecutive
 void scenario_items(int sc, const Grid &g, int RY, int mc, int mce, std::vector<int4> &iint,
                     std::vector<int4> &iedge)
 {
@@ -1137,6 +1165,24 @@ void scenario_items(int sc, const Grid &g, int RY, int mc, int mce, std::vector<
             for (int gr = 0; gr < (nq + m - 1) / m; ++gr)
                 (edge ? iedge : iint).push_back(make_int4(sc, bx, by, gr));
         }
+}
+
+// Cut list [o, o + n) of all into kXCD contiguous sublists of about equal
+// cost (an item costs its scenario's nz + 2 levels); x receives the kXCD + 1
+// absolute offsets.
+static void xcd_split(const std::vector<int4> &all, int o, int n, const std::vector<StepArgs> &ae,
+                      int *x)
+{
+    double total = 0.0;
+    for (int k = o; k < o + n; ++k) total += ae[all[k].x].g.nz + 2;
+    x[0] = o;
+    double acc = 0.0;
+    int d = 1;
+    for (int k = o; k < o + n && d < kXCD; ++k) {
+        acc += ae[all[k].x].g.nz + 2;
+        while (d < kXCD && acc >= total * d / kXCD) x[d++] = k + 1;
+    }
+    while (d <= kXCD) x[d++] = o + n;
 }
 
 template <int RY, int NW, int MC>
@@ -1154,13 +1200,15 @@ int batch_plan_t(FluxBatchPlan &b, const std::vector<StepArgs> &ae, const std::v
         scenario_items((int)k, ae[k].g, RY, MC, MCE, iint[gr], iedge[gr]);
     }
     // deeper columns first (cost per item grows with nz); lists laid out as
-    // [group 0 interior | group 0 edge | group 1 interior | group 1 edge].
-    // When interior and edge tiles take the same slab count, each group is
-    // one mixed list drawn by one persistent launch: two persistent launches
-    // side by side would split the device by arrival, not by work.
+    // [group 0 interior | group 0 edge | group 1 interior | group 1 edge],
+    // each cut into per-XCD sublists.  When interior and edge tiles take the
+    // same slab count, each group is one mixed list drawn by one persistent
+    // launch: two persistent launches side by side would split the device by
+    // arrival, not by work.
     auto deeper = [&](const int4 &x, const int4 &y) { return ae[x.x].g.nz > ae[y.x].g.nz; };
     b.mixed = MCE == MC;
     std::vector<int4> all;
+    int xo[4 * (kXCD + 1)];
     for (int gr = 0; gr < 2; ++gr) {
         if (b.mixed) {
             iint[gr].insert(iint[gr].end(), iedge[gr].begin(), iedge[gr].end());
@@ -1174,15 +1222,19 @@ int batch_plan_t(FluxBatchPlan &b, const std::vector<StepArgs> &ae, const std::v
         all.insert(all.end(), iint[gr].begin(), iint[gr].end());
         b.off_edge[gr] = (int)all.size();
         all.insert(all.end(), iedge[gr].begin(), iedge[gr].end());
+        xcd_split(all, b.off_int[gr], b.nint[gr], ae, xo + (2 * gr) * (kXCD + 1));
+        xcd_split(all, b.off_edge[gr], b.nedge[gr], ae, xo + (2 * gr + 1) * (kXCD + 1));
     }
     if (b.par[0].alloc(sizeof(FluxParams) * n, nullptr) || b.par[1].alloc(sizeof(FluxParams) * n, nullptr) ||
         b.items.alloc(sizeof(int4) * std::max<size_t>(1, all.size()), nullptr) ||
-        b.counter.alloc(4 * sizeof(unsigned), nullptr))
+        b.xoff.alloc(sizeof(xo), nullptr) ||
+        b.counter.alloc(sizeof(unsigned) * kBatchCounters, nullptr))
         return F3D_ENOMEM;
     F3D_HIP(hipMemcpy(b.par[0].p, pe.data(), sizeof(FluxParams) * n, hipMemcpyHostToDevice));
     F3D_HIP(hipMemcpy(b.par[1].p, po.data(), sizeof(FluxParams) * n, hipMemcpyHostToDevice));
     if (!all.empty())
         F3D_HIP(hipMemcpy(b.items.p, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice));
+    F3D_HIP(hipMemcpy(b.xoff.p, xo, sizeof(xo), hipMemcpyHostToDevice));
     if (b.mixed) {
         TRY((batch_grid<RY, NW, MC, L_MIXED>(b.grid_int)));
         b.grid_edge = 0;
@@ -1218,7 +1270,7 @@ int flux_batch_plan(FluxBatchPlan &b, const std::vector<StepArgs> &even,
 int flux_batch_step(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join,
                     const FluxBatchPlan &b, bool odd0, bool join_side)
 {
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(4), 0, s, b.counter.as<unsigned>());
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(kBatchCounters), 0, s, b.counter.as<unsigned>());
     F3D_CHECK_LAUNCH();
     const int ne = b.nedge[0] + b.nedge[1], ni = b.nint[0] + b.nint[1];
     const bool split = side && fork && join && ni > 0 && ne > 0;
