@@ -6,7 +6,7 @@
 //   setsrc of every context,
 //   one persistent launch over the (scenario, tile, slab group) work list of
 //     the interior tiles (and one of the edge tiles on a second stream),
-//   ghost frame + outflux of every context,
+//   the outflux of every context,
 // and the host side advances each context's clock as f3d_step does
 // (Fall3d.f90:112-114).  The device work of a step is captured once per
 // iiter parity in a HIP graph and replayed.  Every work item reads and
@@ -40,8 +40,8 @@ struct f3d_batch {
     std::vector<int> group;        // 1: on the other iiter parity than context 0
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // device work of an even / odd step
     bool use_graph = true;
-    // frame arguments of every context for a step in which context 0 takes
-    // an even / odd step, and the (context, member) rows of the frame launch
+    // outflux arguments of every context for a step in which context 0 takes
+    // an even / odd step, and the (context, member) rows of the outflux launch
     DeviceBuffer fdesc[2], frows;
     int nfrows = 0;
     // source arguments of every context (even / odd as above) and the first
@@ -226,13 +226,12 @@ int record_step(f3d_batch *b, bool odd0)
 {
     hipStream_t s = b->stream;
     const int o = odd0 ? 1 : 0;
-    // the sources of every context, then their frames (c_in -> c_out's
-    // ghost frame), one launch each, then the flux lists: a persistent flux
-    // launch holds every CU until its lists are drawn, so nothing runs well
-    // beside it
+    // the sources of every context in one launch, then the flux lists (a
+    // persistent flux launch holds every CU until its lists are drawn, so
+    // nothing runs well beside it), then the outflux of every context from
+    // the ghosts the flux launch left
     TRY(f32::setsrc_batch(s, b->sdesc[o].as<f32::SrcArgs>(), b->sfirst.as<int>(), (int)b->ctx.size(),
                           b->nsrc));
-    TRY(f32::frame_batch(s, b->fdesc[o].as<f32::FrameArgs>(), b->frows.as<int2>(), b->nfrows));
     // the events bracket the persistent flux launches (the side stream's
     // list is joined back before the stop event); only when timing, which
     // launches directly (never inside a capture)
@@ -243,6 +242,7 @@ int record_step(f3d_batch *b, bool odd0)
         b->ev_next += 1;
         if (b->ev_next == (int)b->ev_stop.size()) flush_timing(b);
     }
+    TRY(f32::outflux_batch(s, b->fdesc[o].as<f32::FrameArgs>(), b->frows.as<int2>(), b->nfrows));
     return F3D_OK;
 }
 
@@ -420,7 +420,10 @@ int f3d_batch_step(f3d_batch *b, int nsteps)
         } else {
             TRY(record_step(b, odd));
         }
-        for (f3d_ctx *c : b->ctx) c->cur = 1 - c->cur;
+        for (f3d_ctx *c : b->ctx) {
+            c->cur = 1 - c->cur;
+            c->frame_dirty = true;
+        }
     }
     TRY(members_after(b));
     return F3D_OK;
@@ -466,6 +469,8 @@ int f3d_batch_autotune(f3d_batch *b, int trials, int *tx, int *ty, int *slabs, d
 {
     if (!check_batch(b)) return F3D_EINVAL;
     TRY(check_members(b));
+    // the tuning runs overwrite the ghost buffers: complete the frames first
+    for (f3d_ctx *c : b->ctx) TRY(fp32_complete_frame(c));
     TRY(after_members(b));
     std::vector<double *> scr(b->ctx.size(), nullptr);
     int rc = F3D_OK;
@@ -510,6 +515,8 @@ int f3d_autotune(f3d_ctx *c, int trials, int *tx, int *ty, int *slabs, double *b
         return F3D_ESTATE;
     }
     const Grid &g = c->g;
+    // the tuning runs overwrite the ghost buffer: complete the frame first
+    TRY(fp32_complete_frame(c));
     double *scr = nullptr;
     if (hipMalloc(&scr, sizeof(double) * (size_t)g.NXY * (g.nc + 1) * g.nm) != hipSuccess) {
         set_error("f3d_autotune: scratch allocation failed");

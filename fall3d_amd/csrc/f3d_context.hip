@@ -25,6 +25,7 @@ f32::StepArgs fp32_args(f3d_ctx *c, int cur_in, int iiter)
     f32::StepArgs a{c->g,      c->d_c32[cur_in].as<float>(), c->d_c32[1 - cur_in].as<float>(),
                     c->d_cdry.as<double>(), c->dt, iiter, c->d_zc.as<ZCoef>(), c->side,
                     c->ev_fork, c->ev_join};
+    a.gbuf = c->d_gbuf.as<float>();
     if (c->wet_on) {
         a.wet = 1;
         a.lam = c->d_lam.as<double>();
@@ -89,12 +90,27 @@ f32::FrameArgs fp32_frame_args(f3d_ctx *c, const f32::StepArgs &a)
                            c->d_scratch.as<double>(), c->d_outmass.as<double>());
 }
 
-int fp32_frame(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a)
+int fp32_outflux(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a)
 {
-    return f32::frame_and_outflux(s, a, fp32_fields(c), c->d_vx.as<double>(), c->d_vy.as<double>(),
-                                  c->d_vz.as<double>(), c->d_vset.as<double>(),
-                                  c->d_dz.as<double>(), c->d_scratch.as<double>(),
-                                  c->d_outmass.as<double>());
+    return f32::outflux(s, a, fp32_fields(c), c->d_vx.as<double>(), c->d_vy.as<double>(),
+                        c->d_vz.as<double>(), c->d_vset.as<double>(), c->d_dz.as<double>(),
+                        c->d_scratch.as<double>(), c->d_outmass.as<double>());
+}
+
+// An FP32 step leaves the x-face ghosts of the new state in the ghost
+// buffer and the rest of its ghost frame unwritten (it follows from the
+// step's input, the other state buffer, until the next step overwrites
+// it); readers of the whole state (ghost frame included) call this first.
+// Queued on the context's stream.
+int fp32_complete_frame(f3d_ctx *c)
+{
+    if (c->precision != F3D_FP32 || !c->frame_dirty) return F3D_OK;
+    const f32::Fields f = fp32_fields(c);
+    f32::FrameFill a{c->g, c->d_c32[c->cur].as<float>(), c->d_c32[1 - c->cur].as<float>(),
+                     c->d_gbuf.as<float>(), c->d_vy.as<double>(), f.bfac, f.tfac};
+    TRY(f32::frame_fill(c->stream, a));
+    c->frame_dirty = false;
+    return F3D_OK;
 }
 
 }  // namespace f3d
@@ -199,27 +215,14 @@ int step_fp32(f3d_ctx *c)
     const f32::Fields f = fp32_fields(c);
     const f32::StepArgs a = fp32_args(c, c->cur, c->iiter);
     TRY(fp32_setsrc(c, s, a));
-    // The frame reads only c_in and writes only c_out's ghost frame, which
-    // the flux kernels neither read nor write: it runs on the side stream
-    // after the edge tiles, beside the interior tiles.
-    const bool side = c->side && c->ev_fork && c->ev_join;
-    if (side) {
-        // fork here as well: the flux launch forks only when it has both
-        // interior and edge tiles
-        F3D_HIP(hipEventRecord(c->ev_fork, s));
-        F3D_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-    }
     record_start(c);
     TRY(f32::fused_step(s, a, f, c->tile));
     record_stop(c);
-    if (side) {
-        TRY(fp32_frame(c, c->side, a));
-        F3D_HIP(hipEventRecord(c->ev_join, c->side));
-        F3D_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
-    } else {
-        TRY(fp32_frame(c, s, a));
-    }
+    // the flux launch left the x-face ghosts in the ghost buffer; the
+    // outflux reads those and c_in's other faces
+    TRY(fp32_outflux(c, s, a));
     c->cur = 1 - c->cur;
+    c->frame_dirty = true;
     return F3D_OK;
 }
 
@@ -344,6 +347,7 @@ static int create_impl(f3d_ctx **out, const f3d_grid *gr, int precision, int nm)
         rc |= c->d_c32[1].alloc(sizeof(float) * g.G * g.nc * nm, B);
         rc |= c->d_f32.alloc(sizeof(float) * f32::fields_floats(g), B);
         rc |= c->d_vsum1.alloc(d8 * g.NXY * g.nc, B);
+        rc |= c->d_gbuf.alloc(sizeof(float) * f32::ghost_floats(g) * g.nc * nm, B);
     }
     if (rc) {
         f3d_destroy(c);
@@ -365,7 +369,7 @@ static int create_impl(f3d_ctx **out, const f3d_grid *gr, int precision, int nm)
     // zero state, coefficients and budgets (Master.f90:getmem1 initialises to 0)
     for (DeviceBuffer *b : {&c->d_rkhor, &c->d_rkh1, &c->d_rkver, &c->d_vset, &c->d_vdep,
                             &c->d_cdry, &c->d_cwet, &c->d_outmass, &c->d_c64, &c->d_c32[0],
-                            &c->d_c32[1], &c->d_srcoff})
+                            &c->d_c32[1], &c->d_gbuf, &c->d_srcoff})
         if (b->p) rc |= (hipMemsetAsync(b->p, 0, b->bytes, s) != hipSuccess);
     if (hipStreamSynchronize(s) != hipSuccess || rc) {
         set_error("f3d_create: initial upload failed");
@@ -799,6 +803,7 @@ int f3d_get_member_state(f3d_ctx *c, int m, double *ch, double *cdry)
         if (c->precision == F3D_FP64) {
             F3D_HIP(hipMemcpyAsync(ch, c->d_c64.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
         } else {
+            TRY(fp32_complete_frame(c));
             std::vector<float> tmp(n);
             F3D_HIP(hipMemcpyAsync(tmp.data(), c->d_c32[c->cur].as<float>() + n * m,
                                    n * sizeof(float), hipMemcpyDeviceToHost, s));
@@ -835,9 +840,14 @@ int f3d_set_state(f3d_ctx *c, const double *ch, const double *cdry)
         } else {
             std::vector<float> tmp(n);
             for (size_t i = 0; i < n; ++i) tmp[i] = (float)ch[i];
-            F3D_HIP(hipMemcpyAsync(c->d_c32[c->cur].p, tmp.data(), n * sizeof(float),
-                                   hipMemcpyHostToDevice, s));
+            // into both buffers: a step writes every frame cell of its output
+            // but the edges, which stay as set (the clip of the next state
+            // read completes them)
+            for (int b = 0; b < 2; ++b)
+                F3D_HIP(hipMemcpyAsync(c->d_c32[b].p, tmp.data(), n * sizeof(float),
+                                       hipMemcpyHostToDevice, s));
             F3D_HIP(hipStreamSynchronize(s));
+            c->frame_dirty = false;
         }
     }
     if (cdry)

@@ -111,7 +111,14 @@ struct StepArgs {
     const int *kwet = nullptr, *wetcls = nullptr;
     double *cwet = nullptr;
     const double *dz64 = nullptr;
+    // the step's x-face ghosts of c_out, clipped, per slab ghost_floats(g):
+    // [x low | x high] x nz levels x ny cells (cell j-1); the flux kernel
+    // writes them here instead of into c_out's frame (frame_fill puts them
+    // there)
+    float *gbuf = nullptr;
 };
+
+__host__ __device__ inline long ghost_floats(const Grid &g) { return 2L * g.nz * g.ny; }
 
 // wet deposition columns for the fused step: lambda = min(dt*a*P**b, 1)
 // (wetdep.f90:38-40) and the top level with zlayer(k) <= zi (0 when
@@ -159,19 +166,21 @@ int flux_batch_plan(FluxBatchPlan &b, const std::vector<StepArgs> &even,
 // joins the side stream itself)
 int flux_batch_step(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join,
                     const FluxBatchPlan &b, bool odd0, bool join_side = true);
-// Ghost frame of c_out (setbcc values of c_in, clipped) and cmass2d
-// outflux; outmass[m] += member m's outflux (deterministic per member).
-int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,
-                      const double *vy64, const double *vz64, const double *vset64,
-                      const double *dz, double *scratch, double *outmass);
+// cmass2d outflux (cmass2d.f90:38-105) of a step, after its flux launch:
+// the clipped x-face ghosts from the ghost buffer, the y-face and top ghosts
+// from c_in's faces; outmass[m] += member m's outflux (deterministic per
+// member).
+int outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,
+            const double *vy64, const double *vz64, const double *vset64, const double *dz,
+            double *scratch, double *outmass);
 // the same for many contexts in two launches: fa[k] describes context k's
-// frame step, rows[y] = (context, member) for y < nrows; each member's
-// partials and tree are the ones frame_and_outflux uses
+// step, rows[y] = (context, member) for y < nrows; each member's partials
+// and tree are the ones outflux uses
 struct FrameArgs {
     Grid g;
-    const float *cin;
-    float *cout;
-    const float *bfac, *tfac;
+    const float *gbuf;  // the step's x-face ghosts (StepArgs::gbuf)
+    const float *cin;   // its input, whose faces give the y-face and top ghosts
+    const float *tfac;  // top ghost factor per class (Fields::tfac)
     const double *vx, *vy, *vz, *vset, *dz;
     double dt;
     double *part;     // kFramePartials per member
@@ -180,7 +189,20 @@ struct FrameArgs {
 FrameArgs frame_args(const StepArgs &a, const Fields &f, const double *vx64, const double *vy64,
                      const double *vz64, const double *vset64, const double *dz, double *scratch,
                      double *outmass);
-int frame_batch(hipStream_t s, const FrameArgs *fa, const int2 *rows, int nrows);
+int outflux_batch(hipStream_t s, const FrameArgs *fa, const int2 *rows, int nrows);
+// Completes the ghost frame of a state c the flux kernel wrote from cin
+// (which must still hold that step's input): the x-face ghosts from gbuf,
+// the rest of setbcc (setbcc.f90:19-195) formed from cin's faces, clipped
+// (sizloop.f90:77-78), and the frame's edges (which setbcc never sets)
+// clipped, for every member*class slab.
+struct FrameFill {
+    Grid g;
+    float *c;
+    const float *cin, *gbuf;
+    const double *vy;
+    const float *bfac, *tfac;  // Fields::bfac, Fields::tfac
+};
+int frame_fill(hipStream_t s, const FrameFill &a);
 // setsrc for every (member, class); off[m]..off[m+1] are member m's points,
 // grp/perm group them by target cell in point order
 int setsrc(hipStream_t s, const Grid &g, float *c, const int *off, const int4 *grp, int ngrp,
@@ -264,6 +286,10 @@ struct f3d_ctx {
     f3d::DeviceBuffer d_c64;          // FP64 path
     f3d::DeviceBuffer d_c32[2];       // FP32 path ping-pong
     int cur = 0;                      // FP32: index of the current state buffer
+    // FP32: the last step's x- and y-face ghosts (f32::StepArgs::gbuf);
+    // frame_dirty while they are not yet in d_c32[cur]'s frame
+    f3d::DeviceBuffer d_gbuf;
+    bool frame_dirty = false;
     f3d::DeviceBuffer d_cdry, d_cwet;
     f3d::DeviceBuffer d_scratch, d_outmass, d_budget;
     // FP32 step fields
@@ -301,7 +327,8 @@ f32::Fields fp32_fields(const f3d_ctx *c);
 // step arguments reading state buffer cur_in and writing the other
 f32::StepArgs fp32_args(f3d_ctx *c, int cur_in, int iiter);
 int fp32_setsrc(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a);
-int fp32_frame(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a);
+int fp32_outflux(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a);
+int fp32_complete_frame(f3d_ctx *c);
 f32::FrameArgs fp32_frame_args(f3d_ctx *c, const f32::StepArgs &a);
 f32::SrcArgs fp32_src_args(f3d_ctx *c, const f32::StepArgs &a);
 }  // namespace f3d

@@ -72,6 +72,13 @@ struct FluxParams {
     const int *kwet, *wetcls;
     double *cwet;
     const double *dz64;
+    // the x-face ghosts of c_out (one column per level, a cache line per
+    // cell in c's layout) go to a compact buffer, per slab two faces (x low,
+    // x high) of nz levels of ny cells, gslab floats; the rest of the frame
+    // follows from c_in's faces, where the outflux and the frame
+    // completion read it
+    float *gbuf;
+    unsigned gslab;
     const ZLev *zl;
     double dt64;
     float dt, dtdx2, dtdy2;
@@ -207,6 +214,12 @@ enum : unsigned {
 // LDS planes of one tile: SP, SF, SQ (one per slab), SA, SY and, on odd
 // steps, the two SC planes
 constexpr int flux_planes(int mc, bool odd) { return 3 * mc + 2 + (odd ? 2 : 0); }
+// edge tiles: the x-face ghosts of every level, staged in LDS and written
+// out once per tile (a ghost column is one lane of a region row, so a store
+// per row and level would cost a memory instruction for one value):
+// [face][region row][level][slab], then [face][region row] the row's
+// byte offset in the ghost buffer (kNoStore where the tile does not own it)
+constexpr int flux_xstage(int ry, int mc) { return 2 * ry * kMaxLevels * mc + 2 * ry; }
 
 template <int RY, int NW, int MC, bool ODD, bool EX, bool EY, bool WET>
 struct FluxTile {
@@ -235,6 +248,7 @@ struct FluxTile {
     // rows only (kept in LDS rather than registers, which would keep the
     // loaded record live into a third level)
     float *SC;
+    float *SG;            // x-face ghost staging (edge tiles)
     rsrc_t rf;            // the field buffer
     const ZLev *zt;       // the z-level table, staged in LDS
     float dtop, ldz0;
@@ -299,6 +313,7 @@ struct FluxTile {
         SA = sm + 3 * MC * PL;
         SY = SA + PL;
         SC = SY + PL;
+        SG = sm + flux_planes(MC, ODD) * PL;
         const size_t G = (size_t)p.g.G, N = (size_t)p.g.N;
         rf = mkbuf(p.fbase, p.fbytes);
         const int nq = nc * p.g.nm;
@@ -346,6 +361,21 @@ struct FluxTile {
             fl[sl] = f;
             // output cells: the tile's columns, inside the domain
             so4[sl] = (li >= 3 && li < RX - 3 && (f & F_INNER)) ? co4[sl] : kNoStore;
+            // the row's place in the ghost buffer for each x face, where the
+            // tile owns the face's ghost in this row: the column beside the
+            // ghost is an output column and the row an output row
+            if (EX && li == 0) {
+                unsigned *so = reinterpret_cast<unsigned *>(SG) + 2 * RY * kMaxLevels * MC;
+#pragma unroll
+                for (int f = 0; f < 2; ++f) {
+                    const int lg = (f == 0 ? 0 : nx + 1) - gi0;  // the ghost column's lane
+                    const int la = lg + (f == 0 ? 1 : -1);
+                    const bool own = lg >= 0 && lg < RX && la >= 3 && la < RX - 3 &&
+                                     lj[sl] >= 3 && lj[sl] < RY - 3 && gj >= 1 && gj <= ny;
+                    so[f * RY + lj[sl]] = own ? 4u * ((unsigned)f * (unsigned)nz * (unsigned)ny + (unsigned)(gj - 1))
+                                              : kNoStore;
+                }
+            }
         }
     }
 
@@ -483,6 +513,16 @@ struct FluxTile {
                     if (STEADY || s + 2 <= nzl) nwz[c][sl] = bld(rf, mt4[sl], oW[c] + lz + 4u * FTL);
                 }
                 if (STEADY || s + 1 <= nzl) CAn[sl] = bld2(rf, mt8[sl], p.oA + 2u * lz);
+            }
+            // x-face ghosts (the advection left them unchanged) to the LDS
+            // stage
+#pragma unroll
+            for (int sl = 0; sl < NS; ++sl) {
+                if (EX && has(sl, F_GHOST) && has(sl, F_GX)) {
+                    float *st = SG + ((has(sl, F_GLO) ? 0 : RY) + lj[sl]) * (kMaxLevels * MC) + (s - 1) * MC;
+#pragma unroll
+                    for (int c = 0; c < MC; ++c) st[c] = P[c][sl];
+                }
             }
         }
         __syncthreads();
@@ -727,6 +767,35 @@ struct FluxTile {
         }
     }
 
+    // x-side tiles: the staged x-face ghosts of the tile's output rows,
+    // clipped, to the ghost buffer ([face][level][row] per slab: each
+    // level's rows are one run of consecutive cells)
+    __device__ __forceinline__ void flush_xstage()
+    {
+        __syncthreads();
+        const int ny = p.g.ny, ty = RY - 6;
+        const unsigned *so = reinterpret_cast<const unsigned *>(SG) + 2 * RY * kMaxLevels * MC;
+        // the ghost slabs of the group (a duplicate slab writes nothing)
+        rsrc_t rgb[MC];
+#pragma unroll
+        for (int c = 0; c < MC; ++c)
+            rgb[c] = mkbuf(p.gbuf + (size_t)(memv[c] * p.g.nc + icv[c]) * p.gslab,
+                           live[c] ? 4 * (size_t)p.gslab : 0);
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+#pragma unroll
+            for (int c = 0; c < MC; ++c) {
+                for (int t = threadIdx.x; t < nz * ty; t += NW * 64) {
+                    const int k = t / ty, r = t - k * ty;
+                    // kNoStore plus a level offset stays past the range
+                    const unsigned o = so[f * RY + 3 + r] + 4u * (unsigned)k * (unsigned)ny;
+                    const float v = SG[(f * RY + 3 + r) * (kMaxLevels * MC) + k * MC + c];
+                    bst(fmaxf(0.f, v), rgb[c], o);
+                }
+            }
+        }
+    }
+
     // Prologue, steady and epilogue levels run as three loops so each keeps
     // its own register allocation.
     __device__ __forceinline__ void run()
@@ -751,6 +820,7 @@ struct FluxTile {
             for (; s <= nzl - 3; ++s) level<true>(s);
         }
         for (; s <= nzl + 2; ++s) level<false>(s);
+        if (EX) flush_xstage();
     }
 };
 
@@ -823,7 +893,7 @@ __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
         }
     }
     const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * (RY - 6) - 3;
-    __shared__ float sm[flux_planes(MC, ODD) * RX * RY];
+    __shared__ float sm[flux_planes(MC, ODD) * RX * RY + (KIND == K_EDGE ? flux_xstage(RY, MC) : 0)];
     __shared__ ZLev zt[kMaxLevels + 2];
     for (int k = threadIdx.x; k <= p.g.nz + 1; k += NW * 64) zt[k] = p.zl[k];
     __syncthreads();
@@ -935,6 +1005,8 @@ static int make_params(const StepArgs &a, const Fields &f, FluxParams &p)
     p.wetcls = a.wetcls;
     p.cwet = a.cwet;
     p.dz64 = a.dz64;
+    p.gbuf = a.gbuf;
+    p.gslab = (unsigned)ghost_floats(a.g);
     const char *fb = reinterpret_cast<const char *>(f.base);
     auto off = [&](const void *q) {
         return (unsigned)(reinterpret_cast<const char *>(q) - fb);
@@ -1021,7 +1093,7 @@ __global__ void __launch_bounds__(NW * 64)
     constexpr int PW = sizeof(FluxParams) / sizeof(int);
     static_assert(sizeof(FluxParams) % sizeof(int) == 0, "parameter block of whole dwords");
     static_assert(PW <= NW * 64, "one thread per parameter dword");
-    __shared__ float sm[flux_planes(MC, ODD) * RX * RY];
+    __shared__ float sm[flux_planes(MC, ODD) * RX * RY + (LIST != L_INTERIOR ? flux_xstage(RY, MC) : 0)];
     __shared__ ZLev zt[kMaxLevels + 2];
     __shared__ int sp[PW];
     __shared__ int sitem[4];

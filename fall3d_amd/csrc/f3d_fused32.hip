@@ -2,45 +2,44 @@
 // the FP32 step fields built from the FP64 coefficients after a meteo read
 // (per-cell velocities as Courant numbers, the 12-byte diffusion records
 // and vz+vset per class in the flux kernel's tile-major layout, the z-level
-// table), the ghost frame of c_out with the cmass2d outflux, setsrc on the
+// table), the cmass2d outflux and the completion of the ghost frame, setsrc on the
 // FP32 state, and the FP32 <-> FP64 state conversions.  The deposit and
 // the budgets accumulate in FP64.
+#include <algorithm>
+
 #include "f3d_ctx.h"
 
 namespace f3d {
 namespace f32 {
 namespace {
 
-// ---------------- ghost frame + cmass2d outflux ----------------
-// setbcc (setbcc.f90:19-109) values of c_in written to c_out's frame, then
-// clipped; cmass2d terms (cmass2d.f90:68-127) of the clipped ghosts, FP64.
-// Block bx of nbx for member mem: a grid-stride share of the frame cells of
-// every class; the block's outflux sum goes to part[mem*nbx + bx].
-__device__ __forceinline__ void frame_block(const FrameArgs &a, int bx, int nbx, int mem)
+// ---------------- cmass2d outflux and the ghost frame ----------------
+// cmass2d terms (cmass2d.f90:38-105) of the step's clipped ghosts, FP64:
+// the x-face ghosts the flux kernel left in the ghost buffer; the y-face and
+// top ghosts as setbcc forms them (setbcc.f90:114-169) from c_in's first
+// and last rows and its top level (whole rows: coalesced).  Block bx of nbx
+// for member mem: a grid-stride share of the face cells of every class; the
+// block's outflux sum goes to part[mem*nbx + bx].
+__device__ __forceinline__ void outflux_block(const FrameArgs &a, int bx, int nbx, int mem)
 {
     __shared__ double sh[256];
     const Grid &g = a.g;
-    const float *cin = a.cin;
-    float *cout = a.cout;
     const double *vx = a.vx, *vy = a.vy, *vz = a.vz, *vset = a.vset, *dz = a.dz;
     const double dt = a.dt;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long nxf = (long)ny * nz, nyf = (long)nx * nz, nxy = g.NXY;
-    const long nedge = 4L * nz + 4L * ny + 4L * nx + 8;
-    const long per = nxf + nyf + nxy + nedge;
+    const long per = nxf + nyf + nxy;
     const long total = per * g.nc;
+    const long gf = (long)nz * ny;
     double acc = 0.0;
     for (long t = (long)bx * 256 + threadIdx.x; t < total; t += (long)nbx * 256) {
         const int q = (int)(t / per);  // class
         long r = t - (long)q * per;
-        const float *ci = cin + ((long)mem * g.nc + q) * g.G;
-        float *co = cout + ((long)mem * g.nc + q) * g.G;
+        const float *gb = a.gbuf + ((long)mem * g.nc + q) * ghost_floats(g);
         if (r < nxf) {
             const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
-            const float gl = (vx[midx(nx, ny, 1, j, k)] >= 0.0) ? 0.f : fmaxf(0.f, ci[cidx(nx, ny, 1, j, k)]);
-            const float gr = (vx[midx(nx, ny, nx, j, k)] <= 0.0) ? 0.f : fmaxf(0.f, ci[cidx(nx, ny, nx, j, k)]);
-            co[cidx(nx, ny, 0, j, k)] = gl;
-            co[cidx(nx, ny, nx + 1, j, k)] = gr;
+            const long o = (long)(k - 1) * ny + (j - 1);
+            const float gl = gb[o], gr = gb[gf + o];
             const double faty = (j == 1 || j == ny) ? 0.5 : 1.0;
             const double area = faty * g.dy * 0.5 * (dz[k] + dz[k - 1]);
             acc += -(dt * area * vx[midx(nx, ny, 1, j, k)] * (double)gl);
@@ -50,10 +49,9 @@ __device__ __forceinline__ void frame_block(const FrameArgs &a, int bx, int nbx,
         r -= nxf;
         if (r < nyf) {
             const int i = (int)(r % nx) + 1, k = (int)(r / nx) + 1;
+            const float *ci = a.cin + ((long)mem * g.nc + q) * g.G;
             const float gd = (vy[midx(nx, ny, i, 1, k)] >= 0.0) ? 0.f : fmaxf(0.f, ci[cidx(nx, ny, i, 1, k)]);
             const float gu = (vy[midx(nx, ny, i, ny, k)] <= 0.0) ? 0.f : fmaxf(0.f, ci[cidx(nx, ny, i, ny, k)]);
-            co[cidx(nx, ny, i, 0, k)] = gd;
-            co[cidx(nx, ny, i, ny + 1, k)] = gu;
             const double fatx = (i == 1 || i == nx) ? 0.5 : 1.0;
             const double area = fatx * g.dx * 0.5 * (dz[k] + dz[k - 1]);
             acc += -(dt * area * vy[midx(nx, ny, i, 1, k)] * (double)gd);
@@ -61,47 +59,14 @@ __device__ __forceinline__ void frame_block(const FrameArgs &a, int bx, int nbx,
             continue;
         }
         r -= nyf;
-        if (r < nxy) {
-            const int i = (int)(r % nx) + 1, j = (int)(r / nx) + 1;
-            const float gb = fmaxf(0.f, ci[cidx(nx, ny, i, j, 1)] * a.bfac[(long)q * nxy + r]);
-            const float gt = fmaxf(0.f, a.tfac[(long)q * nxy + r] * ci[cidx(nx, ny, i, j, nz)]);
-            co[cidx(nx, ny, i, j, 0)] = gb;
-            co[cidx(nx, ny, i, j, nz + 1)] = gt;
-            const double fatx = (i == 1 || i == nx) ? 0.5 : 1.0;
-            const double faty = (j == 1 || j == ny) ? 0.5 : 1.0;
-            const double area = fatx * faty * g.dx * g.dy;
-            const double velo = vz[midx(nx, ny, i, j, nz)] + vset[(long)q * g.N + midx(nx, ny, i, j, nz)];
-            acc += dt * area * velo * (double)gt;
-            continue;
-        }
-        r -= nxy;
-        // frame edges are never set by setbcc: carried over (clipped);
-        // corners copy the adjacent interior corner (setbcc.f90:173-195)
-        long o;
-        float v;
-        if (r < 4L * nz) {
-            const int e = (int)(r / nz), k = (int)(r % nz) + 1;
-            const int i = (e & 1) ? nx + 1 : 0, j = (e & 2) ? ny + 1 : 0;
-            o = cidx(nx, ny, i, j, k);
-            v = fmaxf(0.f, ci[o]);
-        } else if ((r -= 4L * nz) < 4L * ny) {
-            const int e = (int)(r / ny), j = (int)(r % ny) + 1;
-            const int i = (e & 1) ? nx + 1 : 0, k = (e & 2) ? nz + 1 : 0;
-            o = cidx(nx, ny, i, j, k);
-            v = fmaxf(0.f, ci[o]);
-        } else if ((r -= 4L * ny) < 4L * nx) {
-            const int e = (int)(r / nx), i = (int)(r % nx) + 1;
-            const int j = (e & 1) ? ny + 1 : 0, k = (e & 2) ? nz + 1 : 0;
-            o = cidx(nx, ny, i, j, k);
-            v = fmaxf(0.f, ci[o]);
-        } else {
-            r -= 4L * nx;
-            const int e = (int)r;
-            const int i = (e & 1) ? nx + 1 : 0, j = (e & 2) ? ny + 1 : 0, k = (e & 4) ? nz + 1 : 0;
-            o = cidx(nx, ny, i, j, k);
-            v = fmaxf(0.f, ci[cidx(nx, ny, i ? nx : 1, j ? ny : 1, k ? nz : 1)]);
-        }
-        co[o] = v;
+        const int i = (int)(r % nx) + 1, j = (int)(r / nx) + 1;
+        const float *ci = a.cin + ((long)mem * g.nc + q) * g.G;
+        const float gt = fmaxf(0.f, a.tfac[(long)q * nxy + r] * ci[cidx(nx, ny, i, j, nz)]);
+        const double fatx = (i == 1 || i == nx) ? 0.5 : 1.0;
+        const double faty = (j == 1 || j == ny) ? 0.5 : 1.0;
+        const double area = fatx * faty * g.dx * g.dy;
+        const double velo = vz[midx(nx, ny, i, j, nz)] + vset[(long)q * g.N + midx(nx, ny, i, j, nz)];
+        acc += dt * area * velo * (double)gt;
     }
     sh[threadIdx.x] = acc;
     __syncthreads();
@@ -112,16 +77,89 @@ __device__ __forceinline__ void frame_block(const FrameArgs &a, int bx, int nbx,
     if (threadIdx.x == 0) a.part[(long)mem * nbx + bx] = sh[0];
 }
 
-__global__ void __launch_bounds__(256) k_frame(FrameArgs a)
+__global__ void __launch_bounds__(256) k_outflux(FrameArgs a)
 {
-    frame_block(a, blockIdx.x, gridDim.x, blockIdx.y);  // y: member
+    outflux_block(a, blockIdx.x, gridDim.x, blockIdx.y);  // y: member
 }
 
-// the frames of many contexts: row y of the grid is (context, member)
-__global__ void __launch_bounds__(256) k_frame_batch(const FrameArgs *fa, const int2 *rows)
+// the outflux of many contexts: row y of the grid is (context, member)
+__global__ void __launch_bounds__(256) k_outflux_batch(const FrameArgs *fa, const int2 *rows)
 {
     const int2 r = rows[blockIdx.y];
-    frame_block(fa[r.x], blockIdx.x, gridDim.x, r.y);
+    outflux_block(fa[r.x], blockIdx.x, gridDim.x, r.y);
+}
+
+// Completes the ghost frame of a state c the flux kernel wrote from cin
+// (setbcc.f90:19-195 on cin, then the clip of sizloop.f90:77-78): the
+// x-face ghosts from the ghost buffer, the y-face ghosts and the bottom and
+// top planes from cin's faces and the wind and deposition factors, the
+// corners from cin's interior corners, and the edges (which setbcc never
+// sets) clipped.  Slab blockIdx.y (member*nc + class).
+__global__ void __launch_bounds__(256) k_frame_fill(FrameFill a)
+{
+    const Grid &g = a.g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long q = blockIdx.y, cls = q % g.nc, nxy = g.NXY;
+    float *cs = a.c + q * g.G;
+    const float *ci = a.cin + q * g.G;
+    const float *gb = a.gbuf + q * ghost_floats(g);
+    const long gf = (long)nz * ny;
+    const long nxf = (long)ny * nz, nyf = (long)nx * nz;
+    const long total = 2 * nxf + 2 * nyf + 2 * nxy + 4L * nz + 4L * ny + 4L * nx + 8;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        long r = t;
+        if (r < 2 * nxf) {
+            const int f = (int)(r / nxf);
+            r -= f * nxf;
+            const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
+            cs[cidx(nx, ny, f ? nx + 1 : 0, j, k)] = gb[f * gf + (long)(k - 1) * ny + (j - 1)];
+            continue;
+        }
+        r -= 2 * nxf;
+        if (r < 2 * nyf) {
+            const int f = (int)(r / nyf);
+            r -= f * nyf;
+            const int i = (int)(r % nx) + 1, k = (int)(r / nx) + 1;
+            const int jf = f ? ny : 1;
+            const double v = a.vy[midx(nx, ny, i, jf, k)];
+            const bool in = f ? v <= 0.0 : v >= 0.0;
+            cs[cidx(nx, ny, i, f ? ny + 1 : 0, k)] = in ? 0.f : fmaxf(0.f, ci[cidx(nx, ny, i, jf, k)]);
+            continue;
+        }
+        r -= 2 * nyf;
+        if (r < 2 * nxy) {
+            const int f = (int)(r / nxy);
+            r -= f * nxy;
+            const int i = (int)(r % nx) + 1, j = (int)(r / nx) + 1;
+            cs[cidx(nx, ny, i, j, f ? nz + 1 : 0)] =
+                f ? fmaxf(0.f, a.tfac[cls * nxy + r] * ci[cidx(nx, ny, i, j, nz)])
+                  : fmaxf(0.f, ci[cidx(nx, ny, i, j, 1)] * a.bfac[cls * nxy + r]);
+            continue;
+        }
+        r -= 2 * nxy;
+        long o;
+        float v;
+        if (r < 4L * nz) {
+            const int e = (int)(r / nz), k = (int)(r % nz) + 1;
+            o = cidx(nx, ny, (e & 1) ? nx + 1 : 0, (e & 2) ? ny + 1 : 0, k);
+            v = cs[o];
+        } else if ((r -= 4L * nz) < 4L * ny) {
+            const int e = (int)(r / ny), j = (int)(r % ny) + 1;
+            o = cidx(nx, ny, (e & 1) ? nx + 1 : 0, j, (e & 2) ? nz + 1 : 0);
+            v = cs[o];
+        } else if ((r -= 4L * ny) < 4L * nx) {
+            const int e = (int)(r / nx), i = (int)(r % nx) + 1;
+            o = cidx(nx, ny, i, (e & 1) ? ny + 1 : 0, (e & 2) ? nz + 1 : 0);
+            v = cs[o];
+        } else {
+            // corners copy the adjacent interior corner (setbcc.f90:173-195)
+            const int e = (int)(r - 4L * nx);
+            const int i = (e & 1) ? nx + 1 : 0, j = (e & 2) ? ny + 1 : 0, k = (e & 4) ? nz + 1 : 0;
+            o = cidx(nx, ny, i, j, k);
+            v = ci[cidx(nx, ny, i ? nx : 1, j ? ny : 1, k ? nz : 1)];
+        }
+        cs[o] = fmaxf(0.f, v);
+    }
 }
 
 // outmass[member] += the sum of its partials, for many contexts (the tree
@@ -405,9 +443,8 @@ FrameArgs frame_args(const StepArgs &a, const Fields &f, const double *vx64, con
 {
     FrameArgs fa;
     fa.g = a.g;
+    fa.gbuf = a.gbuf;
     fa.cin = a.cin;
-    fa.cout = a.cout;
-    fa.bfac = f.bfac;
     fa.tfac = f.tfac;
     fa.vx = vx64;
     fa.vy = vy64;
@@ -420,20 +457,30 @@ FrameArgs frame_args(const StepArgs &a, const Fields &f, const double *vx64, con
     return fa;
 }
 
-int frame_and_outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,
-                      const double *vy64, const double *vz64, const double *vset64,
-                      const double *dz, double *scratch, double *outmass)
+int outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx64,
+            const double *vy64, const double *vz64, const double *vset64, const double *dz,
+            double *scratch, double *outmass)
 {
     const FrameArgs fa = frame_args(a, f, vx64, vy64, vz64, vset64, dz, scratch, outmass);
-    hipLaunchKernelGGL(k_frame, dim3(red::kFramePartials, a.g.nm), dim3(256), 0, s, fa);
+    hipLaunchKernelGGL(k_outflux, dim3(red::kFramePartials, a.g.nm), dim3(256), 0, s, fa);
     F3D_CHECK_LAUNCH();
     return red::finish_rows(s, scratch, red::kFramePartials, a.g.nm, outmass, 1);
 }
 
-int frame_batch(hipStream_t s, const FrameArgs *fa, const int2 *rows, int nrows)
+int frame_fill(hipStream_t s, const FrameFill &a)
+{
+    const Grid &g = a.g;
+    const long total = 2L * g.nz * (g.nx + g.ny) + 2L * g.NXY + 4L * (g.nx + g.ny + g.nz) + 8;
+    const unsigned nb = (unsigned)std::min<long>((total + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_frame_fill, dim3(nb, (unsigned)(g.nc * g.nm)), dim3(256), 0, s, a);
+    F3D_CHECK_LAUNCH();
+    return F3D_OK;
+}
+
+int outflux_batch(hipStream_t s, const FrameArgs *fa, const int2 *rows, int nrows)
 {
     if (nrows <= 0) return F3D_OK;
-    hipLaunchKernelGGL(k_frame_batch, dim3(red::kFramePartials, nrows), dim3(256), 0, s, fa, rows);
+    hipLaunchKernelGGL(k_outflux_batch, dim3(red::kFramePartials, nrows), dim3(256), 0, s, fa, rows);
     F3D_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_finish_batch, dim3(nrows), dim3(1024), 0, s, fa, rows);
     F3D_CHECK_LAUNCH();
