@@ -43,6 +43,7 @@ UNITS = {
     "f3d_batch.hip": EXACT,
     "f3d_noalloc.hip": EXACT,
     "f3d_output.hip": EXACT,
+    "f3d_gravity.hip": EXACT,
 }
 
 

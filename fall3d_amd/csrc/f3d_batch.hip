@@ -80,9 +80,10 @@ void drop_graphs(f3d_batch *b)
         }
 }
 
-// every context has its meteorology and no wet deposition (the batched
-// kernel has no wet instantiation: a context that set precipitation after
-// joining the batch is refused here rather than stepped without it)
+// every context has its meteorology, no wet deposition (the batched kernel
+// has no wet instantiation: a context that set precipitation after joining
+// the batch is refused here rather than stepped without it) and no gravity
+// current (its divcorr belongs to f3d_step's end of step)
 int check_members(const f3d_batch *b)
 {
     for (size_t k = 0; k < b->ctx.size(); ++k) {
@@ -93,6 +94,11 @@ int check_members(const f3d_batch *b)
         }
         if (c->wet_on) {
             set_error("batch: context %zu has wet deposition, which runs through f3d_step only", k);
+            return F3D_EINVAL;
+        }
+        if (c->gravity_on) {
+            set_error("batch: context %zu has the gravity current (divcorr at the log times), which "
+                      "runs through f3d_step only", k);
             return F3D_EINVAL;
         }
     }
@@ -317,6 +323,11 @@ int f3d_batch_create(f3d_batch **out, f3d_ctx *const *ctxs, int n)
         if (ctxs[k] && ctxs[k]->wet_on) {
             set_error("f3d_batch_create: context %d has wet deposition, which runs through "
                       "f3d_step only", k);
+            return F3D_EINVAL;
+        }
+        if (ctxs[k] && ctxs[k]->gravity_on) {
+            set_error("f3d_batch_create: context %d has the gravity current (divcorr at the log "
+                      "times), which runs through f3d_step only", k);
             return F3D_EINVAL;
         }
         if (!ctxs[k] || ctxs[k]->precision != F3D_FP32) {

@@ -514,6 +514,8 @@ static int meteo_stage(f3d_ctx *c)
     if (c->lonlat && c->topg_set)
         TRY(met::corvver(s, g, c->d_hm1.as<double>(), c->d_topg.as<double>(), g.dx, g.dy,
                          c->d_vx.as<double>(), c->d_vy.as<double>(), c->d_vz.as<double>()));
+    // meteo.f90:138-145 the gravity current's radial wind and setdivu
+    TRY(gravity_meteo(c));
     // meteo.f90:149 finddt
     F3D_HIP(hipMemsetAsync(c->d_dtcn.p, 0, sizeof(unsigned long long), s));
     TRY(met::finddt(s, g, c->d_dz.as<double>(), g.dx, g.dy, c->d_vx.as<double>(),
@@ -735,6 +737,8 @@ int f3d_step(f3d_ctx *c, int nsteps)
         c->iiter = c->iiter + 1;
         for (int m = 0; m < c->g.nm; ++m) c->erumass[m] = c->erumass[m] + c->flowrate[m] * c->dt;
         TRY(c->precision == F3D_FP64 ? step_fp64(c) : step_fp32(c));
+        // endstep.f90:71
+        TRY(gravity_endstep(c));
     }
     return F3D_OK;
 }

@@ -1,6 +1,7 @@
 // Device-resident context behind the f3d_* C ABI.
 #pragma once
 
+#include <array>
 #include <vector>
 
 #include "f3d_internal.h"
@@ -313,6 +314,16 @@ struct f3d_ctx {
     bool meteo_ready = false;
     double cd_state = 1.0;
 
+    // gravity current (f3d_set_gravity_current): parameters, source
+    // heights, the wind divergence of the last meteo read (setdivu), the
+    // radial wind's scalars of that read and the last divcorr factor
+    bool gravity_on = false, grav_active = false;
+    f3d_gravity grav{};
+    std::vector<double> grav_zs;
+    f3d::DeviceBuffer d_divu;
+    std::array<double, 9> grav_diag{};
+    double grav_corrdiv = 0.0;
+
     // timing of the dominant step kernel
     bool timing = false;
     std::vector<hipEvent_t> ev_start, ev_stop;
@@ -328,6 +339,11 @@ f32::Fields fp32_fields(const f3d_ctx *c);
 f32::StepArgs fp32_args(f3d_ctx *c, int cur_in, int iiter);
 int fp32_setsrc(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a);
 int fp32_outflux(f3d_ctx *c, hipStream_t s, const f32::StepArgs &a);
+// gravity current: the radial wind and the divergence of a meteo read
+// (after the Hm1 scaling and corvver), and divcorr after a step at the log
+// times
+int gravity_meteo(f3d_ctx *c);
+int gravity_endstep(f3d_ctx *c);
 int fp32_complete_frame(f3d_ctx *c);
 f32::FrameArgs fp32_frame_args(f3d_ctx *c, const f32::StepArgs &a);
 f32::SrcArgs fp32_src_args(f3d_ctx *c, const f32::StepArgs &a);

@@ -35,6 +35,15 @@ module fall3d_amd
      real(c_double) :: erumass = 0.0_c_double, outmass = 0.0_c_double
   end type f3d_run_info
 
+  ! the GRAVITY_CURRENT block (readat.f90:149-200), in the grid's coordinates
+  type, bind(C), public :: f3d_gravity
+     real(c_double) :: c_flow_rate = 0.0_c_double, lambda_grav = 0.0_c_double
+     real(c_double) :: k_entrain = 0.0_c_double, brunt_vaisala = 0.0_c_double
+     real(c_double) :: center_x = 0.0_c_double, center_y = 0.0_c_double
+     real(c_double) :: xorigr = 0.0_c_double, yorigr = 0.0_c_double
+     real(c_double) :: dx = 0.0_c_double, dy = 0.0_c_double, beg_time = 0.0_c_double
+  end type f3d_gravity
+
   public :: f3d_create, f3d_destroy, f3d_set_classes, f3d_set_models, f3d_set_meteo, &
             f3d_get_dt, f3d_set_dt, f3d_set_source, f3d_step, f3d_budget, f3d_get_state, &
             f3d_set_state, f3d_get_coef, f3d_sync, f3d_get_clock, f3d_set_clock, &
@@ -42,7 +51,8 @@ module fall3d_amd
   ! meteo-stage extensions and output products
   public :: f3d_set_landuse, f3d_set_topography, f3d_set_precipitation, &
             f3d_output_products, f3d_get_vdep, f3d_set_pressure, f3d_output_concentration, &
-            f3d_write_restart, f3d_read_restart, f3d_restart_write, f3d_restart_read
+            f3d_write_restart, f3d_read_restart, f3d_restart_write, f3d_restart_read, &
+            f3d_set_gravity_current, f3d_get_gravity_current, f3d_get_divu
   ! ensembles and resident meteorology (no counterpart in the reference)
   public :: f3d_create_ensemble, f3d_set_member_source, f3d_ensemble_budget, &
             f3d_get_member_state, f3d_set_meteo_window
@@ -106,6 +116,26 @@ module fall3d_amd
      end function f3d_set_topography
 
      ! prate (Master, mm/h): wet deposition in later f3d_step calls (not in batches)
+     ! gc: the parameters, or c_null_ptr to turn the gravity current off
+     integer(c_int) function f3d_set_gravity_current(ctx, gc, nsou, zs) &
+          bind(C, name='f3d_set_gravity_current')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, gc
+       integer(c_int), value :: nsou
+       real(c_double), intent(in) :: zs(*)
+     end function f3d_set_gravity_current
+     integer(c_int) function f3d_get_gravity_current(ctx, diag, corrdiv, active) &
+          bind(C, name='f3d_get_gravity_current')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: diag(9), corrdiv
+       integer(c_int), intent(out) :: active
+     end function f3d_get_gravity_current
+     integer(c_int) function f3d_get_divu(ctx, divu) bind(C, name='f3d_get_divu')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: divu(*)
+     end function f3d_get_divu
      integer(c_int) function f3d_set_precipitation(ctx, prate) bind(C, name='f3d_set_precipitation')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

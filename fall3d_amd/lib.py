@@ -39,11 +39,19 @@ CTX_SYMBOLS = ["f3d_create", "f3d_create_ensemble", "f3d_set_member_source",
                "f3d_batch_set_tile", "f3d_batch_get_tile", "f3d_batch_use_graph",
                "f3d_batch_step", "f3d_batch_sync", "f3d_batch_autotune", "f3d_autotune",
                "f3d_set_meteo_window", "f3d_mem_info", "f3d_cmass3d_noalloc",
-               "f3d_batch_timing", "f3d_batch_kernel_time"]
+               "f3d_batch_timing", "f3d_batch_kernel_time", "f3d_set_gravity_current",
+               "f3d_get_gravity_current", "f3d_get_divu"]
 
 
 class F3DError(RuntimeError):
     pass
+
+
+class _Gravity(ctypes.Structure):
+    """f3d_gravity: the GRAVITY_CURRENT block (readat.f90:149-200)."""
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("c_flow_rate", "lambda_grav", "k_entrain", "brunt_vaisala", "center_x",
+                 "center_y", "xorigr", "yorigr", "dx", "dy", "beg_time")]
 
 
 class _RunInfo(ctypes.Structure):
@@ -97,6 +105,9 @@ def lib():
         L.f3d_get_vdep.argtypes = [vp, dp]
         L.f3d_set_topography.argtypes = [vp, dp]
         L.f3d_set_precipitation.argtypes = [vp, dp]
+        L.f3d_set_gravity_current.argtypes = [vp, ctypes.POINTER(_Gravity), i, dp]
+        L.f3d_get_gravity_current.argtypes = [vp, dp, dp, ip]
+        L.f3d_get_divu.argtypes = [vp, dp]
         L.f3d_output_products.argtypes = [vp, i, dp, dp, dp]
         L.f3d_set_pressure.argtypes = [vp, dp]
         L.f3d_output_concentration.argtypes = [vp, i, dp, dp, i, i, dp, dp]
@@ -142,6 +153,7 @@ def lib():
                      "f3d_get_dt", "f3d_set_dt", "f3d_set_source", "f3d_step", "f3d_budget",
                      "f3d_get_state", "f3d_set_state", "f3d_get_coef", "f3d_set_landuse",
                      "f3d_get_vdep", "f3d_set_topography", "f3d_set_precipitation",
+                     "f3d_set_gravity_current", "f3d_get_gravity_current", "f3d_get_divu",
                      "f3d_output_products", "f3d_set_pressure", "f3d_output_concentration",
                      "f3d_write_restart", "f3d_read_restart", "f3d_restart_write",
                      "f3d_restart_read",
@@ -355,6 +367,32 @@ class Context:
         self._prate = _F(prate)
         _check(lib().f3d_set_precipitation(self.h, _P(self._prate)), "f3d_set_precipitation")
 
+    def set_gravity_current(self, zs, **prm):
+        """Gravity current on (prm: c_flow_rate, lambda_grav, k_entrain,
+        brunt_vaisala, center_x, center_y, xorigr, yorigr, dx, dy, beg_time;
+        zs: the source heights) or off (zs None)."""
+        if zs is None:
+            _check(lib().f3d_set_gravity_current(self.h, None, 0, None), "f3d_set_gravity_current")
+            return
+        gc = _Gravity(*[float(prm[n]) for n, _ in _Gravity._fields_])
+        self._zs = np.ascontiguousarray(zs, dtype=np.float64)
+        _check(lib().f3d_set_gravity_current(self.h, ctypes.byref(gc), int(self._zs.size),
+                                             _P(self._zs)), "f3d_set_gravity_current")
+
+    def gravity_state(self):
+        """(diag[9], corrdiv, active) of the last meteo read and divcorr."""
+        diag = np.zeros(9)
+        cd, act = ctypes.c_double(), ctypes.c_int()
+        _check(lib().f3d_get_gravity_current(self.h, _P(diag), ctypes.byref(cd), ctypes.byref(act)),
+               "f3d_get_gravity_current")
+        return diag, cd.value, bool(act.value)
+
+    def divu(self):
+        """The wind divergence (nx,ny,nz) of setdivu."""
+        out = np.zeros((self.nx, self.ny, self.nz), order="F")
+        _check(lib().f3d_get_divu(self.h, _P(out)), "f3d_get_divu")
+        return out
+
     def products(self, member=0):
         """Output products of a member: load (nx,ny,0:nc) kg/m2, deposit
         thickness (nx,ny) cm, z-cumulative concentration (nx,ny) g/m2."""
@@ -473,6 +511,8 @@ class Run:
         self.ctx.set_source(sc.isrc, sc.jsrc, sc.ksrc, sc.tmrat, sc.flowrate)
         if getattr(sc, "prate", None) is not None:
             self.ctx.set_precipitation(sc.prate)
+        if getattr(sc, "gravity", None) is not None:
+            self.ctx.set_gravity_current(sc.zs, **sc.gravity)
         self.ctx.cd_state = cd_state
         self.nsteps = 0
 

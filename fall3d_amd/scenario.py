@@ -88,6 +88,13 @@ class Scenario:
     dlat: float = 0.0
     # precipitation rate (nx,ny) in mm/h: wet deposition on (wetdep.f90)
     prate: Optional[np.ndarray] = None
+    # gravity current (the GRAVITY_CURRENT block, readat.f90:149-200): the
+    # fields of f3d_gravity (c_flow_rate, lambda_grav, k_entrain,
+    # brunt_vaisala, center_x, center_y, xorigr, yorigr, dx, dy, beg_time;
+    # coordinates in the grid's own units) and the source heights zs
+    # (nsou,) in m; None: off
+    gravity: Optional[dict] = None
+    zs: Optional[np.ndarray] = None
 
     @property
     def dz(self) -> np.ndarray:
@@ -370,6 +377,23 @@ def config_small(nx=9, ny=7, nz=6, nc=3, seed=1, modkv=KV_CONSTANT, modkh=KH_RAM
     return _finish(f"small_{nx}x{ny}x{nz}x{nc}_s{seed}", nx, ny, nz, dx, dy, zl, classes,
                    u, v, w, topg, src, 1e5, 3600.0, modkv=modkv, modkh=modkh, modv=modv,
                    rkv0=50.0, rkh0=2000.0)
+
+
+def with_gravity_current(sc: Scenario, mer=1e7, H=None, beg_time=-1800.0, c_flow_rate=430.0,
+                         lambda_grav=0.2, k_entrain=0.1, brunt_vaisala=0.02):
+    """The scenario with a Suzuki column of mass rate mer at the domain
+    centre and the gravity current of its umbrella on (UTM grid, origin 0:
+    the reference places node i at xorigr + i*dx, addradialwind.f90:159).
+    The source heights zs are the levels of the source nodes."""
+    H = sc.zlayer[-1] * 0.6 if H is None else H
+    x0, y0 = (sc.nx - 1) * sc.dx / 2, (sc.ny - 1) * sc.dy / 2
+    src = suzuki_source(sc.nx, sc.ny, sc.dx, sc.dy, sc.zlayer, x0, y0, H, mer, 4.0, 5.0, sc.fc)
+    zs = np.asarray(sc.zlayer)[src[2] - 1].astype(np.float64)
+    grav = dict(c_flow_rate=c_flow_rate, lambda_grav=lambda_grav, k_entrain=k_entrain,
+                brunt_vaisala=brunt_vaisala, center_x=x0, center_y=y0, xorigr=0.0, yorigr=0.0,
+                dx=sc.dx, dy=sc.dy, beg_time=beg_time)
+    return dataclasses.replace(sc, name=f"{sc.name}_gc", isrc=src[0], jsrc=src[1], ksrc=src[2],
+                               tmrat=src[3], flowrate=float(mer), gravity=grav, zs=zs)
 
 
 def config_ensemble_member(base: Scenario, seed: int):

@@ -234,6 +234,43 @@ int f3d_set_pressure(f3d_ctx *ctx, const double *p);
 int f3d_output_concentration(f3d_ctx *ctx, int member, double *cgrnd, double *con, int asl,
                              int nfl, const double *fl, double *cfl);
 
+/* Gravity current of the umbrella cloud: the GRAVITY_CURRENT block of the
+ * input file (readat.f90:149-200, Master.f90:69-78).  Coordinates are the
+ * grid's own, as XORIGR/YORIGR and the spacing dx, dy are (degrees on
+ * LON-LAT grids, m on UTM grids); center is RADIAL_WIND_LON/LAT (LON-LAT) or
+ * RADIAL_WIND_X/Y (UTM); beg_time is the eruption start in s, on the clock
+ * of f3d_set_clock. */
+typedef struct {
+    double c_flow_rate, lambda_grav, k_entrain, brunt_vaisala;
+    double center_x, center_y;
+    double xorigr, yorigr, dx, dy;
+    double beg_time;
+} f3d_gravity;
+/* Turns the gravity current on (gc non-null) or off (gc NULL) for a
+ * single-member context; zs (nsou) are the source heights of the current
+ * source (Master zs), of which the reference takes maxval(zs) as the column
+ * top: call again when the source changes.  Every later meteo read adds the
+ * radial wind after the Hm1 scaling and corvver (addradialwind.f90, called
+ * at meteo.f90:140; not while the source's total rate sum(tmrat) is at most
+ * 0.1) and forms the wind divergence (setdivu.f90, meteo.f90:145), before
+ * finddt; every f3d_step that ends at a log time (the first step, and
+ * whenever mod(time, 600) <= dt, endstep.f90:53) applies divcorr
+ * (divcorr.f90, endstep.f90:71) on that step's budgets.  The reference
+ * clamps the crater node (i_wind, j_wind) on the low side only; the
+ * library also clamps it to nx, ny.  Replaces (has no counterpart for)
+ * nothing: the reference's three routines take module state, so they have
+ * no argument-level drop-in. */
+int f3d_set_gravity_current(f3d_ctx *ctx, const f3d_gravity *gc, int nsou, const double *zs);
+/* The radial wind's scalars of the last meteo read, diag[9] = {min_radius,
+ * max_radius, radius_grav, c_grav_factor, kmin, kmax, i_wind, j_wind,
+ * k_wind} (addradialwind.f90:52-155), the factor corrdiv of the last
+ * divcorr, and whether the last meteo read added the wind; any may be NULL. */
+int f3d_get_gravity_current(f3d_ctx *ctx, double *diag, double *corrdiv, int *active);
+/* The wind divergence divu (nx,ny,nz) of setdivu.f90:16-30 (Master divu)
+ * on the current scaled wind: formed at every meteo read while the gravity
+ * current is on, else on this call. */
+int f3d_get_divu(f3d_ctx *ctx, double *divu);
+
 /* Run date, extent and clock recorded in a restart file (the global
  * attributes of wrirst.f90:120-172). */
 typedef struct {
@@ -308,7 +345,10 @@ const char *f3d_last_error(void);
  * sources and budgets (f3d_budget, f3d_get_state work on it as usual); a
  * context's result in a batch equals the one it gets from f3d_step with the
  * same tile shape.  The contexts must stay alive while the batch exists;
- * they may be on different iiter parities (each keeps its own sweep order). */
+ * they may be on different iiter parities (each keeps its own sweep order).
+ * Contexts with wet deposition or the gravity current step through f3d_step
+ * only: f3d_batch_create refuses them, and f3d_batch_step refuses a context
+ * that turned either on after joining. */
 typedef struct f3d_batch f3d_batch;
 int f3d_batch_create(f3d_batch **batch, f3d_ctx *const *ctxs, int n);
 void f3d_batch_destroy(f3d_batch *batch);

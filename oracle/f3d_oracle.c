@@ -1090,3 +1090,132 @@ void or_sizloop(double *c, double *cdry, int nx, int ny, int nz, int nc, int iit
     }
     or_cmass2d(c, vx, vy, vz, vset, nx, ny, nz, nc, dX1, dX2, dz, dt, outmass);
 }
+
+/* ---------------- gravity current (addradialwind, setdivu, divcorr) ---------------- */
+
+/* setdivu.f90:16-30: the divergence of the (scaled) wind field; one-sided at
+ * the first column, row and level, where the reference takes v(max(1,i-1)) */
+void or_setdivu(int nx, int ny, int nz, const double *vx, const double *vy, const double *vz,
+                const double *Hm1, double dX1, double dX2, const double *dz, double *divu)
+{
+    for (int i = 1; i <= nx; ++i)
+        for (int j = 1; j <= ny; ++j)
+            for (int k = 1; k <= nz; ++k) {
+                const double vi = vx[MI(i, j, k)], vi_1 = vx[MI(imax(1, i - 1), j, k)];
+                const double vj = vy[MI(i, j, k)], vj_1 = vy[MI(i, imax(1, j - 1), k)];
+                const double vk = vz[MI(i, j, k)], vk_1 = vz[MI(i, j, imax(1, k - 1))];
+                divu[MI(i, j, k)] = (vi - vi_1) / (Hm1[PI2(i, j)] * dX1) + (vj - vj_1) / dX2 +
+                                    (vk - vk_1) / dz[k];
+            }
+}
+
+/* addradialwind.f90:47-173: the zero-divergence radial wind of the umbrella
+ * cloud added to the scaled vx, vy between the levels nearest hmin and hmax.
+ * prm = {c_flow_rate, lambda_grav, k_entrain, brunt_vaisala, centre x,
+ * centre y, xorigr, yorigr, dx, dy} in the grid's coordinates (dx, dy:
+ * degrees on LON-LAT grids, m on UTM grids); time_eruption = time -
+ * beg_time; mass_flow_rate = sum(tmrat); h_tot = maxval(zs).  Returns 0
+ * when the source is off (mass flow rate <= 0.1, the single-precision
+ * literal of the reference), 1 after adding the wind; out (optional) =
+ * {min_radius, max_radius, radius_grav, c_grav_factor, kmin, kmax, i_wind,
+ * j_wind, k_wind}. */
+int or_addradialwind(int nx, int ny, int nz, double *vx, double *vy, const double *Hm1,
+                     double dX1, double dX2, const double *zlayer, const double *prm,
+                     double time_eruption, double mass_flow_rate, double h_tot, double *out)
+{
+    const double pi = 3.14159265358979323846;
+    const double c_flow_rate = prm[0], lambda_grav = prm[1], k_entrain = prm[2],
+                 brunt_vaisala = prm[3], cx = prm[4], cy = prm[5], xorigr = prm[6],
+                 yorigr = prm[7], dx = prm[8], dy = prm[9];
+    const double h11 = Hm1[PI2(1, 1)];
+    const double min_radius = 1.5 * sqrt(h11 * dX1 * h11 * dX1 + dX2 * dX2);
+    if (mass_flow_rate <= (double)0.1f) return 0;
+    const double vol_flow_rate =
+        c_flow_rate * sqrt(k_entrain) * pow(mass_flow_rate, 0.75) / pow(brunt_vaisala, 0.625);
+    const double c_gravr = 3.0 * lambda_grav * brunt_vaisala * vol_flow_rate / (2.0 * pi);
+    double radius_grav = pow(c_gravr * (time_eruption * time_eruption), 0.33333);
+    radius_grav = fmax_f(radius_grav, min_radius);
+    const double c_gravu = sqrt(2.0 * lambda_grav * brunt_vaisala * vol_flow_rate / (3.0 * pi));
+    const double c_grav_factor = 0.75 * c_gravu * sqrt(radius_grav);
+    const double th_grav = c_gravu / (sqrt(radius_grav) * lambda_grav * brunt_vaisala);
+    const double h_umbr = h_tot / 1.32;
+    const double hmin = fmax_f(h_umbr - 0.5 * th_grav, 0.5 * h_tot);
+    const double hmax = fmin_f(h_umbr + 0.5 * th_grav, h_tot);
+    /* integer conversion truncates toward zero; the reference clamps the
+     * low side only */
+    int i_wind = (int)((cx - xorigr) / dx), j_wind = (int)((cy - yorigr) / dy);
+    i_wind = imax(1, i_wind);
+    j_wind = imax(1, j_wind);
+    int k_wind = 1;
+    double store = zlayer[nz - 1];
+    for (int k = 1; k <= nz; ++k) {
+        const double dist = fabs(zlayer[k - 1] - h_umbr);
+        if (dist <= store) {
+            k_wind = k;
+            store = dist;
+        }
+    }
+    const double ux = vx[MI(i_wind, j_wind, k_wind)], uy = vy[MI(i_wind, j_wind, k_wind)];
+    const double v_wind = sqrt(ux * ux + uy * uy);
+    const double max_radius = 1.7777 * c_gravr / fmax_f(v_wind * v_wind, 0.1);
+    int kmin = 1, kmax = nz;
+    store = zlayer[nz - 1];
+    for (int k = 1; k <= nz; ++k) {
+        const double dist = fabs(zlayer[k - 1] - hmin);
+        if (dist <= store) {
+            kmin = k;
+            store = dist;
+        }
+    }
+    store = zlayer[nz - 1];
+    for (int k = 1; k <= nz; ++k) {
+        const double dist = fabs(zlayer[k - 1] - hmax);
+        if (dist <= store) {
+            kmax = k;
+            store = dist;
+        }
+    }
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            const double h = Hm1[PI2(i, j)];
+            const double deltax = dX1 * h * ((xorigr + i * dx) - cx) / dx;
+            const double deltay = dX2 * ((yorigr + j * dy) - cy) / dy;
+            const double radius = sqrt(deltax * deltax + deltay * deltay);
+            if (fabs(radius) <= min_radius) continue;
+            if (fabs(radius) > max_radius) continue;
+            const double rvel =
+                c_grav_factor / radius * (1.0 + radius * radius / (3.0 * (radius_grav * radius_grav)));
+            for (int k = kmin; k <= kmax; ++k) {
+                vx[MI(i, j, k)] = vx[MI(i, j, k)] + rvel * deltax / radius / h;
+                vy[MI(i, j, k)] = vy[MI(i, j, k)] + rvel * deltay / radius;
+            }
+        }
+    if (out) {
+        const double o[9] = {min_radius, max_radius, radius_grav, c_grav_factor, kmin, kmax,
+                             i_wind, j_wind, k_wind};
+        for (int n = 0; n < 9; ++n) out[n] = o[n];
+    }
+    return 1;
+}
+
+/* divcorr.f90:22-38 (endstep.f90:71, at the log times after writim): the
+ * relative mass imbalance (erumass - massvol - massdep - outmass)/erumass,
+ * clamped to +-0.05, scales every class in the cells whose |divu*dt| >=
+ * 0.005.  Returns the factor corrdiv. */
+double or_divcorr(double *c, int nx, int ny, int nz, int nc, const double *divu, double dt,
+                  double erumass, double massvol, double massdep, double outmass)
+{
+    const long G = (long)(nx + 2) * (ny + 2) * (nz + 2);
+    const double compmass = massvol + massdep + outmass;
+    const double eps = 0.005, eps1 = 10.0 * eps;
+    const double corrdiv = fmax_f(fmin_f((erumass - compmass) / erumass, eps1), -eps1);
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const double dive = divu[MI(i, j, k)];
+                if (fabs(dive * dt) >= eps)
+                    for (int ic = 0; ic < nc; ++ic)
+                        c[G * ic + CI(i, j, k)] = c[G * ic + CI(i, j, k)] * (1.0 + corrdiv);
+            }
+    return corrdiv;
+}

@@ -106,6 +106,18 @@ void or_sizloop(double *c, double *cdry, int nx, int ny, int nz, int nc, int iit
                 double dX2, double *outmass, int nthreads, double *cwet, const double *diam,
                 const double *zlayer, const double *zi, const double *prate);
 
+/* Gravity current: setdivu.f90:16-30; addradialwind.f90:47-173 (prm =
+ * {c_flow_rate, lambda_grav, k_entrain, brunt_vaisala, centre x, centre y,
+ * xorigr, yorigr, dx, dy}; returns 0 when the source is off, out: 9
+ * diagnostics or null); divcorr.f90:22-38 (returns corrdiv). */
+void or_setdivu(int nx, int ny, int nz, const double *vx, const double *vy, const double *vz,
+                const double *Hm1, double dX1, double dX2, const double *dz, double *divu);
+int or_addradialwind(int nx, int ny, int nz, double *vx, double *vy, const double *Hm1,
+                     double dX1, double dX2, const double *zlayer, const double *prm,
+                     double time_eruption, double mass_flow_rate, double h_tot, double *out);
+double or_divcorr(double *c, int nx, int ny, int nz, int nc, const double *divu, double dt,
+                  double erumass, double massvol, double massdep, double outmass);
+
 #ifdef __cplusplus
 }
 #endif

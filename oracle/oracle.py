@@ -77,12 +77,38 @@ def lib():
         L.or_standard_atm.argtypes = [d, dp, dp]
         L.or_output_concentration.argtypes = [i, i, i, i, dp, dp, dp, dp, dp, i, dp, dp, dp, i,
                                               dp]
+        L.or_setdivu.argtypes = [i, i, i, dp, dp, dp, dp, d, d, dp, dp]
+        L.or_addradialwind.restype = i
+        L.or_addradialwind.argtypes = [i, i, i, dp, dp, dp, d, d, dp, dp, d, d, d, dp]
+        L.or_divcorr.restype = d
+        L.or_divcorr.argtypes = [dp, i, i, i, i, dp, d, d, d, d, d]
         _lib = L
     return _lib
 
 
 def F(a, dtype=np.float64):
     return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def mass_flow_rate(tmrat):
+    """sum(tmrat(1:nsou,1:nc)) in array order (addradialwind.f90:70): one
+    running sum down the columns, as the reference's SUM forms it."""
+    s = 0.0
+    for v in np.asarray(tmrat, dtype=np.float64).ravel(order="F"):
+        s = s + float(v)
+    return s
+
+
+def gravity_prm(g):
+    """The ten parameters of or_addradialwind from a scenario's gravity dict."""
+    return np.array([g["c_flow_rate"], g["lambda_grav"], g["k_entrain"], g["brunt_vaisala"],
+                     g["center_x"], g["center_y"], g["xorigr"], g["yorigr"], g["dx"], g["dy"]])
+
+
+def log_time(iiter, time, dt):
+    """endstep.f90:53: the log (writim, divcorr) runs after the first step
+    and whenever mod(time, 600) <= dt."""
+    return iiter == 1 or np.fmod(time, 600.0) <= dt
 
 
 class _RunBase:
@@ -203,6 +229,18 @@ class OracleRun(_RunBase):
         if lonlat:
             L.or_corvver(nx, ny, nz, P(self.hm1), P(F(sc.topg)), sc.dx, sc.dy, P(self.vx),
                          P(self.vy), P(self.vz))
+        if getattr(sc, "gravity", None) is not None:
+            # meteo.f90:140,145: the radial wind, then the divergence
+            self.vy = F(sc.vy).copy(order="F")
+            g = sc.gravity
+            self.gdiag = np.zeros(9)
+            self.gactive = L.or_addradialwind(
+                nx, ny, nz, P(self.vx), P(self.vy), P(self.hm1), sc.dx, sc.dy, P(self.zl),
+                P(gravity_prm(g)), self.time - g["beg_time"], mass_flow_rate(sc.tmrat),
+                float(np.max(sc.zs)), P(self.gdiag))
+            self.divu = np.zeros((nx, ny, nz), order="F")
+            L.or_setdivu(nx, ny, nz, P(self.vx), P(self.vy), P(self.vz), P(self.hm1), sc.dx,
+                         sc.dy, P(self.dzv), P(self.divu))
         dtc = ctypes.c_double()
         self.dt = L.or_finddt(nx, ny, nz, nc, P(self.dzv), sc.dx, sc.dy, P(self.vx),
                               P(self.vy), P(self.vz), P(self.rkh1), P(self.rkhor),
@@ -226,6 +264,12 @@ class OracleRun(_RunBase):
                      P(self._diam), P(self.zl), P(self._zi),
                      None if self._prate is None else P(self._prate))
         self.outmass = om.value
+        if getattr(sc, "gravity", None) is not None and log_time(self.iiter, self.time, self.dt):
+            # endstep.f90:53-71: writim's cmass3d, then divcorr
+            b = self.budget()
+            self.corrdiv = L.or_divcorr(P(self.c), self.nx, self.ny, self.nz, self.nc,
+                                        P(self.divu), self.dt, self.erumass, b["massvol"],
+                                        b["massdep"], self.outmass)
 
     def products(self):
         """Output products (printres_nc.f90:150-241): load (nx,ny,0:nc),
@@ -296,6 +340,8 @@ class RefRun(_RunBase):
             L.ref_setcut_fl.argtypes = [dp, dp, dp, d, dp]
             L.ref_standard_atm.argtypes = [d, dp, dp]
             L.ref_ter2asl.argtypes = [dp, dp]
+            L.ref_set_gravity.argtypes = [dp, d, dp]
+            L.ref_get_wind.argtypes = [dp, dp, dp]
             cls._L = L
         return cls._L
 
@@ -325,6 +371,11 @@ class RefRun(_RunBase):
         if getattr(sc, "prate", None) is not None:
             self._prate = F(sc.prate)
             L.ref_set_wetdep(P(self._prate))
+        self._gravity = getattr(sc, "gravity", None) is not None
+        if self._gravity:
+            self._gprm = np.ascontiguousarray(gravity_prm(sc.gravity)[:8])
+            self._zs = np.ascontiguousarray(sc.zs, dtype=np.float64)
+            L.ref_set_gravity(P(self._gprm), sc.gravity["beg_time"], P(self._zs))
 
     def meteo_update(self):
         L = self.L()
@@ -344,6 +395,10 @@ class RefRun(_RunBase):
                        P(self.vset))
         self.vz = np.zeros_like(self.vx)
         L.ref_get_vz(P(self.vz))
+        self.vy = np.zeros_like(self.vx)
+        self.divu = np.zeros_like(self.vx)
+        vxw = np.zeros_like(self.vx)
+        L.ref_get_wind(P(vxw), P(self.vy), P(self.divu))
         return self.dt
 
     def step(self):
@@ -351,6 +406,10 @@ class RefRun(_RunBase):
         L.ref_step()
         self.time += self.dt
         self.iiter += 1
+        if self._gravity and log_time(self.iiter, self.time, self.dt):
+            # endstep.f90:53-71: writim's cmass3d, then divcorr
+            self.budget()
+            L.ref_divcorr()
 
     def state(self):
         L = self.L()

@@ -14,7 +14,9 @@
 !   meteo.f90:108-112    Hm1 scaling of vx, rho and rkh1 (three array
 !                        statements; meteo itself cannot be linked because
 !                        it calls the netCDF reader reamet)
+!   meteo.f90:140,145    addradialwind (gravity current on), setdivu
 !   meteo.f90:149        finddt
+!   endstep.f90:71       divcorr (gravity current on, at the log times)
 !   endstep.f90:80-106   getc0 (restated: endstep.f90 cannot be linked
 !                        because it calls the netCDF writer printres_nc)
 !   writim.f90:18        cmass3d
@@ -26,6 +28,7 @@ module ref_harness
   use Master
   use Parallel, only: parallel_startup, parallel_setgroups
   use Domain,   only: ic_loc, nc_loc, iz_loc, nz_loc
+  use InpOut,   only: nlst
   implicit none
   logical, save :: started = .false.
 contains
@@ -59,6 +62,8 @@ contains
     allocate(cdry(nx,ny,0:nc), cwet(nx,ny,0:nc), cload(nx,ny,0:nc))
     allocate(work(max(nx+1,ny+1),2))
     allocate(isrc(max(nsou,1)), jsrc(max(nsou,1)), ksrc(max(nsou,1)), tmrat(max(nsou,1),nc))
+    allocate(zs(max(nsou,1)), divu(nx,ny,nz))
+    zs = 0.0_rp; divu = 0.0_rp
     c = 0.0_rp; cdry = 0.0_rp; cwet = 0.0_rp; cload = 0.0_rp
     vset = 0.0_rp; vdep = 0.0_rp; work = 0.0_rp
     rkhor = 0.0_rp; rkh1 = 0.0_rp; rkver = 0.0_rp
@@ -70,6 +75,7 @@ contains
   subroutine ref_free() bind(C, name='ref_free')
     deallocate(Hm1, zlayer, dz, topg, vx, vy, vz, tempe, vptemp, rho, rkhor, rkh1, rkver)
     deallocate(ustar, zi, rl, vset, vdep, c, cdry, cwet, cload, work, isrc, jsrc, ksrc, tmrat)
+    deallocate(zs, divu)
   end subroutine ref_free
 
   ! Grid: uniform horizontal spacing (UTM, Hm1 = 1), zlayer(1:nz); dz follows
@@ -196,9 +202,42 @@ contains
        rkh1(1:nx,1:ny,k) = rkhor(1:nx,1:ny,k)/(Hm1(1:nx,1:ny)*Hm1(1:nx,1:ny))
     end do
     if (coord_sys == 'LON-LAT') call corvver
+    if (gravity_current) call addradialwind
+    call setdivu
     call finddt
     dt_ = dt; dtc_ = dtc
   end subroutine ref_scale_and_finddt
+
+  ! The GRAVITY_CURRENT block (readat.f90:149-200): prm = {c_flow_rate,
+  ! lambda_grav, k_entrain, brunt_vaisala, centre x, centre y, xorigr,
+  ! yorigr} (the centre is RADIAL_WIND_LON/LAT on LON-LAT grids, else
+  ! RADIAL_WIND_X/Y), beg_time and the source heights zs(nsou).  The
+  ! routine's log lines (addradialwind.f90:88-96) go to /dev/null.
+  subroutine ref_set_gravity(prm, beg_time_, zs_) bind(C, name='ref_set_gravity')
+    real(c_double) :: prm(8), zs_(nsou)
+    real(c_double), value :: beg_time_
+    c_flow_rate = prm(1); lambda_grav = prm(2); k_entrain = prm(3); brunt_vaisala = prm(4)
+    if (coord_sys == 'LON-LAT') then
+       radial_wind_lon = prm(5); radial_wind_lat = prm(6)
+    else
+       radial_wind_x = prm(5); radial_wind_y = prm(6)
+    end if
+    xorigr = prm(7); yorigr = prm(8)
+    beg_time = beg_time_
+    zs(1:nsou) = zs_(1:nsou)
+    open(unit=nlst, file='/dev/null', status='old', action='write')
+    gravity_current = .true.
+  end subroutine ref_set_gravity
+
+  ! endstep.f90:71 after writim's cmass3d (ref_budget)
+  subroutine ref_divcorr() bind(C, name='ref_divcorr')
+    call divcorr
+  end subroutine ref_divcorr
+
+  subroutine ref_get_wind(vx_, vy_, divu_) bind(C, name='ref_get_wind')
+    real(c_double) :: vx_(nx,ny,nz), vy_(nx,ny,nz), divu_(nx,ny,nz)
+    vx_ = vx; vy_ = vy; divu_ = divu
+  end subroutine ref_get_wind
 
   subroutine ref_set_dt(dt_) bind(C, name='ref_set_dt')
     real(c_double), value :: dt_

@@ -119,6 +119,65 @@ def make_wet_run():
     np.savez_compressed(os.path.join(OUT, "run_10x8x7x3_wetdep.npz"), **rec)
 
 
+# the gravity current (addradialwind, setdivu, divcorr) on: name ->
+# (config_small kwargs, with_gravity_current kwargs, nsteps, LON-LAT grid).
+# A base wind without divergence keeps divcorr's factor inside its clamp of
+# +-0.05; the Fourier-mode wind drives it onto the clamp.
+GRAVITY_RUNS = {
+    "run_12x10x8x3_gravity": (dict(nx=12, ny=10, nz=8, nc=3, seed=23, rms_h=0.5, rms_v=0.0,
+                                   mean_u=1.0, mean_v=0.5), dict(), 30, None),
+    "run_12x10x8x3_gravity_clamped": (dict(nx=12, ny=10, nz=8, nc=3, seed=24, rms_h=3.0,
+                                           mean_u=1.0, mean_v=0.5), dict(c_flow_rate=43.0), 20, None),
+    "run_12x10x8x3_gravity_lonlat": (dict(nx=12, ny=10, nz=8, nc=3, seed=25, rms_h=0.5, rms_v=0.0,
+                                          mean_u=-1.0, mean_v=0.5), dict(), 24, (36.0, 0.04, 0.03)),
+}
+
+
+def gravity_scenario(name):
+    kw, gkw, _, ll = GRAVITY_RUNS[name]
+    sc = S.config_small(**kw)
+    if ll is not None:
+        sc = S.lonlat_grid(sc, *ll)
+    sc = S.with_gravity_current(sc, **gkw)
+    if ll is not None:
+        # the grid's own coordinates: degrees from a western edge at 15 E
+        lon0, lat0 = 15.0, ll[0]
+        g = dict(sc.gravity, dx=sc.dlon, dy=sc.dlat, xorigr=lon0, yorigr=lat0,
+                 center_x=lon0 + (sc.nx - 1) / 2 * sc.dlon, center_y=lat0 + (sc.ny - 1) / 2 * sc.dlat)
+        sc = dataclasses.replace(sc, gravity=g)
+    return sc
+
+
+def make_gravity_run(name):
+    nsteps = GRAVITY_RUNS[name][2]
+    sc = gravity_scenario(name)
+    r = RefRun(sc)
+    r.meteo_update()
+    g = sc.gravity
+    rec = dict(nx=sc.nx, ny=sc.ny, nz=sc.nz, nc=sc.nc, dx=sc.dx, dy=sc.dy, zlayer=sc.zlayer,
+               diam=sc.diam, rhop=sc.rhop, sphe=sc.sphe, vx_in=sc.vx, vy_in=sc.vy, vz=sc.vz,
+               tempe=sc.tempe, vptemp=sc.vptemp, rho_in=sc.rho, ustar=sc.ustar, zi=sc.zi,
+               rl=sc.rl, topg=sc.topg, vdep=sc.vdep, isrc=sc.isrc, jsrc=sc.jsrc, ksrc=sc.ksrc,
+               tmrat=sc.tmrat, flowrate=sc.flowrate, modkv=sc.modkv, modkh=sc.modkh,
+               modv=sc.modv, rkv0=sc.rkv0, rkh0=sc.rkh0, csh=sc.csh, lonlat=sc.lonlat,
+               dlon=sc.dlon, dlat=sc.dlat, hm1=sc.hm1, zs=sc.zs,
+               gravity=np.array([g[k] for k in GRAVITY_KEYS]),
+               psi=r.psi, dt=r.dt, dtc=r.dtc, vx=r.vx, vy=r.vy, divu=r.divu, vz_scaled=r.vz,
+               rho=r.rho, rkhor=r.rkhor, rkh1=r.rkh1, rkver=r.rkver, vset=r.vset, nsteps=nsteps)
+    r.step()
+    rec["c_step1"], rec["cdry_step1"] = r.state()
+    for _ in range(nsteps - 1):
+        r.step()
+    rec["c_final"], rec["cdry_final"] = r.state()
+    for k, v in r.budget().items():
+        rec["budget_" + k] = v
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **rec)
+
+
+GRAVITY_KEYS = ("c_flow_rate", "lambda_grav", "k_entrain", "brunt_vaisala", "center_x", "center_y",
+                "xorigr", "yorigr", "dx", "dy", "beg_time")
+
+
 def make_routines():
     """Per-routine vectors: each reference external called once on seeded
     inputs with mixed-sign velocities (the serial decomposition)."""
@@ -320,9 +379,11 @@ if __name__ == "__main__":
             make_wet_run()
         elif which == "cuts":
             make_cuts()
+        elif which in GRAVITY_RUNS:
+            make_gravity_run(which)
         else:
             make_run(which)
     else:
-        for name in ["routines", "drydep", "corvver", "wet", "cuts"] + list(RUNS):
+        for name in ["routines", "drydep", "corvver", "wet", "cuts"] + list(RUNS) + list(GRAVITY_RUNS):
             subprocess.run([sys.executable, os.path.abspath(__file__), name], check=True)
             print("wrote", name)

@@ -57,8 +57,8 @@ def _call_program():
     mod, _ = module_entries()
     lines = ["program abi_check", "  use iso_c_binding", "  use fall3d_amd", "  implicit none",
              "  type(c_ptr) :: ctx, reg, batch, ctxs(1)", "  type(f3d_grid) :: grid",
-             "  type(f3d_run_info) :: info",
-             "  real(c_double) :: a(8), s, t, u, v", "  integer(c_int) :: rc, i, j, k, ia(2)",
+             "  type(f3d_run_info) :: info", "  type(f3d_gravity), target :: gc",
+             "  real(c_double) :: a(9), s, t, u, v", "  integer(c_int) :: rc, i, j, k, ia(2)",
              "  integer(c_long) :: n", "  integer(c_long_long) :: b1, b2",
              "  character(len=512) :: msg",
              "  if (command_argument_count() < 99) stop 0",
@@ -76,6 +76,8 @@ def _call_program():
              "  rc = f3d_get_member_state(ctx, 0, a, a)", "  rc = f3d_set_state(ctx, a, a)",
              "  rc = f3d_get_coef(ctx, a, a, a, a, a, a)", "  rc = f3d_set_landuse(ctx, a, a)",
              "  rc = f3d_set_topography(ctx, a)", "  rc = f3d_set_precipitation(ctx, a)",
+             "  rc = f3d_set_gravity_current(ctx, c_loc(gc), 1, a)",
+             "  rc = f3d_get_gravity_current(ctx, a, s, i)", "  rc = f3d_get_divu(ctx, a)",
              "  rc = f3d_output_products(ctx, 0, a, a, a)", "  rc = f3d_get_vdep(ctx, a)",
              "  rc = f3d_set_pressure(ctx, a)",
              "  rc = f3d_output_concentration(ctx, 0, a, a, 0, 1, a, a)",

@@ -82,7 +82,13 @@ def test_reference_sizloop_drives_library_dropins(fixture, tmp_path):
     g = load(fixture)
     assert got["dt"] == g["dt"]
     for key in ("c_step1", "c_final", "cdry_final"):
-        assert np.array_equal(got[key], g[key], equal_nan=True), key
+        if "gravity" in g and key != "c_step1":
+            # divcorr scales by 1 + corrdiv formed from writim's cmass3d,
+            # here the library's cmass3d_ (tree-summed, within 1e-13 of the
+            # reference's sum): the state follows to that bar
+            assert np.abs(got[key] - g[key]).max() <= 1e-13 * np.abs(g[key]).max(), key
+        else:
+            assert np.array_equal(got[key], g[key], equal_nan=True), key
     if "cwet_final" in g:
         # the scavenging coefficient dt*a*P**b (wetdep.f90:38) takes the
         # device pow, which may differ from the host libm's in the last bit:

@@ -111,6 +111,8 @@ def test_fp64_context_vs_golden(path):
             assert np.array_equal(co[name], g[name], equal_nan=True), name
         assert rel_linf(co["vset"], g["vset"]) <= 1e-12
         assert abs(r.dt - float(g["dt"])) <= 1e-12 * float(g["dt"])
+        if "gravity" in g:  # the radial wind (vx above) and setdivu on the device
+            assert np.array_equal(r.ctx.divu(), g["divu"])
         r.step(1)
         c, cdry = r.state()
         assert rel_linf(c, g["c_step1"]) <= 1e-11

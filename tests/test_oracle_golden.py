@@ -33,8 +33,12 @@ def test_oracle_matches_reference_run(path):
         assert _eq(getattr(o, name), g[name]), name
     if "vz_scaled" in g:  # corvver on LON-LAT grids (meteo.f90:116-124)
         assert _eq(o.vz, g["vz_scaled"])
-        if int(g["lonlat"]):
+        if int(g["lonlat"]) and "gravity" not in g:
             assert not _eq(g["vz_scaled"], g["vz"])
+    if "gravity" in g:  # addradialwind and setdivu (meteo.f90:140,145)
+        assert o.gactive == 1
+        assert _eq(o.vy, g["vy"]) and _eq(o.divu, g["divu"])
+        assert not _eq(g["vy"], g["vy_in"])  # the radial wind reached some columns
     o.step()
     assert _eq(o.c, g["c_step1"])
     assert _eq(o.cdry, g["cdry_step1"])
