@@ -221,10 +221,15 @@ constexpr int flux_planes(int mc, bool odd) { return 3 * mc + 2 + (odd ? 2 : 0);
 // byte offset in the ghost buffer (kNoStore where the tile does not own it)
 constexpr int flux_xstage(int ry, int mc) { return 2 * ry * kMaxLevels * mc + 2 * ry; }
 
-template <int RY, int NW, int MC, bool ODD, bool EX, bool EY, bool WET>
+template <int RY, int NW, int MC, bool ODD, bool EX, bool EY, bool WET, bool SCL = false>
 struct FluxTile {
     static constexpr int NS = RY / NW;
     static constexpr bool EDGE = EX || EY;
+    // SCL: the group's two slabs are two members of one class (ensembles
+    // with an even member count take slabs class-major): they share the
+    // class's vz+vset loads, its ghost factors and advctzc's face factors
+    static constexpr bool SC2 = SCL && MC == 2;
+    static constexpr int cv(int c) { return SC2 ? 0 : c; }
     // groups of at most 768 lanes (a budget of 168 registers) run the
     // steady levels in pairs, prefetch c a level early and store each
     // level's outputs after its loads, interior and edge tiles alike; the
@@ -316,13 +321,18 @@ struct FluxTile {
         SG = sm + flux_planes(MC, ODD) * PL;
         const size_t G = (size_t)p.g.G, N = (size_t)p.g.N;
         rf = mkbuf(p.fbase, p.fbytes);
-        const int nq = nc * p.g.nm;
+        const int nm = p.g.nm, nq = nc * nm;
 #pragma unroll
         for (int c = 0; c < MC; ++c) {
             live[c] = qv[c] < nq;
-            const int q = live[c] ? qv[c] : qv[0];
-            icv[c] = q % nc;
-            memv[c] = q / nc;
+            // slabs are taken class-major (slab s = class*nm + member; with
+            // one member s is the class), so the slabs of a group and the
+            // groups of a tile that run at once are members of few classes;
+            // the state and deposits stay member-major (q)
+            const int s = live[c] ? qv[c] : qv[0];
+            icv[c] = s / nm;
+            memv[c] = s - icv[c] * nm;
+            const int q = memv[c] * nc + icv[c];
             rin[c] = mkbuf(p.cin + (size_t)q * G, 4 * G);
             // the duplicate slab of an odd class count writes nothing
             rout[c] = mkbuf(p.cout + (size_t)q * G, live[c] ? 4 * G : 0);
@@ -400,15 +410,21 @@ struct FluxTile {
             h2[sl] = bld(rf, m4[sl], p.oH);
             if (WET) kwl[sl] = p.kwet[m4[sl] / 4u];
             gv[sl] = EDGE ? bld(rf, mt8[sl] + (has(sl, F_GX) ? 0u : 4u), p.oA) : 0.f;
+            float bfv[MC], v1v[MC];
+#pragma unroll
+            for (int c = 0; c < MC; ++c) {
+                bfv[c] = c == cv(c) ? bld(rf, m4[sl], oBF[c]) : bfv[cv(c)];
+                v1v[c] = c == cv(c) ? bld(rf, mt4[sl], oW[c]) : v1v[cv(c)];
+            }
 #pragma unroll
             for (int c = 0; c < MC; ++c) {
                 const float c1 = bld(rin[c], co4[sl], 4 * SZ);
                 const float c2 = bld(rin[c], co4[sl], 8 * SZ);
-                const float gb = c1 * bld(rf, m4[sl], oBF[c]);  // c(0), setbcc.f90:108-113
-                const float v1 = bld(rf, mt4[sl], oW[c]);
+                const float gb = c1 * bfv[c];  // c(0), setbcc.f90:108-113
+                const float v1 = v1v[c];
                 c0[c][sl] = c1;
                 cp1[c][sl] = c2;
-                vk[c][sl] = v1;
+                if (c == cv(c)) vk[c][sl] = v1;
                 const float d1 = (c2 - c1) * z1.dzk;
                 const float d0 = (c1 - gb) * ldz0;
                 const float av = fabsf(v1);
@@ -417,7 +433,7 @@ struct FluxTile {
                 dprev[c][sl] = d1;
                 gprev[c][sl] = gb;
                 nc2[c][sl] = (3 <= nz) ? bld(rin[c], co4[sl], 12 * SZ) : 0.f;
-                nwz[c][sl] = bld(rf, mt4[sl], oW[c] + 4u * FTL);
+                if (c == cv(c)) nwz[c][sl] = bld(rf, mt4[sl], oW[c] + 4u * FTL);
                 b0[c][sl] = q0[c][sl] = 0.f;
                 dc[c][sl] = qc[c][sl] = 0.f;
                 P[c][sl] = phi[c][sl] = 0.f;
@@ -464,31 +480,34 @@ struct FluxTile {
 #pragma unroll
                 for (int c = 0; c < MC; ++c) {
                     float cp2 = 0.f, vkn = 0.f, W1n, W2n, dn = 0.f;
+                    const float vkc = vk[cv(c)][sl];
                     if (STEADY || s < nzl) {
                         // top ghost c(nz+1) = tfac*c(nz) (setbcc.f90:126-130)
                         cp2 = (STEADY || s + 2 <= nzl) ? nc2[c][sl]
-                                                       : bld(rf, m4[sl], oTF[c]) * cp1[c][sl];
-                        vkn = nwz[c][sl];
-                        W1n = zn.dtdzz * (vkn * cp1[c][sl] - vk[c][sl] * c0[c][sl]);
+                                                       : bld(rf, m4[sl], oTF[cv(c)]) * cp1[c][sl];
+                        vkn = nwz[cv(c)][sl];
+                        W1n = zn.dtdzz * (vkn * cp1[c][sl] - vkc * c0[c][sl]);
                         dn = (cp2 - cp1[c][sl]) * zn.dzk;
-                        const float av = 0.5f * fabsf(vkn + vk[c][sl]);
+                        const float av = 0.5f * fabsf(vkn + vkc);
                         W2n = minmod(dn, dprev[c][sl]) * av * (zn.a2 - zn.a2bc * av);
                     } else {
-                        W1n = dtop * vk[c][sl] * (cp1[c][sl] - c0[c][sl]);  // advctzc.f90:175-181
+                        W1n = dtop * vkc * (cp1[c][sl] - c0[c][sl]);  // advctzc.f90:175-181
                         W2n = W2k[c][sl];
                     }
                     const float Un = W1n + (W2n - W2k[c][sl]);
-                    float a = c0[c][sl] - ((vk[c][sl] > 0.f) ? Uc[c][sl] : Un);
+                    float a = c0[c][sl] - ((vkc > 0.f) ? Uc[c][sl] : Un);
                     if (EDGE) a = inner(sl) ? a : (has(sl, F_GHOST) && !zero ? c0[c][sl] : 0.f);
                     c0[c][sl] = cp1[c][sl];
                     cp1[c][sl] = cp2;
-                    vk[c][sl] = vkn;
+                    if (!SC2) vk[c][sl] = vkn;
                     Uc[c][sl] = Un;
                     W2k[c][sl] = W2n;
                     dprev[c][sl] = dn;
                     if (!ODD) a = adv_x<EX>(a, Ax, Fx, inner(sl), has(sl, F_XLO), has(sl, F_XHI));
                     P[c][sl] = a;
                 }
+                // a shared class window moves once both slabs have used it
+                if (SC2) vk[0][sl] = (STEADY || s < nzl) ? nwz[0][sl] : 0.f;
             }
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl)
@@ -510,7 +529,8 @@ struct FluxTile {
                         nc2[c][sl] = nc3[c][sl];
                     else if (STEADY || s + 3 <= nzl)
                         nc2[c][sl] = bldnt(rin[c], co4[sl], 4u * SZ * (unsigned)(s + 3));
-                    if (STEADY || s + 2 <= nzl) nwz[c][sl] = bld(rf, mt4[sl], oW[c] + lz + 4u * FTL);
+                    if ((STEADY || s + 2 <= nzl) && c == cv(c))
+                        nwz[c][sl] = bld(rf, mt4[sl], oW[c] + lz + 4u * FTL);
                 }
                 if (STEADY || s + 1 <= nzl) CAn[sl] = bld2(rf, mt8[sl], p.oA + 2u * lz);
             }
@@ -824,11 +844,11 @@ struct FluxTile {
     }
 };
 
-template <int RY, int NW, int MC, bool ODD, bool EX, bool EY, bool WET = false>
+template <int RY, int NW, int MC, bool ODD, bool EX, bool EY, bool WET = false, bool SCL = false>
 __device__ __forceinline__ void flux_tile(const FluxParams &p, const int *qv, const int gi0,
                                           const int gj0, float *sm, const ZLev *zt)
 {
-    FluxTile<RY, NW, MC, ODD, EX, EY, WET> t(p, qv, gi0, gj0, sm, zt);
+    FluxTile<RY, NW, MC, ODD, EX, EY, WET, SCL> t(p, qv, gi0, gj0, sm, zt);
     t.run();
 }
 
@@ -853,7 +873,7 @@ __device__ __forceinline__ int edge_row(const FluxParams &p, int r)
 // One workgroup per (tile of kind KIND, group of MC consecutive
 // member*class slabs).  Consecutive tiles run down y, so the 6-row halos of
 // neighbours meet in L2.
-template <int RY, int NW, int MC, bool ODD, int KIND, bool WET>
+template <int RY, int NW, int MC, bool ODD, int KIND, bool WET, bool SCL = false>
 __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
 {
     const int b = blockIdx.x;
@@ -898,16 +918,16 @@ __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
     for (int k = threadIdx.x; k <= p.g.nz + 1; k += NW * 64) zt[k] = p.zl[k];
     __syncthreads();
     if (KIND == K_INTERIOR)
-        flux_tile<RY, NW, MC, ODD, false, false, WET>(p, qv, gi0, gj0, sm, zt);
+        flux_tile<RY, NW, MC, ODD, false, false, WET, SCL>(p, qv, gi0, gj0, sm, zt);
     else if (kind == 1)
-        flux_tile<RY, NW, MC, ODD, true, false, WET>(p, qv, gi0, gj0, sm, zt);
+        flux_tile<RY, NW, MC, ODD, true, false, WET, SCL>(p, qv, gi0, gj0, sm, zt);
     else if (kind == 2)
-        flux_tile<RY, NW, MC, ODD, false, true, WET>(p, qv, gi0, gj0, sm, zt);
+        flux_tile<RY, NW, MC, ODD, false, true, WET, SCL>(p, qv, gi0, gj0, sm, zt);
     else
-        flux_tile<RY, NW, MC, ODD, true, true, WET>(p, qv, gi0, gj0, sm, zt);
+        flux_tile<RY, NW, MC, ODD, true, true, WET, SCL>(p, qv, gi0, gj0, sm, zt);
 }
 
-template <int RY, int NW, int MC, int KIND, bool WET>
+template <int RY, int NW, int MC, int KIND, bool WET, bool SCL = false>
 void launch_part_t(hipStream_t s, FluxParams p, int ntiles, bool odd)
 {
     if (ntiles <= 0) return;
@@ -916,17 +936,23 @@ void launch_part_t(hipStream_t s, FluxParams p, int ntiles, bool odd)
     p.per_xcd = (p.nlogical + kXCD - 1) / kXCD;
     const unsigned nb = (unsigned)(p.per_xcd * kXCD);
     if (odd)
-        hipLaunchKernelGGL((k_flux<RY, NW, MC, true, KIND, WET>), dim3(nb), dim3(NW * 64), 0, s, p);
+        hipLaunchKernelGGL((k_flux<RY, NW, MC, true, KIND, WET, SCL>), dim3(nb), dim3(NW * 64), 0, s, p);
     else
-        hipLaunchKernelGGL((k_flux<RY, NW, MC, false, KIND, WET>), dim3(nb), dim3(NW * 64), 0, s, p);
+        hipLaunchKernelGGL((k_flux<RY, NW, MC, false, KIND, WET, SCL>), dim3(nb), dim3(NW * 64), 0, s, p);
 }
 
-// the wet-deposition instantiation only when the step has precipitation
+// the wet-deposition instantiation only when the step has precipitation;
+// dry ensembles of an even member count in the default 58x18, two-slab
+// configuration take the shared-class instantiation (same arithmetic, the
+// class's loads and factors shared by the pair)
 template <int RY, int NW, int MC, int KIND>
 void launch_part(hipStream_t s, const FluxParams &p, int ntiles, bool odd, bool wet)
 {
+    constexpr bool kShared = RY == 24 && NW == 12 && MC == 2;
     if (wet)
         launch_part_t<RY, NW, MC, KIND, true>(s, p, ntiles, odd);
+    else if (kShared && p.g.nm > 1 && p.g.nm % 2 == 0)
+        launch_part_t<RY, NW, MC, KIND, false, kShared>(s, p, ntiles, odd);
     else
         launch_part_t<RY, NW, MC, KIND, false>(s, p, ntiles, odd);
 }
