@@ -1102,8 +1102,8 @@ constexpr int kListCtrs = kXCD;
 constexpr int kBatchCounters = 4 * kListCtrs;
 
 // Persistent form of k_flux over a work list: item = (scenario, bx, by,
-// slab group).  The list is cut into kXCD contiguous sublists of about equal
-// cost, one per XCD (workgroup b of a launch runs on XCD b % kXCD): a
+// slab group).  The list is dealt out in chunks to kXCD sublists, one per
+// XCD (workgroup b of a launch runs on XCD b % kXCD): a
 // workgroup draws items from its own XCD's sublist through that sublist's
 // atomic counter, so the slab groups of one tile, and neighbouring tiles,
 // share that XCD's L2; once its sublist is exhausted it draws from the
@@ -1265,22 +1265,24 @@ void scenario_items(int sc, const Grid &g, int RY, int mc, int mce, std::vector<
         }
 }
 
-// Cut list [o, o + n) of all into kXCD contiguous sublists of about equal
-// cost (an item costs its scenario's nz + 2 levels); x receives the kXCD + 1
-// absolute offsets.
-static void xcd_split(const std::vector<int4> &all, int o, int n, const std::vector<StepArgs> &ae,
-                      int *x)
+// Deal list [o, o + n) of all out to kXCD sublists, in place: the list is
+// cut into chunks of about kChunk items (a chunk keeps the slab groups of a
+// tile, and neighbouring tiles, together) and chunk t goes to sublist
+// t % kXCD, so every sublist holds the same mix of deep and shallow items
+// (the list is sorted deepest first) and ends on shallow ones.  x receives
+// the kXCD + 1 absolute offsets of the sublists.
+static void xcd_deal(std::vector<int4> &all, int o, int n, int *x)
 {
-    double total = 0.0;
-    for (int k = o; k < o + n; ++k) total += ae[all[k].x].g.nz + 2;
-    x[0] = o;
-    double acc = 0.0;
-    int d = 1;
-    for (int k = o; k < o + n && d < kXCD; ++k) {
-        acc += ae[all[k].x].g.nz + 2;
-        while (d < kXCD && acc >= total * d / kXCD) x[d++] = k + 1;
+    constexpr int kChunk = 32;
+    std::vector<int4> part[kXCD];
+    for (int k = 0; k < n; ++k) part[(k / kChunk) % kXCD].push_back(all[o + k]);
+    int at = o;
+    for (int d = 0; d < kXCD; ++d) {
+        x[d] = at;
+        std::copy(part[d].begin(), part[d].end(), all.begin() + at);
+        at += (int)part[d].size();
     }
-    while (d <= kXCD) x[d++] = o + n;
+    x[kXCD] = at;
 }
 
 template <int RY, int NW, int MC>
@@ -1320,8 +1322,8 @@ int batch_plan_t(FluxBatchPlan &b, const std::vector<StepArgs> &ae, const std::v
         all.insert(all.end(), iint[gr].begin(), iint[gr].end());
         b.off_edge[gr] = (int)all.size();
         all.insert(all.end(), iedge[gr].begin(), iedge[gr].end());
-        xcd_split(all, b.off_int[gr], b.nint[gr], ae, xo + (2 * gr) * (kXCD + 1));
-        xcd_split(all, b.off_edge[gr], b.nedge[gr], ae, xo + (2 * gr + 1) * (kXCD + 1));
+        xcd_deal(all, b.off_int[gr], b.nint[gr], xo + (2 * gr) * (kXCD + 1));
+        xcd_deal(all, b.off_edge[gr], b.nedge[gr], xo + (2 * gr + 1) * (kXCD + 1));
     }
     if (b.par[0].alloc(sizeof(FluxParams) * n, nullptr) || b.par[1].alloc(sizeof(FluxParams) * n, nullptr) ||
         b.items.alloc(sizeof(int4) * std::max<size_t>(1, all.size()), nullptr) ||

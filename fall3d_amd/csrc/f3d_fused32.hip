@@ -40,10 +40,13 @@ __device__ __forceinline__ void outflux_block(const FrameArgs &a, int bx, int nb
             const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
             const long o = (long)(k - 1) * ny + (j - 1);
             const float gl = gb[o], gr = gb[gf + o];
+            // a zero ghost (inflow, or no ash) adds nothing: its wind is
+            // not read
+            if (gl == 0.f && gr == 0.f) continue;
             const double faty = (j == 1 || j == ny) ? 0.5 : 1.0;
             const double area = faty * g.dy * 0.5 * (dz[k] + dz[k - 1]);
-            acc += -(dt * area * vx[midx(nx, ny, 1, j, k)] * (double)gl);
-            acc += dt * area * vx[midx(nx, ny, nx, j, k)] * (double)gr;
+            if (gl != 0.f) acc += -(dt * area * vx[midx(nx, ny, 1, j, k)] * (double)gl);
+            if (gr != 0.f) acc += dt * area * vx[midx(nx, ny, nx, j, k)] * (double)gr;
             continue;
         }
         r -= nxf;
@@ -60,8 +63,13 @@ __device__ __forceinline__ void outflux_block(const FrameArgs &a, int bx, int nb
         }
         r -= nyf;
         const int i = (int)(r % nx) + 1, j = (int)(r / nx) + 1;
+        // the top ghost is zero wherever vz+vset <= 0 at the top (tfac = 0;
+        // settling columns): nothing more is read there
+        const float tf = a.tfac[(long)q * nxy + r];
+        if (tf == 0.f) continue;
         const float *ci = a.cin + ((long)mem * g.nc + q) * g.G;
-        const float gt = fmaxf(0.f, a.tfac[(long)q * nxy + r] * ci[cidx(nx, ny, i, j, nz)]);
+        const float gt = fmaxf(0.f, tf * ci[cidx(nx, ny, i, j, nz)]);
+        if (gt == 0.f) continue;
         const double fatx = (i == 1 || i == nx) ? 0.5 : 1.0;
         const double faty = (j == 1 || j == ny) ? 0.5 : 1.0;
         const double area = fatx * faty * g.dx * g.dy;
