@@ -531,8 +531,13 @@ def bench_ensemble(args, nsteps=10):
     e0 = _errors(g0, _oracle_run(mem[0], nsteps + 2))
     n64, t64 = _fp64_rate(mem[0], 2)
     fp64 = n64 / t64
+    # the same members as separate single-member contexts, each with its own
+    # copy of the meteorology, stepped together by one batch with the same
+    # tile: what sharing the meteorology across the members is worth
+    sep = _separate_members_rate(mem, (tx, ty, slabs), nsteps)
     log(f"ensemble: {len(mem)} members, {el * 1e3 / nsteps:.2f} ms/step, {value:.3e} updates/s, "
-        f"tile {tx}x{ty}/{slabs}; member 0 load L1 {e0['load_rel_l1']:.2e}")
+        f"tile {tx}x{ty}/{slabs}; as separate contexts {sep['ms_per_step']:.2f} ms/step; "
+        f"member 0 load L1 {e0['load_rel_l1']:.2e}")
     return {"workload": f"{len(mem)} members x {base.nx}x{base.ny}x{base.nz} x {base.nc} classes",
             "autotuned_tile": f"{tx}x{ty} slabs {slabs}", "updates_per_s": value,
             "ms_per_step": el * 1e3 / nsteps,
@@ -540,7 +545,36 @@ def bench_ensemble(args, nsteps=10):
                          "frac": ach / HBM_PEAK_GBS, "bytes_per_update": b_alg,
                          "kernel_avg_ms": kms, "kernel": "k_flux over all member*class slabs"},
             "fp64_plain_updates_per_s": fp64, "speedup_vs_fp64_plain": value / fp64,
+            "separate_members": sep, "shared_meteo_speedup": value / sep["updates_per_s"],
             "errors": {"member": 0, "steps": nsteps + 2, **e0}}
+
+
+def _separate_members_rate(mem, tile, nsteps):
+    """The ensemble's members as len(mem) single-member FP32 contexts, each
+    with its own meteo read, batched with the given tile (graph replay)."""
+    from fall3d_amd import lib as F
+    runs = [F.Run(m, precision=F.FP32) for m in mem]
+    b = None
+    try:
+        for r in runs:
+            r.meteo_update()
+        b = F.Batch([r.ctx for r in runs])
+        b.set_tile(tile[0], tile[1], tile[2])
+        b.step(2)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(nsteps)
+        b.sync()
+        el = time.perf_counter() - t0
+    finally:
+        if b is not None:
+            b.close()
+        for r in runs:
+            r.close()
+    upd = sum(m.nx * m.ny * m.nz * m.nc for m in mem)
+    return {"contexts": len(mem), "updates_per_s": upd * nsteps / el,
+            "ms_per_step": el * 1e3 / nsteps,
+            "form": "one batch of single-member contexts (per-member meteorology), same tile"}
 
 
 # ---------------- serving ----------------
