@@ -121,6 +121,7 @@ bool check_ctx(const f3d_ctx *c)
         set_error("null context");
         return false;
     }
+    ++c->api_seq;
     return true;
 }
 

@@ -244,6 +244,11 @@ struct f3d_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;                     // edge tiles of the flux kernel
     unsigned gen = 0;  // bumped when fields or sources change (batched steps re-plan)
+    // bumped by every API entry on the context (check_ctx); a batch orders
+    // its stream after the context's streams again only when this moved, or
+    // another batch stepped the context, since it last did
+    mutable unsigned long api_seq = 0;
+    const void *last_batch = nullptr;  // the batch whose work the streams last waited for
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     long long bytes = 0;
 

@@ -228,3 +228,35 @@ def test_context_calls_are_ordered_after_batched_steps():
         for key in ("massvol", "massdep", "outmass", "erumass"):
             assert b3[key] == bud[k][key], (k, key)
             assert b5[key] == bud2[k][key], (k, key)
+
+
+def test_batches_sharing_a_context_stay_ordered():
+    # back-to-back f3d_batch_step calls with no call on the contexts in
+    # between skip re-ordering a batch after its members' streams; a context
+    # that another batch stepped in the meantime is still waited for
+    scs = _mixture(3, seed=71)
+    a = _runs(scs, None)
+    try:
+        b1 = F.Batch([a[0].ctx, a[1].ctx])
+        b2 = F.Batch([a[1].ctx, a[2].ctx])
+        b1.step(2)
+        b1.step(1)
+        b2.step(3)
+        b1.step(1)
+        b1.step(2)
+        b1.sync()
+        b2.sync()
+        st = [r.state() for r in a]
+        b1.close()
+        b2.close()
+    finally:
+        _close(a)
+    for k, n in ((0, 6), (1, 9), (2, 3)):
+        alone = _runs([scs[k]], None)[0]
+        try:
+            alone.step(n)
+            c, cdry = alone.state()
+        finally:
+            alone.close()
+        assert np.array_equal(c, st[k][0]), k
+        assert np.array_equal(cdry[:, :, 1:], st[k][1][:, :, 1:]), k
