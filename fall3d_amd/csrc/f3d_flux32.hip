@@ -86,6 +86,10 @@ struct FluxParams {
     int bx0, bx1, by0, by1;  // the rectangle of interior tiles
     int nlogical;
     int per_xcd;
+    // K_ALL launches: the interior part's logical blocks and their count per
+    // XCD (nlogical and per_xcd then count the edge part)
+    int nlogical_i;
+    int per_xcd_i;
 };
 
 // value held by lane i-1 (lane 0 reads 0)
@@ -355,7 +359,12 @@ struct FluxTile {
             const unsigned mt = ((unsigned)(cj - 1) * p.ftx + btx) * 64u + (unsigned)li;
             mt4[sl] = 4u * mt;
             mt8[sl] = 8u * mt;
-            mt12[sl] = 12u * mt;
+            // rows 0, 1, RY-2 and RY-1 reach the output rows (3 .. RY-4) only
+            // through the advection (c, vz+vset, ax, ay): their diffusion
+            // feeds rows 1 .. RY-2, whose y diffusion is never stored.  Their
+            // diffusion records all read the level's first record, so such a
+            // row fetches one line per level instead of six.
+            mt12[sl] = (lj[sl] < 2 || lj[sl] >= RY - 2) ? 0u : 12u * mt;
             unsigned f = 0;
             const bool inx = !EX || (gi >= 1 && gi <= nx);
             const bool iny = !EY || (gj >= 1 && gj <= ny);
@@ -854,11 +863,13 @@ __device__ __forceinline__ void flux_tile(const FluxParams &p, const int *qv, co
 
 // Tile kinds.  The rectangle [bx0, bx1] x [by0, by1] holds the interior
 // tiles (region inside the domain); a column outside [bx0, bx1] meets an x
-// face of the frame, a row outside [by0, by1] a y face.  Interior tiles are
-// one kernel; the x-side, y-side and corner tiles share a second one (one
-// launch, so the small kinds do not each wait for a launch's tail), which
-// picks the tile's code path per workgroup.
-enum : int { K_INTERIOR = 0, K_EDGE = 1 };
+// face of the frame, a row outside [by0, by1] a y face.  K_INTERIOR launches
+// the interior tiles; K_EDGE the x-side, y-side and corner tiles in one
+// launch (so the small kinds do not each wait for a launch's tail), picking
+// the tile's code path per workgroup; K_ALL both parts in one launch, each
+// XCD taking its share of the edge tiles first and then its share of the
+// interior tiles.
+enum : int { K_INTERIOR = 0, K_EDGE = 1, K_ALL = 2 };
 
 // the j-th edge column (outside [bx0, bx1]) / edge row (outside [by0, by1])
 __device__ __forceinline__ int edge_col(const FluxParams &p, int c)
@@ -877,8 +888,22 @@ template <int RY, int NW, int MC, bool ODD, int KIND, bool WET, bool SCL = false
 __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
 {
     const int b = blockIdx.x;
-    const int L = (b % kXCD) * p.per_xcd + b / kXCD;  // XCD-aware logical block
-    if (L >= p.nlogical) return;
+    int L;  // XCD-aware logical block of its part
+    bool edge = KIND == K_EDGE;
+    if (KIND == K_ALL) {
+        const int x = b % kXCD, i = b / kXCD;
+        edge = i < p.per_xcd;
+        if (edge) {
+            L = x * p.per_xcd + i;
+            if (L >= p.nlogical) return;
+        } else {
+            L = x * p.per_xcd_i + (i - p.per_xcd);
+            if (L >= p.nlogical_i) return;
+        }
+    } else {
+        L = (b % kXCD) * p.per_xcd + b / kXCD;
+        if (L >= p.nlogical) return;
+    }
     const int nq = p.g.nc * p.g.nm;
     const int ngrp = (nq + MC - 1) / MC;
     const int grp = L % ngrp;
@@ -891,7 +916,7 @@ __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
     const int nmx = p.bx1 - p.bx0 + 1;
     const int nex = p.ntx - nmx;
     int bx, by, kind = 0;  // 0 interior, 1 x side, 2 y side, 3 corner
-    if (KIND == K_INTERIOR) {
+    if (!edge) {
         by = p.by0 + t % nmy;
         bx = p.bx0 + t / nmy;
     } else {
@@ -913,11 +938,11 @@ __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
         }
     }
     const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * (RY - 6) - 3;
-    __shared__ float sm[flux_planes(MC, ODD) * RX * RY + (KIND == K_EDGE ? flux_xstage(RY, MC) : 0)];
+    __shared__ float sm[flux_planes(MC, ODD) * RX * RY + (KIND != K_INTERIOR ? flux_xstage(RY, MC) : 0)];
     __shared__ ZLev zt[kMaxLevels + 2];
     for (int k = threadIdx.x; k <= p.g.nz + 1; k += NW * 64) zt[k] = p.zl[k];
     __syncthreads();
-    if (KIND == K_INTERIOR)
+    if (KIND == K_INTERIOR || (KIND == K_ALL && !edge))
         flux_tile<RY, NW, MC, ODD, false, false, WET, SCL>(p, qv, gi0, gj0, sm, zt);
     else if (kind == 1)
         flux_tile<RY, NW, MC, ODD, true, false, WET, SCL>(p, qv, gi0, gj0, sm, zt);
@@ -941,6 +966,24 @@ void launch_part_t(hipStream_t s, FluxParams p, int ntiles, bool odd)
         hipLaunchKernelGGL((k_flux<RY, NW, MC, false, KIND, WET, SCL>), dim3(nb), dim3(NW * 64), 0, s, p);
 }
 
+// edge and interior tiles in one launch: the edge part's logical blocks
+// first on every XCD, then the interior part's
+template <int RY, int NW, int MC, bool WET, bool SCL = false>
+void launch_all_t(hipStream_t s, FluxParams p, int nedge, int nint, bool odd)
+{
+    if (nedge + nint <= 0) return;
+    const int ngrp = (p.g.nc * p.g.nm + MC - 1) / MC;
+    p.nlogical = nedge * ngrp;
+    p.per_xcd = (p.nlogical + kXCD - 1) / kXCD;
+    p.nlogical_i = nint * ngrp;
+    p.per_xcd_i = (p.nlogical_i + kXCD - 1) / kXCD;
+    const unsigned nb = (unsigned)((p.per_xcd + p.per_xcd_i) * kXCD);
+    if (odd)
+        hipLaunchKernelGGL((k_flux<RY, NW, MC, true, K_ALL, WET, SCL>), dim3(nb), dim3(NW * 64), 0, s, p);
+    else
+        hipLaunchKernelGGL((k_flux<RY, NW, MC, false, K_ALL, WET, SCL>), dim3(nb), dim3(NW * 64), 0, s, p);
+}
+
 // the wet-deposition instantiation only when the step has precipitation;
 // dry ensembles of an even member count in the default 58x18, two-slab
 // configuration take the shared-class instantiation (same arithmetic, the
@@ -955,6 +998,18 @@ void launch_part(hipStream_t s, const FluxParams &p, int ntiles, bool odd, bool 
         launch_part_t<RY, NW, MC, KIND, false, kShared>(s, p, ntiles, odd);
     else
         launch_part_t<RY, NW, MC, KIND, false>(s, p, ntiles, odd);
+}
+
+template <int RY, int NW, int MC>
+void launch_all(hipStream_t s, const FluxParams &p, int nedge, int nint, bool odd, bool wet)
+{
+    constexpr bool kShared = RY == 24 && NW == 12 && MC == 2;
+    if (wet)
+        launch_all_t<RY, NW, MC, true>(s, p, nedge, nint, odd);
+    else if (kShared && p.g.nm > 1 && p.g.nm % 2 == 0)
+        launch_all_t<RY, NW, MC, false, kShared>(s, p, nedge, nint, odd);
+    else
+        launch_all_t<RY, NW, MC, false>(s, p, nedge, nint, odd);
 }
 
 template <int RY, int NW, int MC>
@@ -983,22 +1038,30 @@ int launch_flux(hipStream_t s, const StepArgs &a, const FluxParams &p0, bool odd
     const int nex = p.ntx - nmx, ney = p.nty - nmy;
     const int nint = nmx * nmy;
     const int nedge = p.ntx * p.nty - nint;
-    // edge tiles on the side stream when there is one; 1024-lane groups take
-    // one slab per group there (two would exceed the register budget)
+    // groups of up to 768 lanes: edge and interior tiles in one launch.  Two
+    // launches on two streams ran 6-14% apart depending on which hardware
+    // queues the streams were given.  1024-lane groups take one slab per
+    // edge group (two would exceed the register budget), so their edge
+    // tiles go to the side stream, beside the interior launch.
     constexpr int MCE = (NW * 64 > 768) ? 1 : MC;
-    const bool fork = a.side && a.fork && a.join && nint > 0 && nedge > 0;
-    hipStream_t se = fork ? a.side : s;
-    if (fork) {
-        F3D_HIP(hipEventRecord(a.fork, s));
-        F3D_HIP(hipStreamWaitEvent(se, a.fork, 0));
-    }
-    launch_part<RY, NW, MCE, K_EDGE>(se, p, nedge, odd, a.wet != 0);
-    F3D_CHECK_LAUNCH();
-    launch_part<RY, NW, MC, K_INTERIOR>(s, p, nint, odd, a.wet != 0);
-    F3D_CHECK_LAUNCH();
-    if (fork) {
-        F3D_HIP(hipEventRecord(a.join, se));
-        F3D_HIP(hipStreamWaitEvent(s, a.join, 0));
+    if constexpr (MCE == MC) {
+        launch_all<RY, NW, MC>(s, p, nedge, nint, odd, a.wet != 0);
+        F3D_CHECK_LAUNCH();
+    } else {
+        const bool fork = a.side && a.fork && a.join && nint > 0 && nedge > 0;
+        hipStream_t se = fork ? a.side : s;
+        if (fork) {
+            F3D_HIP(hipEventRecord(a.fork, s));
+            F3D_HIP(hipStreamWaitEvent(se, a.fork, 0));
+        }
+        launch_part<RY, NW, MCE, K_EDGE>(se, p, nedge, odd, a.wet != 0);
+        F3D_CHECK_LAUNCH();
+        launch_part<RY, NW, MC, K_INTERIOR>(s, p, nint, odd, a.wet != 0);
+        F3D_CHECK_LAUNCH();
+        if (fork) {
+            F3D_HIP(hipEventRecord(a.join, se));
+            F3D_HIP(hipStreamWaitEvent(s, a.join, 0));
+        }
     }
     return F3D_OK;
 }
