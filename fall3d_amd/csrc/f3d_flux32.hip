@@ -399,6 +399,14 @@ struct FluxTile {
     }
 
     __device__ __forceinline__ bool has(int sl, unsigned b) const { return (fl[sl] & b) != 0u; }
+    // offset of the wind at a ghost's face cell (ax for x faces, ay for y
+    // faces) in the (ax, ay) records; only ghost lanes use it, so the others
+    // read past the buffer's range, which returns 0 without a memory request
+    // (an x-side row then fetches one line instead of four)
+    __device__ __forceinline__ unsigned ghost_wind(int sl) const
+    {
+        return has(sl, F_GHOST) ? mt8[sl] + (has(sl, F_GX) ? 0u : 4u) : kNoStore;
+    }
     __device__ __forceinline__ bool inner(int sl) const { return !EDGE || has(sl, F_INNER); }
     __device__ __forceinline__ float *row(float *plane, int c, int r) const
     {
@@ -418,7 +426,7 @@ struct FluxTile {
             D0[sl] = make_float3(0.f, 0.f, 0.f);
             h2[sl] = bld(rf, m4[sl], p.oH);
             if (WET) kwl[sl] = p.kwet[m4[sl] / 4u];
-            gv[sl] = EDGE ? bld(rf, mt8[sl] + (has(sl, F_GX) ? 0u : 4u), p.oA) : 0.f;
+            gv[sl] = EDGE ? bld(rf, ghost_wind(sl), p.oA) : 0.f;
             float bfv[MC], v1v[MC];
 #pragma unroll
             for (int c = 0; c < MC; ++c) {
@@ -531,7 +539,7 @@ struct FluxTile {
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
                 if (EDGE && (STEADY || s + 1 <= nzl))
-                    gv[sl] = bld(rf, mt8[sl] + (has(sl, F_GX) ? 0u : 4u), p.oA + 2u * lz);
+                    gv[sl] = bld(rf, ghost_wind(sl), p.oA + 2u * lz);
 #pragma unroll
                 for (int c = 0; c < MC; ++c) {
                     if (PIPE)
