@@ -138,21 +138,46 @@ __device__ __forceinline__ float adv_x(float a, float A, float Fc, bool inner, b
 
 // diffx on one region row.  EX: at the line ends the reference uses the
 // un-averaged products rho*K and one-sided fluxes clamped by sign
-// (diffx.f90:20-29, 45-53).
+// (diffx.f90:20-29, 45-53).  The per-row factors of a level are formed once
+// for the row's slabs (XDiff): the left face's factor Dl and clamp ml and the
+// right face's Dr and mr.  In the interior, Dl = D (the face-averaged
+// product) and Dr = D of the right neighbour, with open clamps (-inf, +inf),
+// so Dl*(q - ql) and Dr*(qr - q) are the same products the interior
+// formula forms (D*(q - ql) here and on the right neighbour); at the first
+// column Dl = dtdx2*pk with the flux clamped to inflow (max 0), Dr =
+// dtdx2*pk of the right neighbour; at the last, Dl = dtdx2*pk of the left
+// neighbour, Dr = dtdx2*pk with the flux clamped to outflow (min 0).  Lanes
+// outside the domain are not held: after diffx their values reach only
+// their own column, which is never stored.
+struct XDiff {
+    float Dl, Dr, ml, mr;
+};
 template <bool EX>
-__device__ __forceinline__ float diff_x(float d, float lr, float D, float pk, float dtdx2,
-                                        bool inner, bool lo, bool hi)
+__device__ __forceinline__ XDiff diff_x_factors(float D, float pk, float dtdx2, bool lo, bool hi)
+{
+    XDiff k;
+    k.Dl = D;
+    if (!EX) return k;
+    const float pkl = from_left(pk), pkr = from_right(pk);
+    k.Dl = lo ? dtdx2 * pk : (hi ? dtdx2 * pkl : D);
+    k.Dr = lo ? dtdx2 * pkr : (hi ? dtdx2 * pk : from_right(D));
+    k.ml = lo ? 0.f : -__builtin_inff();
+    k.mr = (hi && !lo) ? 0.f : __builtin_inff();
+    return k;
+}
+template <bool EX>
+__device__ __forceinline__ float diff_x(float d, float lr, const XDiff &k)
 {
     const float q = d * lr;
     const float ql = from_left(q);
-    const float G = D * (q - ql);
-    const float Gr = from_right(G);
-    if (!EX) return d + (Gr - G);
+    if (!EX) {
+        const float G = k.Dl * (q - ql);
+        return d + (from_right(G) - G);
+    }
     const float qr = from_right(q);
-    const float pkl = from_left(pk), pkr = from_right(pk);
-    const float gl = lo ? fmaxf(0.f, dtdx2 * pk * (q - ql)) : (hi ? dtdx2 * pkl * (q - ql) : G);
-    const float gr = lo ? dtdx2 * pkr * (qr - q) : (hi ? fminf(0.f, dtdx2 * pk * (qr - q)) : Gr);
-    return inner ? d + (gr - gl) : d;
+    const float gl = fmaxf(k.Dl * (q - ql), k.ml);
+    const float gr = fminf(k.Dr * (qr - q), k.mr);
+    return d + (gr - gl);
 }
 
 // Buffer resources (gfx9 raw buffers: stride 0, range in bytes): loads and
@@ -626,7 +651,10 @@ struct FluxTile {
                 const float pp = (sl < NS - 1) ? pyk[sl + 1] : eyp;
                 const float Dl = hd2 * (pyk[sl] + pm);
                 const float Du = hd2 * (pp + pyk[sl]);
-                const float Dx = ODD ? 0.5f * p.dtdx2 * (epk[sl] + from_left(epk[sl])) : 0.f;
+                XDiff xk{};
+                if (ODD)
+                    xk = diff_x_factors<EX>(0.5f * p.dtdx2 * (epk[sl] + from_left(epk[sl])), epk[sl],
+                                            p.dtdx2, has(sl, F_XLO), has(sl, F_XHI));
 #pragma unroll
                 for (int c = 0; c < MC; ++c) {
                     // diffy (diffy.f90:30-44; line ends 20-29, 45-53)
@@ -641,10 +669,10 @@ struct FluxTile {
                         ghi = lo ? d2 * pp * (qp - q) : (hi ? fminf(0.f, d2 * pyk[sl] * (qp - q)) : ghi);
                     }
                     float out = dc[c][sl] + (ghi - glo);
-                    if (EDGE) out = inner(sl) ? out : dc[c][sl];
-                    if (ODD)
-                        out = diff_x<EX>(out, elr[sl], Dx, epk[sl], p.dtdx2, inner(sl),
-                                         has(sl, F_XLO), has(sl, F_XHI));
+                    // on odd steps diffx follows, and a ghost column must
+                    // still hold its value for the first and last column
+                    if (EX && ODD) out = inner(sl) ? out : dc[c][sl];
+                    if (ODD) out = diff_x<EX>(out, elr[sl], xk);
                     // sizloop.f90:65-67, wetdep.f90:28-50 (WET launches only):
                     // below the PBL height a fraction lambda of c goes to cwet,
                     // level by level; the output lane owns its column's slot
@@ -751,11 +779,10 @@ struct FluxTile {
             // take the raw product at the line ends
             const float pkx = Dk.z * h2[sl];
             if (!ODD) {
-                const float Dx = 0.5f * p.dtdx2 * (pkx + from_left(pkx));
+                const XDiff xk = diff_x_factors<EX>(0.5f * p.dtdx2 * (pkx + from_left(pkx)), pkx, p.dtdx2,
+                                                    has(sl, F_XLO), has(sl, F_XHI));
 #pragma unroll
-                for (int c = 0; c < MC; ++c)
-                    dc[c][sl] = diff_x<EX>(dc[c][sl], lr0, Dx, pkx, p.dtdx2, inner(sl), has(sl, F_XLO),
-                                           has(sl, F_XHI));
+                for (int c = 0; c < MC; ++c) dc[c][sl] = diff_x<EX>(dc[c][sl], lr0, xk);
             } else {
                 *row(SC, 0, lj[sl]) = pkx;
                 *row(SC, 1, lj[sl]) = lr0;
