@@ -301,6 +301,7 @@ struct FluxTile {
     unsigned co4[NS], so4[NS], m4[NS], mt4[NS], mt8[NS], mt12[NS], fl[NS];
     unsigned FTL;  // records per level of the tile-major layout
     int lj[NS];
+    int gjb;  // the region's first row in the grid (wave-uniform)
     // Class-independent coefficients: (ax, ay) of the current and the next
     // level, (lr, pz, pky) of the current level and the one below, and the
     // column's 1/Hm1^2, which turns pky = rho*rkhor into pkx = rho*rkh1.
@@ -341,6 +342,7 @@ struct FluxTile {
         SZ = (unsigned)((nx + 2) * (ny + 2));
         li = threadIdx.x & (RX - 1);
         w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        gjb = gj0;
         SP = sm;
         SF = sm + MC * PL;
         SQ = sm + 2 * MC * PL;
@@ -424,6 +426,11 @@ struct FluxTile {
     }
 
     __device__ __forceinline__ bool has(int sl, unsigned b) const { return (fl[sl] & b) != 0u; }
+    // the first / last domain row (y-side tiles): the same for every lane of
+    // a row, so the y line-end clamps branch on it instead of selecting per
+    // lane
+    __device__ __forceinline__ bool row_ylo(int sl) const { return EY && gjb + lj[sl] == 1; }
+    __device__ __forceinline__ bool row_yhi(int sl) const { return EY && gjb + lj[sl] == p.g.ny; }
     // offset of the wind at a ghost's face cell (ax for x faces, ay for y
     // faces) in the (ax, ay) records; only ghost lanes use it, so the others
     // read past the buffer's range, which returns 0 without a memory request
@@ -663,10 +670,12 @@ struct FluxTile {
                     const float q = qc[c][sl];
                     float glo = Dl * (q - qm);
                     float ghi = Du * (qp - q);
-                    if (EY) {
-                        const bool lo = has(sl, F_YLO), hi = has(sl, F_YHI);
-                        glo = lo ? fmaxf(0.f, d2 * pyk[sl] * (q - qm)) : (hi ? d2 * pm * (q - qm) : glo);
-                        ghi = lo ? d2 * pp * (qp - q) : (hi ? fminf(0.f, d2 * pyk[sl] * (qp - q)) : ghi);
+                    if (EY && row_ylo(sl)) {
+                        glo = fmaxf(0.f, d2 * pyk[sl] * (q - qm));
+                        ghi = d2 * pp * (qp - q);
+                    } else if (EY && row_yhi(sl)) {
+                        glo = d2 * pm * (q - qm);
+                        ghi = fminf(0.f, d2 * pyk[sl] * (qp - q));
                     }
                     float out = dc[c][sl] + (ghi - glo);
                     // on odd steps diffx follows, and a ghost column must
@@ -730,8 +739,8 @@ struct FluxTile {
                     float pl = (sl > 0) ? phi[c][sl - 1] : epl[c];
                     float pr = (sl < NS - 1) ? phi[c][sl + 1] : epr[c];
                     if (EY) {
-                        pl = has(sl, F_YLO) ? apl[c][sl] : pl;
-                        pr = has(sl, F_YHI) ? apr[c][sl] : pr;
+                        if (row_ylo(sl)) pl = apl[c][sl];
+                        if (row_yhi(sl)) pr = apr[c][sl];
                     }
                     bnew = (ay > 0.f) ? Pc + (pl - ph) : Pc + (ph - pr);
                     if (EDGE) bnew = inner(sl) ? bnew : Pc;
