@@ -146,9 +146,11 @@ __device__ __forceinline__ float adv_x(float a, float A, float Fc, bool inner, b
 // formula forms (D*(q - ql) here and on the right neighbour); at the first
 // column Dl = dtdx2*pk with the flux clamped to inflow (max 0), Dr =
 // dtdx2*pk of the right neighbour; at the last, Dl = dtdx2*pk of the left
-// neighbour, Dr = dtdx2*pk with the flux clamped to outflow (min 0).  Lanes
-// outside the domain are not held: after diffx their values reach only
-// their own column, which is never stored.
+// neighbour, Dr = dtdx2*pk with the flux clamped to outflow (min 0).  Cells
+// outside the domain keep their value (a y-ghost row's q feeds diffy of the
+// first domain row).  Every lane shift is taken before the per-lane
+// choices: a shift evaluated under a lane-dependent branch would read 0 from
+// the lanes the branch leaves out.
 struct XDiff {
     float Dl, Dr, ml, mr;
 };
@@ -158,15 +160,15 @@ __device__ __forceinline__ XDiff diff_x_factors(float D, float pk, float dtdx2, 
     XDiff k;
     k.Dl = D;
     if (!EX) return k;
-    const float pkl = from_left(pk), pkr = from_right(pk);
+    const float pkl = from_left(pk), pkr = from_right(pk), Dr = from_right(D);
     k.Dl = lo ? dtdx2 * pk : (hi ? dtdx2 * pkl : D);
-    k.Dr = lo ? dtdx2 * pkr : (hi ? dtdx2 * pk : from_right(D));
+    k.Dr = lo ? dtdx2 * pkr : (hi ? dtdx2 * pk : Dr);
     k.ml = lo ? 0.f : -__builtin_inff();
     k.mr = (hi && !lo) ? 0.f : __builtin_inff();
     return k;
 }
 template <bool EX>
-__device__ __forceinline__ float diff_x(float d, float lr, const XDiff &k)
+__device__ __forceinline__ float diff_x(float d, float lr, const XDiff &k, bool inner)
 {
     const float q = d * lr;
     const float ql = from_left(q);
@@ -177,7 +179,7 @@ __device__ __forceinline__ float diff_x(float d, float lr, const XDiff &k)
     const float qr = from_right(q);
     const float gl = fmaxf(k.Dl * (q - ql), k.ml);
     const float gr = fminf(k.Dr * (qr - q), k.mr);
-    return d + (gr - gl);
+    return inner ? d + (gr - gl) : d;
 }
 
 // Buffer resources (gfx9 raw buffers: stride 0, range in bytes): loads and
@@ -678,10 +680,11 @@ struct FluxTile {
                         ghi = fminf(0.f, d2 * pyk[sl] * (qp - q));
                     }
                     float out = dc[c][sl] + (ghi - glo);
-                    // on odd steps diffx follows, and a ghost column must
+                    // the last sweep's result is only stored, except on odd
+                    // steps where diffx follows: a ghost column must then
                     // still hold its value for the first and last column
                     if (EX && ODD) out = inner(sl) ? out : dc[c][sl];
-                    if (ODD) out = diff_x<EX>(out, elr[sl], xk);
+                    if (ODD) out = diff_x<EX>(out, elr[sl], xk, inner(sl));
                     // sizloop.f90:65-67, wetdep.f90:28-50 (WET launches only):
                     // below the PBL height a fraction lambda of c goes to cwet,
                     // level by level; the output lane owns its column's slot
@@ -791,7 +794,7 @@ struct FluxTile {
                 const XDiff xk = diff_x_factors<EX>(0.5f * p.dtdx2 * (pkx + from_left(pkx)), pkx, p.dtdx2,
                                                     has(sl, F_XLO), has(sl, F_XHI));
 #pragma unroll
-                for (int c = 0; c < MC; ++c) dc[c][sl] = diff_x<EX>(dc[c][sl], lr0, xk);
+                for (int c = 0; c < MC; ++c) dc[c][sl] = diff_x<EX>(dc[c][sl], lr0, xk, inner(sl));
             } else {
                 *row(SC, 0, lj[sl]) = pkx;
                 *row(SC, 1, lj[sl]) = lr0;
