@@ -63,11 +63,11 @@ struct Fields {
     // Courant numbers dt/dx*vx, dt/dy*vy (dt-dependent), see k_fields_face
     const float *ax, *ay;
     // Read by the flux kernel, in its tile-major layout: for each level,
-    // row j and tile column bx (58 columns, kFluxTileCols), the 64 columns
-    // of the tile's region 58*bx-2 .. 58*bx+61 (clamped to 1..nx) are 64
-    // consecutive records, so one region row is one aligned, whole-line
-    // load.  Record index ((k-1)*ny + j-1)*ftx*64 + bx*64 + lane; ftl =
-    // ny*ftx*64 records per level.
+    // row j and tile column bx (flux_tiles columns), the 64 columns of the
+    // tile's region 58*bx .. 58*bx+63 (clamped to 1..nx) are 64 consecutive
+    // records, so one region row is one aligned, whole-line load.  Record
+    // index ((k-1)*ny + j-1)*ftx*64 + bx*64 + lane; ftl = ny*ftx*64 records
+    // per level.
     const float2 *cA;  // (ax, ay)
     const float *cD;   // (1/rho, rho*rkver, rho*rkhor), 12-byte records
     const float *wzt;  // vz+vset per class, nz*ftl records per class
@@ -82,6 +82,19 @@ struct Fields {
 // output columns of one tile of the flux kernel (a 64-lane region row
 // minus the halo of 3 on either side)
 constexpr int kFluxTileCols = 58;
+
+// Tiles of the flux kernel across n cells, for regions of `region` cells
+// (64 columns, or RY rows) with `region - 6` output cells.  Tile b's region
+// starts at cell (region - 6)*b.  The first tile owns cells 1 .. region-4:
+// its halo on that side is the ghost at cell 0, in lane (row) 0.  The last
+// tile owns up to cell n, the ghost n+1 at most in its last lane (row).  An
+// inner tile owns lanes 3 .. region-4.  So t tiles cover (region-6)*t + 4
+// cells.
+__host__ __device__ inline int flux_tiles(int n, int region)
+{
+    const int own = region - 6;
+    return n <= region - 2 ? 1 : 1 + (n - (region - 2) + own - 1) / own;
+}
 
 // Build Fields from the FP64 coefficient arrays (meteo.f90:108-112 applied).
 int build_fields(hipStream_t s, const Grid &g, const double *vx, const double *vy,

@@ -240,6 +240,7 @@ enum : unsigned {
     F_GHOST = 32u,  // setbcc face ghost
     F_GX = 64u,     // ... on an x face (else a y face)
     F_GLO = 128u,   // ... on the low face (i = 0 or j = 0)
+    F_OWN = 256u,   // a domain cell of the tile's own columns and rows (stored)
 };
 
 // LDS planes of one tile: SP, SF, SQ (one per slab), SA, SY and, on odd
@@ -304,6 +305,10 @@ struct FluxTile {
     unsigned FTL;  // records per level of the tile-major layout
     int lj[NS];
     int gjb;  // the region's first row in the grid (wave-uniform)
+    // the row is one of the tile's own rows (rows 3 .. RY-4; from row 1 in
+    // the first tile row, whose row 0 is the ghost row 0; up to row ny in
+    // the last tile row): wave-uniform
+    bool orow[NS];
     // Class-independent coefficients: (ax, ay) of the current and the next
     // level, (lr, pz, pky) of the current level and the one below, and the
     // column's 1/Hm1^2, which turns pky = rho*rkhor into pkx = rho*rkh1.
@@ -375,7 +380,13 @@ struct FluxTile {
         }
         const int gi = gi0 + li;
         FTL = p.ftl;
-        const unsigned btx = (unsigned)((gi0 + 2) / TXR);  // the tile's column
+        const unsigned btx = (unsigned)(gi0 / TXR);  // the tile's column
+        // own lanes and rows: 3 .. R-4 of a region of R, widened to the ghost
+        // at the domain's ends (the first tile starts at the ghost column or
+        // row 0, the last reaches the ghost n+1 in its last lane or row)
+        const int oxl = gi0 <= 0 ? 1 : 3, oxh = (nx - gi0 <= RX - 2) ? nx - gi0 : RX - 4;
+        const int oyl = gj0 <= 0 ? 1 : 3, oyh = (ny - gj0 <= RY - 2) ? ny - gj0 : RY - 4;
+        const bool olane = li >= oxl && li <= oxh;
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
             lj[sl] = w * NS + sl;
@@ -388,12 +399,13 @@ struct FluxTile {
             const unsigned mt = ((unsigned)(cj - 1) * p.ftx + btx) * 64u + (unsigned)li;
             mt4[sl] = 4u * mt;
             mt8[sl] = 8u * mt;
-            // rows 0, 1, RY-2 and RY-1 reach the output rows (3 .. RY-4) only
+            orow[sl] = lj[sl] >= oyl && lj[sl] <= oyh;
+            // rows more than one away from the own rows reach them only
             // through the advection (c, vz+vset, ax, ay): their diffusion
-            // feeds rows 1 .. RY-2, whose y diffusion is never stored.  Their
-            // diffusion records all read the level's first record, so such a
-            // row fetches one line per level instead of six.
-            mt12[sl] = (lj[sl] < 2 || lj[sl] >= RY - 2) ? 0u : 12u * mt;
+            // feeds rows whose y diffusion is never stored.  Their diffusion
+            // records all read the level's first record, so such a row
+            // fetches one line per level instead of six.
+            mt12[sl] = (lj[sl] < oyl - 1 || lj[sl] > oyh + 1) ? 0u : 12u * mt;
             unsigned f = 0;
             const bool inx = !EX || (gi >= 1 && gi <= nx);
             const bool iny = !EY || (gj >= 1 && gj <= ny);
@@ -406,9 +418,10 @@ struct FluxTile {
             // range, y faces inside the x range
             if (EX && iny && (gi == 0 || gi == nx + 1)) f |= F_GHOST | F_GX | (gi == 0 ? F_GLO : 0u);
             if (EY && inx && (gj == 0 || gj == ny + 1)) f |= F_GHOST | (gj == 0 ? F_GLO : 0u);
+            if (olane && orow[sl] && (f & F_INNER)) f |= F_OWN;
             fl[sl] = f;
-            // output cells: the tile's columns, inside the domain
-            so4[sl] = (li >= 3 && li < RX - 3 && (f & F_INNER)) ? co4[sl] : kNoStore;
+            // output cells: the tile's own columns, inside the domain
+            so4[sl] = (olane && (f & F_INNER)) ? co4[sl] : kNoStore;
             // the row's place in the ghost buffer for each x face, where the
             // tile owns the face's ghost in this row: the column beside the
             // ghost is an output column and the row an output row
@@ -418,8 +431,8 @@ struct FluxTile {
                 for (int f = 0; f < 2; ++f) {
                     const int lg = (f == 0 ? 0 : nx + 1) - gi0;  // the ghost column's lane
                     const int la = lg + (f == 0 ? 1 : -1);
-                    const bool own = lg >= 0 && lg < RX && la >= 3 && la < RX - 3 &&
-                                     lj[sl] >= 3 && lj[sl] < RY - 3 && gj >= 1 && gj <= ny;
+                    const bool own = lg >= 0 && lg < RX && la >= oxl && la <= oxh && orow[sl] &&
+                                     gj >= 1 && gj <= ny;
                     so[f * RY + lj[sl]] = own ? 4u * ((unsigned)f * (unsigned)nz * (unsigned)ny + (unsigned)(gj - 1))
                                               : kNoStore;
                 }
@@ -654,7 +667,7 @@ struct FluxTile {
             const float d2 = p.dtdy2, hd2 = 0.5f * p.dtdy2;
 #pragma unroll
             for (int sl = 0; sl < NS; ++sl) {
-                if (lj[sl] < 3 || lj[sl] >= RY - 3) continue;  // wave-uniform
+                if (!orow[sl]) continue;  // wave-uniform
                 // face-averaged rho*rkhor times dt/dy^2 below and above the row
                 const float pm = (sl > 0) ? pyk[sl - 1] : eym;
                 const float pp = (sl < NS - 1) ? pyk[sl + 1] : eyp;
@@ -828,7 +841,7 @@ struct FluxTile {
     // holding the outputs costs registers they do not have.)
     __device__ __forceinline__ bool stored(int sl, int c) const
     {
-        return li >= 3 && li < RX - 3 && inner(sl) && (c == 0 || live[c]);
+        return has(sl, F_OWN) && (c == 0 || live[c]);
     }
     __device__ __forceinline__ void flush(const int s)
     {
@@ -836,20 +849,21 @@ struct FluxTile {
         const unsigned ko = 4u * SZ * (unsigned)(s - 2);
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl) {
-            if (lj[sl] < 3 || lj[sl] >= RY - 3) continue;  // wave-uniform
+            if (!orow[sl]) continue;  // wave-uniform
 #pragma unroll
             for (int c = 0; c < MC; ++c)
                 bst(sto[c][sl], rout[c], so4[sl] + ko);
         }
     }
 
-    // x-side tiles: the staged x-face ghosts of the tile's output rows,
+    // x-side tiles: the staged x-face ghosts of the tile's own rows,
     // clipped, to the ghost buffer ([face][level][row] per slab: each
-    // level's rows are one run of consecutive cells)
+    // level's rows are one run of consecutive cells; rows the tile does not
+    // own carry kNoStore)
     __device__ __forceinline__ void flush_xstage()
     {
         __syncthreads();
-        const int ny = p.g.ny, ty = RY - 6;
+        const int ny = p.g.ny;
         const unsigned *so = reinterpret_cast<const unsigned *>(SG) + 2 * RY * kMaxLevels * MC;
         // the ghost slabs of the group (a duplicate slab writes nothing)
         rsrc_t rgb[MC];
@@ -861,11 +875,11 @@ struct FluxTile {
         for (int f = 0; f < 2; ++f) {
 #pragma unroll
             for (int c = 0; c < MC; ++c) {
-                for (int t = threadIdx.x; t < nz * ty; t += NW * 64) {
-                    const int k = t / ty, r = t - k * ty;
+                for (int t = threadIdx.x; t < nz * RY; t += NW * 64) {
+                    const int k = t / RY, r = t - k * RY;
                     // kNoStore plus a level offset stays past the range
-                    const unsigned o = so[f * RY + 3 + r] + 4u * (unsigned)k * (unsigned)ny;
-                    const float v = SG[(f * RY + 3 + r) * (kMaxLevels * MC) + k * MC + c];
+                    const unsigned o = so[f * RY + r] + 4u * (unsigned)k * (unsigned)ny;
+                    const float v = SG[(f * RY + r) * (kMaxLevels * MC) + k * MC + c];
                     bst(fmaxf(0.f, v), rgb[c], o);
                 }
             }
@@ -984,7 +998,7 @@ __global__ void __launch_bounds__(NW * 64) k_flux(FluxParams p)
             bx = edge_col(p, e / nry);
         }
     }
-    const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * (RY - 6) - 3;
+    const int gi0 = bx * TXR, gj0 = by * (RY - 6);
     __shared__ float sm[flux_planes(MC, ODD) * RX * RY + (KIND != K_INTERIOR ? flux_xstage(RY, MC) : 0)];
     __shared__ ZLev zt[kMaxLevels + 2];
     for (int k = threadIdx.x; k <= p.g.nz + 1; k += NW * 64) zt[k] = p.zl[k];
@@ -1064,14 +1078,14 @@ int launch_flux(hipStream_t s, const StepArgs &a, const FluxParams &p0, bool odd
 {
     FluxParams p = p0;
     const int nx = p.g.nx, ny = p.g.ny, ty = RY - 6;
-    p.ntx = (nx + TXR - 1) / TXR;
-    p.nty = (ny + ty - 1) / ty;
-    // tile bx is interior when 1 <= gi0 and gi0 + 63 <= nx (gi0 = bx*58 - 2),
+    p.ntx = flux_tiles(nx, RX);
+    p.nty = flux_tiles(ny, RY);
+    // tile bx is interior when 1 <= gi0 and gi0 + 63 <= nx (gi0 = bx*58),
     // likewise in y with RY rows
     p.bx0 = 1;
-    p.bx1 = min(p.ntx - 1, (nx - (RX - 3)) >= 0 ? (nx - (RX - 3)) / TXR : -1);
+    p.bx1 = min(p.ntx - 1, (nx - (RX - 1)) >= 0 ? (nx - (RX - 1)) / TXR : -1);
     p.by0 = 1;
-    p.by1 = min(p.nty - 1, (ny - (RY - 3)) >= 0 ? (ny - (RY - 3)) / ty : -1);
+    p.by1 = min(p.nty - 1, (ny - (RY - 1)) >= 0 ? (ny - (RY - 1)) / ty : -1);
     // an axis without interior tiles: every column (row) is an edge one
     if (p.bx1 < p.bx0) {
         p.bx0 = p.ntx;
@@ -1165,9 +1179,9 @@ static int make_params(const StepArgs &a, const Fields &f, FluxParams &p)
     p.oW = off(f.wzt);
     p.ftx = (unsigned)f.ftx;
     p.ftl = (unsigned)f.ftl;
-    if (f.ftx != (a.g.nx + TXR - 1) / TXR) {
+    if (f.ftx != flux_tiles(a.g.nx, RX)) {
         set_error("flux kernel: field layout of %d tile columns, grid needs %d", f.ftx,
-                  (a.g.nx + TXR - 1) / TXR);
+                  flux_tiles(a.g.nx, RX));
         return F3D_EINVAL;
     }
     p.oBF = off(f.bfac);
@@ -1270,7 +1284,7 @@ __global__ void __launch_bounds__(NW * 64)
         int qv[MC];
 #pragma unroll
         for (int c = 0; c < MC; ++c) qv[c] = grp * MC + c;
-        const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * (RY - 6) - 3;
+        const int gi0 = bx * TXR, gj0 = by * (RY - 6);
         const bool ex = gi0 < 1 || gi0 + RX - 1 > p.g.nx;
         const bool ey = gj0 < 1 || gj0 + RY - 1 > p.g.ny;
         if (LIST == L_INTERIOR || (LIST == L_MIXED && !ex && !ey)) {
@@ -1363,11 +1377,11 @@ void scenario_items(int sc, const Grid &g, int RY, int mc, int mce, std::vector<
                     std::vector<int4> &iedge)
 {
     const int ty = RY - 6;
-    const int ntx = (g.nx + TXR - 1) / TXR, nty = (g.ny + ty - 1) / ty;
+    const int ntx = flux_tiles(g.nx, RX), nty = flux_tiles(g.ny, RY);
     const int nq = g.nc * g.nm;
     for (int bx = 0; bx < ntx; ++bx)
         for (int by = 0; by < nty; ++by) {
-            const int gi0 = 1 + bx * TXR - 3, gj0 = 1 + by * ty - 3;
+            const int gi0 = bx * TXR, gj0 = by * ty;
             const bool edge = gi0 < 1 || gi0 + RX - 1 > g.nx || gj0 < 1 || gj0 + RY - 1 > g.ny;
             const int m = edge ? mce : mc;
             for (int gr = 0; gr < (nq + m - 1) / m; ++gr)

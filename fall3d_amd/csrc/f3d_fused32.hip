@@ -223,7 +223,7 @@ __global__ void k_fields_pack(Grid g, Fields f, float2 *cA, float *cD, float *wz
     const long t = r >> 6;
     const int bx = (int)(t % f.ftx);
     const long jk = t / f.ftx;  // (j-1) + ny*(k-1)
-    const int ci = min(max(kFluxTileCols * bx - 2 + li, 1), g.nx);
+    const int ci = min(max(kFluxTileCols * bx + li, 1), g.nx);
     const long src = (ci - 1) + (long)g.nx * jk;
     cA[r] = make_float2(f.ax[src], f.ay[src]);
     cD[3 * r] = f.lr[src];
@@ -355,7 +355,7 @@ __global__ void k_from64(const double *a, float *b, long n)
 // records per level of the flux kernel's tile-major layout
 static size_t tiled_level(const Grid &g)
 {
-    return (size_t)g.ny * (size_t)((g.nx + kFluxTileCols - 1) / kFluxTileCols) * 64;
+    return (size_t)g.ny * (size_t)flux_tiles(g.nx, kFluxTileCols + 6) * 64;
 }
 
 // the flux kernel's tile-major groups lead the buffer ((ax, ay), the 12-byte
@@ -378,7 +378,7 @@ size_t fields_floats(const Grid &g)
 Fields fields_view(const Grid &g, const float *b0, const double *vsum1)
 {
     Fields f;
-    f.ftx = (g.nx + kFluxTileCols - 1) / kFluxTileCols;
+    f.ftx = flux_tiles(g.nx, kFluxTileCols + 6);
     f.ftl = (long)tiled_level(g);
     const size_t t = (size_t)f.ftl * (size_t)g.nz;
     f.cA = reinterpret_cast<const float2 *>(b0);

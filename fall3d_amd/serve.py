@@ -140,13 +140,15 @@ def estimate_fp32_bytes(nx, ny, nz, nc):
     the FP64 meteorology and coefficients (10 fields and vset), the 2-D
     fields and deposits, the two FP32 state buffers with their ghost frame,
     and the FP32 step fields (7 per-cell and nc per-class fields in the
-    tile-major layout of 64 records per 58 columns, the separate copies),
+    tile-major layout of 64 records per tile column, the separate copies),
     with 5% and 1 MiB of slack.  Used for a request shape not seen before;
     once a context of the shape exists its measured size replaces it."""
     n = nx * ny * nz
     nxy = nx * ny
     g = (nx + 2) * (ny + 2) * (nz + 2)
-    ftl = -(-nx // 58) * 64 * ny
+    # tile columns: 58 own columns each, 60 at either end of the domain
+    ftx = 1 if nx <= 62 else 1 + -(-(nx - 62) // 58)
+    ftl = ftx * 64 * ny
     b64 = 8 * (10 * n + n * nc + nxy * (5 + nc + 2 * (nc + 1) + nc))
     b32 = 4 * (2 * g * nc + (5 + nc) * ftl * nz + (9 + nc) * n + 2 * nc * nxy + nxy)
     return int(1.05 * (b64 + b32)) + (1 << 20)
