@@ -437,6 +437,36 @@ def test_fp32_tile_kinds_one_axis_without_interior_tiles(shape):
     assert m["c_l1"] <= 1e-4 and m["load_l1"] <= 1e-4, m
 
 
+@pytest.mark.parametrize("shape", [(61, 20), (62, 21), (63, 19), (120, 39), (121, 38), (179, 22)],
+                         ids=lambda s: f"{s[0]}x{s[1]}")
+def test_fp32_every_tile_shape_gives_the_same_bits(shape):
+    # sizes around the tiling's steps: the first and last tiles own their
+    # halo (t tiles cover 58t+4 columns and (RY-6)t+4 rows), so 62 columns
+    # fit one tile and 63 need two, 120 two and 121 three; every cell takes
+    # the same operands whichever tile, lane and row compute it, so every
+    # tile shape, slab count and wave count gives the same bits
+    nx, ny = shape
+    sc = S.config_small(nx=nx, ny=ny, nz=5, nc=3, seed=nx + ny, mean_u=-4.0, mean_v=3.0)
+    outs = {}
+    for tile in TILES:
+        r = F.Run(sc, precision=F.FP32, tile=tile)
+        try:
+            r.meteo_update()
+            r.step(7)
+            c, cdry = r.state()
+            outs[_tile_id(tile)] = (c, cdry, r.budget())
+        finally:
+            r.close()
+    ref_id = _tile_id(TILES[0])
+    c0, d0, b0 = outs[ref_id]
+    assert np.all(np.isfinite(c0)) and c0.max() > 0
+    for k, (c, d, b) in outs.items():
+        assert np.array_equal(c, c0), (k, np.argwhere(c != c0)[:5])
+        assert np.array_equal(d[:, :, 1:], d0[:, :, 1:]), k
+        for key in ("massvol", "massdep", "outmass"):
+            assert b[key] == b0[key], (k, key)
+
+
 @pytest.mark.parametrize("tile", [(58, 18, 0, 2), (58, 10, 0, 2), (58, 26, 0, 1), (58, 18, 8, 2)],
                          ids=_tile_id)
 def test_fp32_deterministic(tile):
