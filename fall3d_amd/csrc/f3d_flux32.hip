@@ -1236,6 +1236,23 @@ constexpr int kBatchCounters = 4 * kListCtrs;
 // interior items only, 1 edge items only, 2 both (the tile's path is picked
 // per item).
 enum : int { L_INTERIOR = 0, L_EDGE = 1, L_MIXED = 2 };
+// one batch item's tile code, by tile kind
+template <int RY, int NW, int MC, bool ODD, int LIST>
+__device__ __forceinline__ void batch_tile(const FluxParams &p, const int *qv, int gi0, int gj0, bool ex,
+                                           bool ey, float *sm, const ZLev *zt)
+{
+    if (LIST == L_INTERIOR || (LIST == L_MIXED && !ex && !ey)) {
+        flux_tile<RY, NW, MC, ODD, false, false>(p, qv, gi0, gj0, sm, zt);
+    } else {
+        if (ex && ey)
+            flux_tile<RY, NW, MC, ODD, true, true>(p, qv, gi0, gj0, sm, zt);
+        else if (ex)
+            flux_tile<RY, NW, MC, ODD, true, false>(p, qv, gi0, gj0, sm, zt);
+        else
+            flux_tile<RY, NW, MC, ODD, false, true>(p, qv, gi0, gj0, sm, zt);
+    }
+}
+
 template <int RY, int NW, int MC, bool ODD, int LIST>
 __global__ void __launch_bounds__(NW * 64)
     k_flux_batch(const FluxParams *scen, const int4 *items, const int *xoff, unsigned *counter)
@@ -1287,16 +1304,13 @@ __global__ void __launch_bounds__(NW * 64)
         const int gi0 = bx * TXR, gj0 = by * (RY - 6);
         const bool ex = gi0 < 1 || gi0 + RX - 1 > p.g.nx;
         const bool ey = gj0 < 1 || gj0 + RY - 1 > p.g.ny;
-        if (LIST == L_INTERIOR || (LIST == L_MIXED && !ex && !ey)) {
-            flux_tile<RY, NW, MC, ODD, false, false>(p, qv, gi0, gj0, sm, zt);
-        } else {
-            if (ex && ey)
-                flux_tile<RY, NW, MC, ODD, true, true>(p, qv, gi0, gj0, sm, zt);
-            else if (ex)
-                flux_tile<RY, NW, MC, ODD, true, false>(p, qv, gi0, gj0, sm, zt);
-            else
-                flux_tile<RY, NW, MC, ODD, false, true>(p, qv, gi0, gj0, sm, zt);
-        }
+        // the last group of an odd slab count holds one slab: it runs the
+        // one-slab tile code (the same arithmetic per slab, about 0.63 of a
+        // two-slab group's time, where a dead second slab saves only 5%)
+        if (MC == 2 && grp * MC + 1 >= p.g.nc * p.g.nm)
+            batch_tile<RY, NW, 1, ODD, LIST>(p, qv, gi0, gj0, ex, ey, sm, zt);
+        else
+            batch_tile<RY, NW, MC, ODD, LIST>(p, qv, gi0, gj0, ex, ey, sm, zt);
         __syncthreads();  // LDS is reused by the next item
     }
 }
