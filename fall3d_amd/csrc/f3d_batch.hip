@@ -44,8 +44,8 @@ struct f3d_batch {
     bool use_graph = true;
     // outflux arguments of every context for a step in which context 0 takes
     // an even / odd step, and the (context, member) rows of the outflux launch
-    DeviceBuffer fdesc[2], frows;
-    int nfrows = 0;
+    DeviceBuffer fdesc[2], frows, frofs;
+    int nfrows = 0, nfblocks = 0;  // outflux rows and their workgroups in all
     // source arguments of every context (even / odd as above) and the first
     // item of each context in the one setsrc launch
     DeviceBuffer sdesc[2], sfirst;
@@ -183,8 +183,15 @@ int replan(f3d_batch *b)
     for (size_t k = 0; k < n; ++k)
         for (int m = 0; m < b->ctx[k]->g.nm; ++m) rows.push_back(make_int2((int)k, m));
     b->nfrows = (int)rows.size();
-    if (b->frows.alloc(sizeof(int2) * rows.size(), nullptr)) return F3D_ENOMEM;
+    std::vector<int> rofs(rows.size() + 1, 0);
+    for (size_t r = 0; r < rows.size(); ++r)
+        rofs[r + 1] = rofs[r] + f32::outflux_blocks(b->ctx[rows[r].x]->g);
+    b->nfblocks = rofs.back();
+    if (b->frows.alloc(sizeof(int2) * rows.size(), nullptr) ||
+        b->frofs.alloc(sizeof(int) * rofs.size(), nullptr))
+        return F3D_ENOMEM;
     F3D_HIP(hipMemcpy(b->frows.p, rows.data(), sizeof(int2) * rows.size(), hipMemcpyHostToDevice));
+    F3D_HIP(hipMemcpy(b->frofs.p, rofs.data(), sizeof(int) * rofs.size(), hipMemcpyHostToDevice));
     std::vector<int> first(n + 1, 0);
     for (size_t k = 0; k < n; ++k) first[k + 1] = first[k] + b->ctx[k]->nsrcgrp * b->ctx[k]->g.nc;
     b->nsrc = first[n];
@@ -260,7 +267,8 @@ int record_step(f3d_batch *b, bool odd0)
         b->ev_next += 1;
         if (b->ev_next == (int)b->ev_stop.size()) flush_timing(b);
     }
-    TRY(f32::outflux_batch(s, b->fdesc[o].as<f32::FrameArgs>(), b->frows.as<int2>(), b->nfrows));
+    TRY(f32::outflux_batch(s, b->fdesc[o].as<f32::FrameArgs>(), b->frows.as<int2>(), b->frofs.as<int>(),
+                           b->nfrows, b->nfblocks));
     return F3D_OK;
 }
 

@@ -197,13 +197,26 @@ struct FrameArgs {
     const float *tfac;  // top ghost factor per class (Fields::tfac)
     const double *vx, *vy, *vz, *vset, *dz;
     double dt;
-    double *part;     // kFramePartials per member
+    double *part;     // outflux_blocks(g) per member, at most kFramePartials
     double *outmass;  // per member
 };
+// Workgroups (and partial sums) of one member's outflux: at least 2048
+// face cells per workgroup of 256, at most red::kFramePartials.  A function
+// of the grid alone, so a member's budget is the same bits alone, in an
+// ensemble or in a batch.
+__host__ __device__ inline int outflux_blocks(const Grid &g)
+{
+    const long cells = ((long)g.ny * g.nz + (long)g.nx * g.nz + (long)g.nx * g.ny) * g.nc;
+    const long nb = (cells + 2047) / 2048;
+    return (int)(nb < 1 ? 1 : (nb > red::kFramePartials ? red::kFramePartials : nb));
+}
 FrameArgs frame_args(const StepArgs &a, const Fields &f, const double *vx64, const double *vy64,
                      const double *vz64, const double *vset64, const double *dz, double *scratch,
                      double *outmass);
-int outflux_batch(hipStream_t s, const FrameArgs *fa, const int2 *rows, int nrows);
+// rows: (context, member); rofs: nrows+1 prefix sums of the rows'
+// outflux_blocks, the last one the launch's workgroup count
+int outflux_batch(hipStream_t s, const FrameArgs *fa, const int2 *rows, const int *rofs, int nrows,
+                  int nblocks);
 // Completes the ghost frame of a state c the flux kernel wrote from cin
 // (which must still hold that step's input): the x-face ghosts from gbuf,
 // the rest of setbcc (setbcc.f90:19-195) formed from cin's faces, clipped

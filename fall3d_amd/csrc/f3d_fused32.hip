@@ -90,11 +90,21 @@ __global__ void __launch_bounds__(256) k_outflux(FrameArgs a)
     outflux_block(a, blockIdx.x, gridDim.x, blockIdx.y);  // y: member
 }
 
-// the outflux of many contexts: row y of the grid is (context, member)
-__global__ void __launch_bounds__(256) k_outflux_batch(const FrameArgs *fa, const int2 *rows)
+// the outflux of many contexts: workgroup b belongs to the (context,
+// member) row r with rofs[r] <= b < rofs[r+1], which holds that row's
+// outflux_blocks workgroups
+__global__ void __launch_bounds__(256) k_outflux_batch(const FrameArgs *fa, const int2 *rows,
+                                                       const int *rofs, int nrows)
 {
-    const int2 r = rows[blockIdx.y];
-    outflux_block(fa[r.x], blockIdx.x, gridDim.x, r.y);
+    const int b = blockIdx.x;
+    int lo = 0, hi = nrows;  // the last row whose first workgroup is <= b
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (rofs[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const int2 r = rows[lo];
+    outflux_block(fa[r.x], b - rofs[lo], rofs[lo + 1] - rofs[lo], r.y);
 }
 
 // Completes the ghost frame of a state c the flux kernel wrote from cin
@@ -176,8 +186,8 @@ __global__ void __launch_bounds__(1024) k_finish_batch(const FrameArgs *fa, cons
 {
     const int2 r = rows[blockIdx.x];
     const FrameArgs &a = fa[r.x];
-    red::block_sum_row(a.part + (long)r.y * red::kFramePartials, red::kFramePartials,
-                       a.outmass + r.y, 1);
+    const int nb = outflux_blocks(a.g);
+    red::block_sum_row(a.part + (long)r.y * nb, nb, a.outmass + r.y, 1);
 }
 
 // FP32 step fields from the FP64 coefficients
@@ -470,9 +480,10 @@ int outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx6
             double *scratch, double *outmass)
 {
     const FrameArgs fa = frame_args(a, f, vx64, vy64, vz64, vset64, dz, scratch, outmass);
-    hipLaunchKernelGGL(k_outflux, dim3(red::kFramePartials, a.g.nm), dim3(256), 0, s, fa);
+    const int nb = outflux_blocks(a.g);
+    hipLaunchKernelGGL(k_outflux, dim3(nb, a.g.nm), dim3(256), 0, s, fa);
     F3D_CHECK_LAUNCH();
-    return red::finish_rows(s, scratch, red::kFramePartials, a.g.nm, outmass, 1);
+    return red::finish_rows(s, scratch, nb, a.g.nm, outmass, 1);
 }
 
 int frame_fill(hipStream_t s, const FrameFill &a)
@@ -485,10 +496,11 @@ int frame_fill(hipStream_t s, const FrameFill &a)
     return F3D_OK;
 }
 
-int outflux_batch(hipStream_t s, const FrameArgs *fa, const int2 *rows, int nrows)
+int outflux_batch(hipStream_t s, const FrameArgs *fa, const int2 *rows, const int *rofs, int nrows,
+                  int nblocks)
 {
     if (nrows <= 0) return F3D_OK;
-    hipLaunchKernelGGL(k_outflux_batch, dim3(red::kFramePartials, nrows), dim3(256), 0, s, fa, rows);
+    hipLaunchKernelGGL(k_outflux_batch, dim3(nblocks), dim3(256), 0, s, fa, rows, rofs, nrows);
     F3D_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_finish_batch, dim3(nrows), dim3(1024), 0, s, fa, rows);
     F3D_CHECK_LAUNCH();
