@@ -190,8 +190,26 @@ int outflux(hipStream_t s, const StepArgs &a, const Fields &f, const double *vx6
 // the same for many contexts in two launches: fa[k] describes context k's
 // step, rows[y] = (context, member) for y < nrows; each member's partials
 // and tree are the ones outflux uses
+// n / d for n < 2^31 by a multiply-high: division by an invariant integer
+// (Granlund and Montgomery, PLDI 1994), with l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1, n / d = (umulhi(n, m) + n) >> l.
+struct FastDiv {
+    unsigned d, m, l;
+};
+inline FastDiv fast_div(unsigned d)
+{
+    unsigned l = 0;
+    while ((1ull << l) < d) ++l;
+    const unsigned long long m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+    return FastDiv{d, (unsigned)m, l};
+}
+__device__ __forceinline__ unsigned fdiv(unsigned n, const FastDiv &f)
+{
+    return (__umulhi(n, f.m) + n) >> f.l;
+}
 struct FrameArgs {
     Grid g;
+    FastDiv per, ny, nx;  // face cells per class (ny*nz + nx*nz + nx*ny), ny, nx
     const float *gbuf;  // the step's x-face ghosts (StepArgs::gbuf)
     const float *cin;   // its input, whose faces give the y-face and top ghosts
     const float *tfac;  // top ghost factor per class (Fields::tfac)

@@ -1167,6 +1167,12 @@ static int make_params(const StepArgs &a, const Fields &f, FluxParams &p)
         set_error("flux kernel: fields (%zu B) exceed 4 GiB or a class slab 2 GiB", f.bytes);
         return F3D_EINVAL;
     }
+    // the outflux indexes the face cells of all classes in 32 bits
+    if (((long)a.g.ny * a.g.nz + (long)a.g.nx * a.g.nz + a.g.NXY) * a.g.nc >= 1L << 31) {
+        set_error("outflux: %d classes of %dx%dx%d face cells exceed 2^31", a.g.nc, a.g.nx, a.g.ny,
+                  a.g.nz);
+        return F3D_EINVAL;
+    }
     if (a.g.nz > kMaxLevels) {
         set_error("flux kernel: nz=%d exceeds the supported %d levels", a.g.nz, kMaxLevels);
         return F3D_EINVAL;

@@ -27,19 +27,22 @@ __device__ __forceinline__ void outflux_block(const FrameArgs &a, int bx, int nb
     const double *vx = a.vx, *vy = a.vy, *vz = a.vz, *vset = a.vset, *dz = a.dz;
     const double dt = a.dt;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const long nxf = (long)ny * nz, nyf = (long)nx * nz, nxy = g.NXY;
-    const long per = nxf + nyf + nxy;
-    const long total = per * g.nc;
+    // face cells of a class: x faces (k, j), y faces (k, i), top (j, i); the
+    // flux kernel's parameters hold per * nc under 2^31, so the cell index
+    // and its quotients are 32-bit
+    const unsigned nxf = (unsigned)ny * nz, nyf = (unsigned)nx * nz, nxy = (unsigned)g.NXY;
+    const unsigned per = a.per.d, total = per * (unsigned)g.nc;
     const long gf = (long)nz * ny;
     double acc = 0.0;
-    for (long t = (long)bx * 256 + threadIdx.x; t < total; t += (long)nbx * 256) {
-        const int q = (int)(t / per);  // class
-        long r = t - (long)q * per;
-        const float *gb = a.gbuf + ((long)mem * g.nc + q) * ghost_floats(g);
+    for (unsigned t = (unsigned)bx * 256 + threadIdx.x; t < total; t += (unsigned)nbx * 256) {
+        const unsigned q = fdiv(t, a.per);  // class
+        unsigned r = t - q * per;
         if (r < nxf) {
-            const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
-            const long o = (long)(k - 1) * ny + (j - 1);
-            const float gl = gb[o], gr = gb[gf + o];
+            const unsigned k0 = fdiv(r, a.ny);
+            const int j = (int)(r - k0 * ny) + 1, k = (int)k0 + 1;
+            // the ghost buffer is [side][k][j]: cell r of the face
+            const float *gb = a.gbuf + ((long)mem * g.nc + q) * ghost_floats(g);
+            const float gl = gb[r], gr = gb[gf + r];
             // a zero ghost (inflow, or no ash) adds nothing: its wind is
             // not read
             if (gl == 0.f && gr == 0.f) continue;
@@ -51,7 +54,8 @@ __device__ __forceinline__ void outflux_block(const FrameArgs &a, int bx, int nb
         }
         r -= nxf;
         if (r < nyf) {
-            const int i = (int)(r % nx) + 1, k = (int)(r / nx) + 1;
+            const unsigned k0 = fdiv(r, a.nx);
+            const int i = (int)(r - k0 * nx) + 1, k = (int)k0 + 1;
             const float *ci = a.cin + ((long)mem * g.nc + q) * g.G;
             const float gd = (vy[midx(nx, ny, i, 1, k)] >= 0.0) ? 0.f : fmaxf(0.f, ci[cidx(nx, ny, i, 1, k)]);
             const float gu = (vy[midx(nx, ny, i, ny, k)] <= 0.0) ? 0.f : fmaxf(0.f, ci[cidx(nx, ny, i, ny, k)]);
@@ -62,11 +66,12 @@ __device__ __forceinline__ void outflux_block(const FrameArgs &a, int bx, int nb
             continue;
         }
         r -= nyf;
-        const int i = (int)(r % nx) + 1, j = (int)(r / nx) + 1;
         // the top ghost is zero wherever vz+vset <= 0 at the top (tfac = 0;
         // settling columns): nothing more is read there
         const float tf = a.tfac[(long)q * nxy + r];
         if (tf == 0.f) continue;
+        const unsigned j0 = fdiv(r, a.nx);
+        const int i = (int)(r - j0 * nx) + 1, j = (int)j0 + 1;
         const float *ci = a.cin + ((long)mem * g.nc + q) * g.G;
         const float gt = fmaxf(0.f, tf * ci[cidx(nx, ny, i, j, nz)]);
         if (gt == 0.f) continue;
@@ -461,6 +466,9 @@ FrameArgs frame_args(const StepArgs &a, const Fields &f, const double *vx64, con
 {
     FrameArgs fa;
     fa.g = a.g;
+    fa.per = fast_div((unsigned)((long)a.g.ny * a.g.nz + (long)a.g.nx * a.g.nz + a.g.NXY));
+    fa.ny = fast_div((unsigned)a.g.ny);
+    fa.nx = fast_div((unsigned)a.g.nx);
     fa.gbuf = a.gbuf;
     fa.cin = a.cin;
     fa.tfac = f.tfac;
