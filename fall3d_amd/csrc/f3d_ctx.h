@@ -219,17 +219,15 @@ struct FrameArgs {
     double *outmass;  // per member
 };
 // Workgroups (and partial sums) of one member's outflux, 256 lanes each:
-// red::kFramePartials when the member's faces give that many at least 2048
-// cells each (a launch that fills the device alone), else one per 8192 face
-// cells (fewer, fuller workgroups where many members share a launch).  A
-// function of the grid alone, so a member's budget is the same bits alone,
-// in an ensemble or in a batch.
+// one per 2048 face cells, at most red::kFramePartials, so that a launch of
+// one or a few members still spreads over the device.  A function of the
+// grid alone, so a member's budget is the same bits alone, in an ensemble or
+// in a batch.
 __host__ __device__ inline int outflux_blocks(const Grid &g)
 {
     const long cells = ((long)g.ny * g.nz + (long)g.nx * g.nz + (long)g.nx * g.ny) * g.nc;
-    if (cells >= (long)red::kFramePartials * 2048) return red::kFramePartials;
-    const long nb = (cells + 8191) / 8192;
-    return (int)(nb < 1 ? 1 : nb);
+    const long nb = (cells + 2047) / 2048;
+    return (int)(nb < 1 ? 1 : (nb > red::kFramePartials ? red::kFramePartials : nb));
 }
 FrameArgs frame_args(const StepArgs &a, const Fields &f, const double *vx64, const double *vy64,
                      const double *vz64, const double *vset64, const double *dz, double *scratch,
